@@ -1,0 +1,335 @@
+"""MI355X-native SALSA pairwise-distance path (all-pairs Smith-Waterman-Gotoh).
+
+Python bindings (ctypes) over the two native libraries built in-tree:
+
+* ``libswg.so``        -- the C ABI of ``include/swg.h`` (HIP kernels for gfx950)
+* ``libsalsa_host.so`` -- the C++ host mirror of ``include/salsa_host.h``
+  (block decomposition, matrix/block-file writers, scoring-matrix loader,
+  synthetic workloads)
+
+There is no CPU fallback: every compute entry point goes through the HIP
+library and fails loudly when it is missing or when no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MATRIX_DIR = os.path.join(HERE, "matrices")
+
+# return codes (include/swg.h)
+SWG_OK = 0
+SWG_E_ARGUMENT = -1
+SWG_E_SYMBOL = -2
+SWG_E_NONINTEGRAL = -3
+SWG_E_DEVICE = -4
+SWG_E_UNSUPPORTED = -5
+SWG_E_JC_UNDEFINED = -6
+SWG_E_CAPACITY = -7
+
+ALIGN_PROTEIN, ALIGN_NUCLEIC = 0, 1
+DIST_PERCENT_IDENTITY, DIST_KIMURA2, DIST_JUKES_CANTOR, DIST_MINMAX_NORM = 0, 1, 2, 3
+MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
+TDIST_INT16, TDIST_DOUBLE = 0, 1
+DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
+SCHED_PERSISTENT, SCHED_PLAIN = 0, 1
+MATRIX_SIZE = 127
+
+ABI_SYMBOLS = (
+    "swg_create", "swg_set_sequences", "swg_compute_block", "swg_compute_block_device",
+    "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
+)
+HOST_SYMBOLS = (
+    "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
+    "salsa_write_matrix", "salsa_write_swms_block", "salsa_load_scoring_matrix",
+    "salsa_synth_dna", "salsa_synth_protein",
+)
+
+
+class SwgError(RuntimeError):
+    """Error returned by libswg (code + swg_last_error message)."""
+
+    def __init__(self, code, msg):
+        super().__init__("swg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class SmithWatermanGotohException(SwgError):
+    """Raised where the reference raises SmithWatermanGotohException
+    (Salsa.CoreTPL/Bio/Algorithms/SmithWaterman/SmithWatermanGotohException.cs:30-47)."""
+
+
+class swg_params(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32), ("alignment_type", ctypes.c_int32), ("distance_fn", ctypes.c_int32),
+        ("tdistance", ctypes.c_int32), ("diag_mode", ctypes.c_int32), ("schedule", ctypes.c_int32),
+        ("gap_open", ctypes.c_float), ("gap_ext", ctypes.c_float),
+        ("scores", ctypes.POINTER(ctypes.c_float)), ("device", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+_libs = {}
+
+
+def _load(name):
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (run __graft_entry__.build())" % path)
+        _libs[name] = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    return _libs[name]
+
+
+def lib():
+    """The HIP library (C ABI of include/swg.h)."""
+    L = _load("libswg.so")
+    if not getattr(L, "_typed", False):
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.swg_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(swg_params)]
+        L.swg_set_sequences.argtypes = [vp, vp, vp, i32]
+        L.swg_compute_block.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp, ctypes.POINTER(i64)]
+        L.swg_compute_block_device.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp, ctypes.POINTER(i64)]
+        L.swg_synchronize.argtypes = [vp]
+        L.swg_set_schedule.argtypes = [vp, i32, i32]
+        L.swg_last_kernel_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(i32)]
+        L.swg_last_error.restype = ctypes.c_char_p
+        L.swg_destroy.argtypes = [vp]
+        L.swg_destroy.restype = None
+        L._typed = True
+    return L
+
+
+def host():
+    """The C++ host-mirror library (include/salsa_host.h)."""
+    lib()
+    H = _load("libsalsa_host.so")
+    if not getattr(H, "_typed", False):
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        H.salsa_range_partition.argtypes = [i32, i32, i32, vp, vp]
+        H.salsa_block_is_computed.argtypes = [i32, i32]
+        H.salsa_compute_rank.argtypes = [vp, i32, i32, i32, vp, i32, ctypes.POINTER(i64)]
+        H.salsa_compute_full_matrix.argtypes = [vp, i32, i32, vp, i32, ctypes.POINTER(i64)]
+        H.salsa_write_matrix.argtypes = [ctypes.c_char_p, vp, i64]
+        H.salsa_write_swms_block.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, ctypes.c_char_p, ctypes.c_char_p,
+                                             i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_char_p), vp, i64, i32]
+        H.salsa_load_scoring_matrix.argtypes = [ctypes.c_char_p, vp]
+        H.salsa_synth_dna.argtypes = [ctypes.c_uint64, i32, i32, ctypes.c_double, ctypes.c_double, i32, vp, vp, i64, vp]
+        H.salsa_synth_dna.restype = i64
+        H.salsa_synth_protein.argtypes = [ctypes.c_uint64, i32, i32, vp, vp, i64, vp]
+        H.salsa_synth_protein.restype = i64
+        H._typed = True
+    return H
+
+
+def _check(rc, exc=SwgError):
+    if rc != SWG_OK:
+        msg = lib().swg_last_error()
+        raise exc(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------- scoring
+def load_scoring_matrix(name_or_path):
+    """ScoringMatrix.Load (Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:145-196):
+    127x127 float32 table, row = s1 symbol, unlisted symbols score 0."""
+    path = name_or_path if os.path.sep in name_or_path else os.path.join(MATRIX_DIR, name_or_path)
+    out = np.zeros((MATRIX_SIZE, MATRIX_SIZE), dtype=np.float32)
+    rc = host().salsa_load_scoring_matrix(path.encode(), _ptr(out))
+    if rc != SWG_OK:
+        raise SwgError(rc, "cannot load scoring matrix %s" % path)
+    return out
+
+
+# ---------------------------------------------------------------- sequences
+class SequenceSet:
+    """Concatenated upper-case residues + offsets (the reference's
+    IList<Sequence> _sequences, Salsa.SmithWatermanMPI/Program.cs:65)."""
+
+    def __init__(self, residues, offsets, labels=None):
+        self.residues = np.ascontiguousarray(residues, dtype=np.uint8)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.labels = labels
+
+    @classmethod
+    def from_strings(cls, seqs, labels=None):
+        bs = [s.encode("ascii") if isinstance(s, str) else bytes(s) for s in seqs]
+        offs = np.zeros(len(bs) + 1, dtype=np.int64)
+        offs[1:] = np.cumsum([len(b) for b in bs])
+        res = np.frombuffer(b"".join(bs), dtype=np.uint8) if bs and offs[-1] else np.zeros(0, np.uint8)
+        return cls(res.copy(), offs, labels)
+
+    def __len__(self):
+        return len(self.offsets) - 1
+
+    @property
+    def lengths(self):
+        return np.diff(self.offsets).astype(np.int32)
+
+    def seq(self, i):
+        return self.residues[self.offsets[i]:self.offsets[i + 1]].tobytes()
+
+    def subset(self, idx):
+        return SequenceSet.from_strings([self.seq(int(i)) for i in idx])
+
+
+def parse_fasta(text):
+    """FastaSequenceParser semantics (Salsa.CoreTPL/Bio/IO/Parser.cs:133-151):
+    label = text after '>' trimmed; residue lines trimmed, upper-cased and
+    concatenated."""
+    labels, seqs = [], []
+    cur = None
+    for raw in text.replace("\r\n", "\n").replace("\r", "\n").split("\n"):
+        line = raw.strip()
+        if raw.lstrip().startswith(">"):
+            labels.append(raw.lstrip()[1:].strip())
+            seqs.append([])
+            cur = seqs[-1]
+        elif line and cur is not None:
+            cur.append(line.upper())
+    return SequenceSet.from_strings(["".join(s) for s in seqs], labels)
+
+
+def synth_dna(seed, n, n_anc, sub_rate, indel_rate, len_kind, len_params):
+    H = host()
+    p = np.ascontiguousarray(len_params, dtype=np.float64)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    need = -H.salsa_synth_dna(seed, n, n_anc, sub_rate, indel_rate, len_kind, _ptr(p), None, 0, _ptr(offs))
+    res = np.zeros(max(need, 1), dtype=np.uint8)
+    got = H.salsa_synth_dna(seed, n, n_anc, sub_rate, indel_rate, len_kind, _ptr(p), _ptr(res), need, _ptr(offs))
+    assert got == need
+    return SequenceSet(res[:need], offs)
+
+
+def synth_protein(seed, n, len_kind, len_params):
+    H = host()
+    p = np.ascontiguousarray(len_params, dtype=np.float64)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    need = -H.salsa_synth_protein(seed, n, len_kind, _ptr(p), None, 0, _ptr(offs))
+    res = np.zeros(max(need, 1), dtype=np.uint8)
+    got = H.salsa_synth_protein(seed, n, len_kind, _ptr(p), _ptr(res), need, _ptr(offs))
+    assert got == need
+    return SequenceSet(res[:need], offs)
+
+
+# ---------------------------------------------------------------- partitioning
+def range_partition(length, nparts, start=0):
+    """RangePartitioner.Partition (Salsa.CoreTPL/RangePartitioner.cs:17-55): list of inclusive (start, end)."""
+    s = np.zeros(nparts, dtype=np.int32)
+    e = np.zeros(nparts, dtype=np.int32)
+    rc = host().salsa_range_partition(start, length, nparts, _ptr(s), _ptr(e))
+    if rc != SWG_OK:
+        raise SwgError(rc, "bad partition request")
+    return list(zip(s.tolist(), e.tolist()))
+
+
+def block_is_computed(rank, j):
+    """Checkerboard ownership (Salsa.SmithWatermanMPI/Program.cs:628-639)."""
+    return bool(host().salsa_block_is_computed(rank, j))
+
+
+# ---------------------------------------------------------------- aligner context
+class Aligner:
+    """One libswg context: the MI355X replacement of the reference's
+    SmithWatermanGotoh + ComputeDistance per block (ComputeBlock,
+    Salsa.SmithWatermanMPI/Program.cs:203-257)."""
+
+    def __init__(self, scores, gap_open, gap_ext, alignment_type=ALIGN_NUCLEIC,
+                 distance_fn=DIST_PERCENT_IDENTITY, tdistance=TDIST_INT16, diag_mode=DIAG_LOWER_MIRROR,
+                 schedule=SCHED_PERSISTENT, mode=MODE_SW_LOCAL, device=0):
+        self._scores = np.ascontiguousarray(scores, dtype=np.float32).reshape(MATRIX_SIZE, MATRIX_SIZE)
+        p = swg_params()
+        p.mode, p.alignment_type, p.distance_fn = mode, alignment_type, distance_fn
+        p.tdistance, p.diag_mode, p.schedule = tdistance, diag_mode, schedule
+        p.gap_open, p.gap_ext = float(gap_open), float(gap_ext)
+        p.scores = self._scores.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        p.device = device
+        self.tdistance = tdistance
+        self.dtype = np.float64 if tdistance == TDIST_DOUBLE else np.int16
+        h = ctypes.c_void_p()
+        _check(lib().swg_create(ctypes.byref(h), ctypes.byref(p)))
+        self._h = h
+        self.seqs = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_sequences(self, seqs):
+        self.seqs = seqs
+        _check(lib().swg_set_sequences(self._h, _ptr(seqs.residues), _ptr(seqs.offsets), len(seqs)))
+
+    def set_schedule(self, schedule, strip_rows=0):
+        _check(lib().swg_set_schedule(self._h, schedule, strip_rows))
+
+    def compute_block(self, row_start, row_end, col_start, col_end, is_diagonal, stats=False, out=None):
+        """ComputeBlock: returns (distances[rows, cols], stats or None, pairs)."""
+        rows, cols = row_end - row_start + 1, col_end - col_start + 1
+        if out is None:
+            out = np.zeros((max(rows, 0), max(cols, 0)), dtype=self.dtype)
+        st = np.zeros((max(rows, 0), max(cols, 0), 4), dtype=np.int32) if stats else None
+        done = ctypes.c_int64(0)
+        _check(lib().swg_compute_block(self._h, row_start, row_end, col_start, col_end, int(bool(is_diagonal)),
+                                       _ptr(out), out.shape[1] if out.ndim == 2 else cols,
+                                       _ptr(st) if stats else None, ctypes.byref(done)))
+        return out, st, done.value
+
+    def compute_block_device(self, row_start, row_end, col_start, col_end, is_diagonal, dev_out_ptr, ld,
+                             dev_stats_ptr=None):
+        done = ctypes.c_int64(0)
+        _check(lib().swg_compute_block_device(self._h, row_start, row_end, col_start, col_end,
+                                              int(bool(is_diagonal)), ctypes.c_void_p(dev_out_ptr), ld,
+                                              ctypes.c_void_p(dev_stats_ptr) if dev_stats_ptr else None,
+                                              ctypes.byref(done)))
+        return done.value
+
+    def synchronize(self):
+        _check(lib().swg_synchronize(self._h))
+
+    def last_kernel_stats(self):
+        ms, cells, strip = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        _check(lib().swg_last_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(cells), ctypes.byref(strip)))
+        return ms.value, cells.value, strip.value
+
+    def compute_full_matrix(self, world=1):
+        """SWMPI end to end on one process (every rank's Compute + Scatter):
+        the n x n matrix WriteFullMatrixOnRank0 writes."""
+        n = len(self.seqs)
+        full = np.zeros((n, n), dtype=self.dtype)
+        pairs = ctypes.c_int64(0)
+        rc = host().salsa_compute_full_matrix(self._h, n, world, _ptr(full), full.itemsize, ctypes.byref(pairs))
+        _check(rc)
+        return full, pairs.value
+
+    def compute_rank(self, world, rank):
+        n = len(self.seqs)
+        (rs, re) = range_partition(n, world)[rank]
+        part = np.zeros((re - rs + 1, n), dtype=self.dtype)
+        pairs = ctypes.c_int64(0)
+        _check(host().salsa_compute_rank(self._h, n, world, rank, _ptr(part), part.itemsize, ctypes.byref(pairs)))
+        return part, pairs.value
+
+
+def write_swms_block(path, project, rank, sysname, fasta, row_block, col_block, row_range, col_range, names, data):
+    arr = (ctypes.c_char_p * 4)(*[n.encode() for n in names])
+    data = np.ascontiguousarray(data)
+    rc = host().salsa_write_swms_block(path.encode(), project.encode(), rank, sysname.encode(), fasta.encode(),
+                                       row_block, col_block, row_range[0], row_range[1], col_range[0], col_range[1],
+                                       arr, _ptr(data), data.size, data.itemsize)
+    if rc != SWG_OK:
+        raise SwgError(rc, "cannot write %s" % path)

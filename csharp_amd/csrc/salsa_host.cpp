@@ -1,0 +1,228 @@
+// C++ restatement of the reference's host-side pieces of the distance path
+// (see include/salsa_host.h for the file:line of each).
+#include "salsa_host.h"
+
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+extern "C" {
+
+int salsa_range_partition(int32_t start, int32_t length, int32_t nparts, int32_t* starts, int32_t* ends)
+{
+    // RangePartitioner.Partition (Salsa.CoreTPL/RangePartitioner.cs:17-55)
+    if (nparts < 1 || !starts || !ends) return SWG_E_ARGUMENT;
+    const int32_t chunk = length / nparts;
+    int32_t rem = length % nparts;
+    for (int32_t i = 0; i < nparts; ++i) {
+        if (rem > 0) {
+            starts[i] = start; ends[i] = start + chunk;
+            start += chunk + 1;
+            --rem;
+        } else {
+            starts[i] = start; ends[i] = start + chunk - 1;
+            start += chunk;
+        }
+    }
+    return SWG_OK;
+}
+
+int salsa_block_is_computed(int32_t rank, int32_t j)
+{
+    // MPIComputeLowerTriangularBlocked.Compute (Salsa.SmithWatermanMPI/Program.cs:628-639)
+    if (rank == j) return 1;
+    const bool odd = ((rank + j) & 1) == 1;
+    if (rank > j && !odd) return 1;
+    if (rank < j && odd) return 1;
+    return 0;
+}
+
+}  // extern "C"
+
+// Computes the owned blocks of `rank` into its stripe (rows of the rank x n).
+static int compute_rank_into(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank, uint8_t* stripe,
+                             int64_t ld_elems, int elem, int64_t* pairs)
+{
+    std::vector<int32_t> rs(world), re(world);
+    int rc = salsa_range_partition(0, n, world, rs.data(), re.data());
+    if (rc) return rc;
+    int64_t total = 0;
+    for (int32_t j = 0; j < world; ++j) {
+        if (!salsa_block_is_computed(rank, j)) continue;
+        // the block's (row_start, col_start) element inside the stripe
+        uint8_t* base = stripe + ((int64_t)0 * ld_elems + rs[j]) * elem;
+        int64_t done = 0;
+        rc = swg_compute_block(ctx, rs[rank], re[rank], rs[j], re[j], rank == j ? 1 : 0, base, ld_elems, nullptr,
+                               &done);
+        if (rc) return rc;
+        total += done;
+    }
+    if (pairs) *pairs = total;
+    return SWG_OK;
+}
+
+extern "C" {
+
+int salsa_compute_rank(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank, void* partial, int32_t elem,
+                       int64_t* pairs)
+{
+    if (!ctx || !partial || world < 1 || rank < 0 || rank >= world || (elem != 2 && elem != 8))
+        return SWG_E_ARGUMENT;
+    return compute_rank_into(ctx, n, world, rank, static_cast<uint8_t*>(partial), n, elem, pairs);
+}
+
+int salsa_compute_full_matrix(swg_ctx* ctx, int32_t n, int32_t world, void* full, int32_t elem, int64_t* pairs)
+{
+    if (!ctx || !full || world < 1 || (elem != 2 && elem != 8)) return SWG_E_ARGUMENT;
+    std::vector<int32_t> rs(world), re(world);
+    int rc = salsa_range_partition(0, n, world, rs.data(), re.data());
+    if (rc) return rc;
+    uint8_t* F = static_cast<uint8_t*>(full);
+    std::memset(F, 0, (size_t)n * n * elem);
+    int64_t total = 0;
+    for (int32_t r = 0; r < world; ++r) {
+        int64_t done = 0;
+        rc = compute_rank_into(ctx, n, world, r, F + (int64_t)rs[r] * n * elem, n, elem, &done);
+        if (rc) return rc;
+        total += done;
+    }
+    // Scatter (Program.cs:645-677): every computed off-diagonal block (r,j)
+    // is transposed into block (j,r) of the owner of rows j
+    for (int32_t r = 0; r < world; ++r)
+        for (int32_t j = 0; j < world; ++j) {
+            if (r == j || !salsa_block_is_computed(r, j)) continue;
+            for (int32_t i = rs[r]; i <= re[r]; ++i)
+                for (int32_t k = rs[j]; k <= re[j]; ++k)
+                    std::memcpy(F + ((int64_t)k * n + i) * elem, F + ((int64_t)i * n + k) * elem, elem);
+        }
+    if (pairs) *pairs = total;
+    return SWG_OK;
+}
+
+int salsa_write_matrix(const char* path, const void* data, int64_t nbytes)
+{
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return SWG_E_ARGUMENT;
+    const size_t w = std::fwrite(data, 1, (size_t)nbytes, f);
+    std::fclose(f);
+    return w == (size_t)nbytes ? SWG_OK : SWG_E_ARGUMENT;
+}
+
+static void put_u16(std::vector<uint8_t>& b, uint32_t v) { b.push_back(v & 0xFF); b.push_back((v >> 8) & 0xFF); }
+static void put_u32(std::vector<uint8_t>& b, uint32_t v)
+{
+    for (int k = 0; k < 4; ++k) b.push_back((v >> (8 * k)) & 0xFF);
+}
+static void put_str16(std::vector<uint8_t>& b, const char* s)
+{
+    const size_t n = s ? std::strlen(s) : 0;
+    put_u16(b, (uint32_t)n);
+    for (size_t k = 0; k < n; ++k) b.push_back((uint8_t)s[k]);
+}
+
+int salsa_write_swms_block(const char* path, const char* project, int32_t rank, const char* sysname,
+                           const char* fasta, int32_t row_block, int32_t col_block,
+                           int32_t row_start, int32_t row_end, int32_t col_start, int32_t col_end,
+                           const char* const* names, const void* data, int64_t nelem, int32_t elem_size)
+{
+    // WriteRowBlockMetaData (Salsa.SmithWatermanMS/Program.cs:1116-1160)
+    std::vector<uint8_t> h;
+    put_str16(h, project);
+    put_u16(h, (uint32_t)rank);
+    put_str16(h, sysname);
+    put_str16(h, fasta);
+    put_u16(h, (uint32_t)row_block);
+    put_u16(h, (uint32_t)col_block);
+    put_u32(h, (uint32_t)row_start);
+    put_u32(h, (uint32_t)row_end);
+    put_u32(h, (uint32_t)col_start);
+    put_u32(h, (uint32_t)col_end);
+    for (int k = 0; k < 4; ++k) put_str16(h, names ? names[k] : "");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return SWG_E_ARGUMENT;
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    // WriteBlocks (:1102-1108): the block's values row-major
+    ok = ok && std::fwrite(data, (size_t)elem_size, (size_t)nelem, f) == (size_t)nelem;
+    std::fclose(f);
+    return ok ? SWG_OK : SWG_E_ARGUMENT;
+}
+
+// ScoringMatrixTokenizer with the default delimiters (whitespace)
+static std::vector<std::string> tokens(const std::string& line)
+{
+    std::vector<std::string> t;
+    std::string cur;
+    for (char ch : line) {
+        if (ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\f') {
+            if (!cur.empty()) { t.push_back(cur); cur.clear(); }
+        } else {
+            cur.push_back(ch);
+        }
+    }
+    if (!cur.empty()) t.push_back(cur);
+    return t;
+}
+
+static std::string trim(const std::string& s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) ++a;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    return s.substr(a, b - a);
+}
+
+int salsa_load_scoring_matrix(const char* path, float* scores)
+{
+    // ScoringMatrix.Load (Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:145-196)
+    if (!path || !scores) return SWG_E_ARGUMENT;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return SWG_E_ARGUMENT;
+    std::vector<std::string> lines;
+    {
+        std::string cur;
+        int ch;
+        while ((ch = std::fgetc(f)) != EOF) {
+            if (ch == '\n') { lines.push_back(cur); cur.clear(); }
+            else cur.push_back((char)ch);
+        }
+        if (!cur.empty()) lines.push_back(cur);
+        std::fclose(f);
+    }
+    const int SIZE = SWG_MATRIX_SIZE;
+    std::memset(scores, 0, sizeof(float) * SIZE * SIZE);
+    char acids[SWG_MATRIX_SIZE];
+    std::memset(acids, 0, sizeof(acids));
+    size_t li = 0;
+    // skip comment lines (:167)
+    while (li < lines.size()) {
+        const std::string t = trim(lines[li]);
+        if (t.empty() || t[0] != '#') break;
+        ++li;
+    }
+    if (li >= lines.size()) return SWG_E_ARGUMENT;
+    {
+        const std::vector<std::string> hdr = tokens(trim(lines[li++]));
+        if ((int)hdr.size() > SIZE) return SWG_E_ARGUMENT;
+        for (size_t j = 0; j < hdr.size(); ++j) acids[j] = hdr[j][0];
+    }
+    for (; li < lines.size(); ++li) {
+        const std::vector<std::string> tk = tokens(trim(lines[li]));
+        if (tk.empty()) return SWG_E_ARGUMENT;  // the reference throws on an empty line
+        const unsigned char acid = (unsigned char)tk[0][0];
+        if (acid >= SIZE) return SWG_E_SYMBOL;
+        size_t next = 1;
+        for (int i = 0; i < SIZE; ++i) {
+            if (acids[i] != 0) {
+                if (next >= tk.size()) return SWG_E_ARGUMENT;
+                const unsigned char col = (unsigned char)acids[i];
+                if (col >= SIZE) return SWG_E_SYMBOL;
+                scores[acid * SIZE + col] = std::strtof(tk[next++].c_str(), nullptr);
+            }
+        }
+    }
+    return SWG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Internal interface between the host scheduler (swg_api.hip) and the HIP
+// kernels (swg_kernels.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace swg {
+
+constexpr int kWave = 64;             // lanes per wavefront = pairs per task
+constexpr int kMaxCodes = 64;         // residue codes incl. the pad code
+constexpr int kNeg = -(1 << 30);      // integer stand-in for -Infinity
+constexpr int kPadScore = -(1 << 20); // score of the pad code: never on an optimal path
+
+// Per-block description, read by the kernels from device memory so that one
+// captured graph can be replayed for every block.
+struct BlockDesc {
+    int32_t ntasks;
+    int32_t mirror;        // 1: also write (b,a) (diagonal block, lower-mirror semantics)
+    int64_t r0, c0, ld;    // output position of pair (a,b): (a-r0)*ld + (b-c0)
+    void* out;             // TDistance array (int16 or double)
+    int32_t* stats;        // optional swg_pair_stats (4 x int32) at the same position
+    int64_t pad_;
+};
+
+// Read-only data of a context (sequence set + scoring).
+struct SeqSet {
+    const uint8_t* res;    // residue codes; each sequence starts 4-byte aligned
+    const int64_t* off;    // start of each sequence in res
+    const int32_t* len;
+    int64_t pad_off;       // offset of an all-pad run of length >= maxlen + 16
+    const int2* prof;      // [ncode][ncode]: (score + gap_open, carried-count increment)
+    int32_t ncode;         // includes the pad code (= ncode-1)
+    int32_t gap_open, gap_ext;
+    int32_t tdist;         // 0 int16, 1 double
+};
+
+// Task lists: task t aligns s1 = task_row[t] against the 64 sequences of
+// tile task_tile[t] (tile_col[tile*64 + lane], -1 = idle lane).
+struct TaskSet {
+    const int32_t* task_row;
+    const int32_t* task_tile;
+    const int32_t* tile_col;
+    const int32_t* tile_maxlen;   // max length over the tile's lanes
+    const BlockDesc* desc;
+    int32_t* queue;               // [0]: next task (persistent schedule)
+};
+
+struct Scratch {
+    int4* bnd;                    // per-slot row-boundary buffer
+    int64_t bnd_stride;           // in int4, per slot
+};
+
+// Launch the local-alignment kernel.  strip_rows in {8,16,32};
+// persistent: grid = nslots and tasks are drawn from queue[0];
+// otherwise grid = nslots with static round-robin over tasks.
+hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
+                     bool persistent, int nslots, hipStream_t stream);
+hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
+                     bool persistent, int nslots, hipStream_t stream);
+// Occupancy: resident workgroups (one wave each) per CU for a variant.
+int sw_blocks_per_cu(int strip_rows, bool nw);
+// Fill kernels (graph-capturable replacement for hipMemsetAsync).
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
+hipError_t launch_fill_bytes_zero(void* p, int64_t nbytes, hipStream_t stream);
+
+}  // namespace swg

@@ -1,0 +1,84 @@
+/*
+ * salsa_host.h -- C++ host layer above the swg.h boundary (libsalsa_host.so).
+ *
+ * The reference's host is C# (.NET 4.0) and no CLR exists on this system, so
+ * this library restates the host-side pieces of the path in C++ with the
+ * reference's names and semantics; a C# deployment keeps its own host and
+ * calls swg.h through P/Invoke instead.
+ *
+ *  - RangePartitioner / BlockPartitioner  (Salsa.CoreTPL/RangePartitioner.cs:17-55,
+ *                                          Salsa.CoreTPL/BlockPartitioner.cs:10-28)
+ *  - MPIComputeLowerTriangularBlocked.Compute + checkerboard ownership
+ *                                          (Salsa.SmithWatermanMPI/Program.cs:574-642)
+ *  - Scatter mirror and WriteFullMatrixOnRank0 layout
+ *                                          (Salsa.SmithWatermanMPI/Program.cs:645-677,786-888)
+ *  - SWMS block files                      (Salsa.SmithWatermanMS/Program.cs:1081-1160)
+ *  - ScoringMatrix.Load                    (Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:145-196)
+ *  - synthetic sequence generators for the benchmark workloads (no reference counterpart)
+ */
+#ifndef SALSA_HOST_H
+#define SALSA_HOST_H
+
+#include <stdint.h>
+#include "swg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* RangePartitioner.Partition(startIndex, length, numPartitions): the first
+ * length % n ranges get one extra element; ends are inclusive. */
+int salsa_range_partition(int32_t start, int32_t length, int32_t nparts, int32_t* starts, int32_t* ends);
+
+/* Checkerboard ownership of block (rank, j) (Program.cs:628-639): computed iff
+ * rank == j, or rank > j and rank+j even, or rank < j and rank+j odd. */
+int salsa_block_is_computed(int32_t rank, int32_t j);
+
+/* MPIComputeLowerTriangularBlocked.Compute for one rank of `world`: calls
+ * ComputeBlock (swg_compute_block) for every owned block of the rank's row
+ * stripe, writing into `partial` (PartialMatrix rows x n, row-major, ld n).
+ * elem_size = sizeof(TDistance): 2 (Int16) or 8 (Double). */
+int salsa_compute_rank(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank, void* partial,
+                       int32_t elem_size, int64_t* pairs);
+
+/* Every rank's Compute followed by the Scatter mirror, on one process: the
+ * n x n matrix that WriteFullMatrixOnRank0 writes (row-major TDistance). */
+int salsa_compute_full_matrix(swg_ctx* ctx, int32_t n, int32_t world, void* full, int32_t elem_size,
+                              int64_t* pairs);
+
+/* Raw row-major matrix file (MatrixBinaryWriter / WriteFullMatrixOnRank0). */
+int salsa_write_matrix(const char* path, const void* data, int64_t nbytes);
+
+/* One SWMS block file "<rank>-<j>.bin" (header of Program.cs:1116-1160 then
+ * the block's TDistance values row-major, little endian). names[4]: sequence
+ * names at row start, row end, column start, column end. */
+int salsa_write_swms_block(const char* path, const char* project, int32_t rank, const char* sysname,
+                           const char* fasta, int32_t row_block, int32_t col_block,
+                           int32_t row_start, int32_t row_end, int32_t col_start, int32_t col_end,
+                           const char* const* names, const void* data, int64_t nelem, int32_t elem_size);
+
+/* ScoringMatrix.Load from a matrix text file into a 127x127 float table. */
+int salsa_load_scoring_matrix(const char* path, float* scores);
+
+/* ---- synthetic workloads ---------------------------------------------- */
+/* Length models: 0 uniform integer [p0,p1]; 1 bimodal normal: weight p0 of
+ * N(p1,p2), else N(p3,p4), rounded and clipped to [p5,p6]; 2 log-normal:
+ * round(exp(N(ln p0, p1))) clipped to [p2,p3]. */
+#define SALSA_LEN_UNIFORM 0
+#define SALSA_LEN_BIMODAL 1
+#define SALSA_LEN_LOGNORMAL 2
+
+/* DNA: n sequences mutated from n_anc uniform-ACGT ancestors.  Returns the
+ * number of residue bytes (offsets[n+1] filled) or -needed when cap is too
+ * small. */
+int64_t salsa_synth_dna(uint64_t seed, int32_t n, int32_t n_anc, double sub_rate, double indel_rate,
+                        int32_t len_kind, const double* len_params, uint8_t* out, int64_t cap, int64_t* offsets);
+/* Protein: residues i.i.d. over the 20 amino acids with background
+ * frequencies from a seeded Dirichlet(alpha=1). */
+int64_t salsa_synth_protein(uint64_t seed, int32_t n, int32_t len_kind, const double* len_params,
+                            uint8_t* out, int64_t cap, int64_t* offsets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

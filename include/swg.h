@@ -1,0 +1,140 @@
+/*
+ * swg.h -- C ABI of the MI355X (gfx950) SALSA pairwise-distance library
+ * (libswg.so).  Plain C types only: pointers, sizes, int/float scalars.
+ *
+ * The library replaces the per-block alignment loop of the reference's C#
+ * programs (DSC-SPIDAL/csharp, SalsaTPL/...):
+ *
+ *   delegate int ComputeBlockHandler(Block block, PartialMatrix<TDistance> matrix,
+ *                                    bool isDiagonal);
+ *     Salsa.SmithWatermanMPI/Program.cs:561 (implementation :203-257)
+ *     Salsa.NeedlemanWunschTPL/Program.cs:977 (implementation :206-267)
+ *     Salsa.SmithWatermanMS/Program.cs:1018 (adds `int rank`, impl :265-387)
+ *
+ * whose inner call chain is SmithWatermanGotoh.Align -> Construct -> Traceback
+ * (Salsa.CoreTPL/Bio/Algorithms/SmithWaterman/SmithWatermanGotoh.cs:52-332)
+ * followed by ComputeDistance (Salsa.SmithWatermanMPI/Program.cs:259-314).
+ *
+ * Conventions: every int-returning call returns SWG_OK (0) on success and a
+ * negative SWG_E* code on failure; swg_last_error() then returns a message
+ * (thread-local).  A context is used by one host thread at a time.
+ */
+#ifndef SWG_H
+#define SWG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---------------------------------------------------- */
+#define SWG_OK                 0
+#define SWG_E_ARGUMENT       (-1)  /* bad argument (reference: ArgumentException / SmithWatermanGotohException on null) */
+#define SWG_E_SYMBOL         (-2)  /* residue >= 127 (reference: IndexOutOfRangeException at scores[c1,c2]) */
+#define SWG_E_NONINTEGRAL    (-3)  /* scoring matrix entry or gap penalty is not an integer: integer kernels refuse it */
+#define SWG_E_DEVICE         (-4)  /* HIP runtime error */
+#define SWG_E_UNSUPPORTED    (-5)  /* configuration not supported by the integer kernels */
+#define SWG_E_JC_UNDEFINED   (-6)  /* Jukes-Cantor log(0): reference throws Exception (Alignment.cs:342-345) */
+#define SWG_E_CAPACITY       (-7)  /* request exceeds a stated size limit */
+
+/* ---- enums mirroring the reference ----------------------------------- */
+/* Salsa.CoreTPL/Bio/Algorithms/AlignmentType.cs:3-15 */
+#define SWG_ALIGN_PROTEIN 0
+#define SWG_ALIGN_NUCLEIC 1
+#define SWG_DIST_PERCENT_IDENTITY 0
+#define SWG_DIST_KIMURA2          1
+#define SWG_DIST_JUKES_CANTOR     2
+#define SWG_DIST_MINMAX_NORM      3
+/* local (SmithWatermanGotoh, SWMPI/SWMS) or global (NeedlemanWunschTPL) */
+#define SWG_MODE_SW_LOCAL  0
+#define SWG_MODE_NW_GLOBAL 1
+/* TDistance: Int16 (USE_INT16 builds) or Double (Salsa.SmithWatermanMPI/Program.cs:16-22) */
+#define SWG_TDIST_INT16  0
+#define SWG_TDIST_DOUBLE 1
+/* diagonal-block semantics: SWMPI/NW compute j<i with s1 = the row (larger
+ * index) and mirror (Salsa.SmithWatermanMPI/Program.cs:209-231); SWMS computes
+ * every (i,j) of the block (Salsa.SmithWatermanMS/Program.cs:341-375) */
+#define SWG_DIAG_LOWER_MIRROR 0
+#define SWG_DIAG_FULL         1
+/* schedules */
+#define SWG_SCHED_PERSISTENT 0   /* length-bucketed tiles, device work queue, hipGraph per block, autotuned */
+#define SWG_SCHED_PLAIN      1   /* one launch per block, natural order, fixed strip height, int32 */
+
+#define SWG_MATRIX_SIZE 127      /* ScoringMatrix.SIZE, Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:34 */
+#define SWG_MAX_SEQ_LEN 32767    /* gap-size arrays are short in the reference (SmithWatermanGotoh.cs:83-84) */
+
+typedef struct swg_ctx swg_ctx;
+
+typedef struct swg_params {
+    int32_t mode;            /* SWG_MODE_* */
+    int32_t alignment_type;  /* SWG_ALIGN_*  (SmithWatermanSection.AlignmentType) */
+    int32_t distance_fn;     /* SWG_DIST_*   (SmithWatermanSection.DistanceFunctionType) */
+    int32_t tdistance;       /* SWG_TDIST_* */
+    int32_t diag_mode;       /* SWG_DIAG_* */
+    int32_t schedule;        /* SWG_SCHED_* */
+    float   gap_open;        /* _gapOpen  (float in SWMPI, Program.cs:58) */
+    float   gap_ext;         /* _gapExtension (Program.cs:59) */
+    const float* scores;     /* ScoringMatrix.Scores: 127*127 row-major, row = s1 symbol (ScoringMatrix.cs:155,188) */
+    int32_t device;          /* HIP device ordinal */
+    int32_t reserved;
+} swg_params;
+
+/* Per-pair statistics (optional output, same row-major position as the
+ * distance): the reference's PairwiseAlignment.Score, Sequence1.Length and
+ * Identity (nucleic) or Similarity (protein). */
+typedef struct swg_pair_stats {
+    int32_t score;    /* INT32_MIN where the reference's Score is -Infinity (empty sequence) */
+    int32_t columns;  /* aligned length */
+    int32_t count;    /* identity (nucleic) or similarity (protein) */
+    int32_t reserved;
+} swg_pair_stats;
+
+/* Create a context: validates the scoring matrix and gap penalties
+ * (integral, as the integer kernels require) and selects the device.
+ * Replaces `new SmithWatermanGotoh(type)` + `ScoringMatrix.Load(name)`
+ * (Salsa.SmithWatermanMPI/Program.cs:206-207). */
+int  swg_create(swg_ctx** ctx, const swg_params* p);
+
+/* Hand over the sequence set (the reference's static _sequences,
+ * Salsa.SmithWatermanMPI/Program.cs:65,94-96): residues concatenated,
+ * upper-case ASCII, offsets[n+1].  Copied and kept resident in HBM. */
+int  swg_set_sequences(swg_ctx* ctx, const uint8_t* residues, const int64_t* offsets, int32_t n);
+
+/* ComputeBlock (Salsa.SmithWatermanMPI/Program.cs:203-257): align every pair
+ * of the block [row_start,row_end] x [col_start,col_end] (inclusive, global
+ * sequence indices, as Range.StartIndex/EndIndex) and write TDistance values
+ * into `out` (host memory, row-major, element (i,j) at
+ * (i-row_start)*out_ld + (j-col_start)).  On a diagonal block with
+ * SWG_DIAG_LOWER_MIRROR only j<i is aligned (s1 = sequence i) and both (i,j)
+ * and (j,i) are written; (i,i) is written as 0.  `stats` (optional, host) gets
+ * one swg_pair_stats per computed (i,j) at the same position.  *pairs_done
+ * gets the reference's return value (number of alignments). */
+int  swg_compute_block(swg_ctx* ctx, int32_t row_start, int32_t row_end,
+                       int32_t col_start, int32_t col_end, int32_t is_diagonal,
+                       void* out, int64_t out_ld, swg_pair_stats* stats, int64_t* pairs_done);
+
+/* Same as swg_compute_block, but `dev_out`/`dev_stats` are device pointers
+ * and the call only enqueues work on the context's stream (`stream`, a
+ * hipStream_t, or NULL for the context's own).  Used for device-resident
+ * pipelines; completes at swg_synchronize(). */
+int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
+                              int32_t col_start, int32_t col_end, int32_t is_diagonal,
+                              void* dev_out, int64_t out_ld, swg_pair_stats* dev_stats,
+                              int64_t* pairs_done);
+int  swg_synchronize(swg_ctx* ctx);
+
+/* Schedule control (persistent vs plain, strip height, forced int mode);
+ * strip_rows 0 = autotune. */
+int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
+/* Timing of the last block's dominant kernel (HIP events on the stream it
+ * ran on): milliseconds and DP cells. */
+int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows);
+
+const char* swg_last_error(void);
+void swg_destroy(swg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def ednafull():
+    import csharp_amd as ca
+    return ca.load_scoring_matrix("EDNAFULL")
+
+
+@pytest.fixture(scope="session")
+def blosum62():
+    import csharp_amd as ca
+    return ca.load_scoring_matrix("BLOSUM62")
