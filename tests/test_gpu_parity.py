@@ -1,0 +1,210 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact comparison of Int16/Double distances and of the per-pair
+statistics (score, aligned length, identity/similarity count) on seeded
+inputs, over every block kind and schedule, plus edge cases.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import csharp_amd as ca
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_block(seqs, r0, r1, c0, c1, diag_lower, M, o, e, nucleic=True, tdouble=False):
+    """Expected block (rows x cols) and stats, computing (i, j) with s1 = s_i."""
+    rows, cols = r1 - r0 + 1, c1 - c0 + 1
+    a, b = [], []
+    for i in range(r0, r1 + 1):
+        for j in range(c0, c1 + 1):
+            if diag_lower and j >= i:
+                continue
+            a.append(i); b.append(j)
+    d, st = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, nucleic=nucleic, tdist_double=tdouble,
+                          threads=THREADS, want_stats=True)
+    out = np.zeros((rows, cols), dtype=np.float64 if tdouble else np.int16)
+    stats = np.zeros((rows, cols, 4), dtype=np.int32)
+    a = np.asarray(a) - r0
+    b = np.asarray(b) - c0
+    out[a, b] = d
+    stats[a, b, 0] = st[:, 0]
+    stats[a, b, 1] = st[:, 1]
+    stats[a, b, 2] = st[:, 2] if nucleic else st[:, 3]
+    if diag_lower:
+        out[b, a] = d
+        stats[b, a] = stats[a, b]
+    return out, stats
+
+
+def assert_same(got, want, gst, wst, what):
+    if got.dtype == np.float64:
+        same = (got == want) | (np.isnan(got) & np.isnan(want))
+    else:
+        same = got == want
+    if not same.all():
+        idx = np.argwhere(~same)[:5]
+        raise AssertionError("%s: %d mismatches, first %s got %s want %s stats got %s want %s" % (
+            what, (~same).sum(), idx.tolist(), [got[tuple(i)] for i in idx], [want[tuple(i)] for i in idx],
+            [gst[tuple(i)].tolist() for i in idx], [wst[tuple(i)].tolist() for i in idx]))
+    assert np.array_equal(gst[..., :3], wst[..., :3]), what + ": stats differ"
+
+
+@pytest.fixture(scope="module")
+def dna_set():
+    # related and unrelated DNA, lengths 1..260, plus edge sequences
+    s = ca.synth_dna(11, 180, 12, 0.10, 0.02, 0, [1, 260])
+    extra = [b"", b"A", b"ACGT" * 40, b"A" * 150, b"ACACACACAC" * 12, b"NNNNACGTNN", b"ACGUXACGT", b"T" * 300]
+    allseq = [s.seq(i) for i in range(len(s))] + extra
+    return ca.SequenceSet.from_strings(allseq)
+
+
+@pytest.fixture(scope="module")
+def protein_set():
+    s = ca.synth_protein(12, 120, 2, [120, 0.5, 1, 400])
+    extra = [b"", b"W", b"XXBZ*AR", b"ACDEFGHIKLMNPQRSTVWY" * 5]
+    return ca.SequenceSet.from_strings([s.seq(i) for i in range(len(s))] + extra)
+
+
+@pytest.mark.parametrize("schedule,strip", [(ca.SCHED_PERSISTENT, 16), (ca.SCHED_PLAIN, 16),
+                                            (ca.SCHED_PERSISTENT, 8), (ca.SCHED_PERSISTENT, 32)])
+def test_offdiag_block_int16(dna_set, ednafull, schedule, strip):
+    say("offdiag int16 schedule=%d strip=%d" % (schedule, strip))
+    al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
+    al.set_schedule(schedule, strip)
+    al.set_sequences(dna_set)
+    n = len(dna_set)
+    r0, r1, c0, c1 = 0, 89, 90, n - 1
+    got, gst, pairs = al.compute_block(r0, r1, c0, c1, False, stats=True)
+    want, wst = oracle_block(dna_set, r0, r1, c0, c1, False, ednafull, 16, 4)
+    assert pairs == (r1 - r0 + 1) * (c1 - c0 + 1)
+    assert_same(got, want, gst, wst, "offdiag")
+    al.close()
+
+
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN])
+def test_diagonal_block_lower_mirror(dna_set, ednafull, schedule):
+    say("diag lower-mirror schedule=%d" % schedule)
+    al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
+    al.set_sequences(dna_set)
+    n = len(dna_set)
+    r0, r1 = 60, n - 1
+    got, gst, pairs = al.compute_block(r0, r1, r0, r1, True, stats=True)
+    want, wst = oracle_block(dna_set, r0, r1, r0, r1, True, ednafull, 16, 4)
+    m = r1 - r0 + 1
+    assert pairs == m * (m - 1) // 2
+    assert_same(got, want, gst, wst, "diag")
+    assert np.all(np.diag(got) == 0)
+    al.close()
+
+
+def test_diagonal_block_full_swms(dna_set, ednafull):
+    say("diag full")
+    al = ca.Aligner(ednafull, 16, 4, diag_mode=ca.DIAG_FULL)
+    al.set_sequences(dna_set)
+    r0, r1 = 100, 170
+    got, gst, pairs = al.compute_block(r0, r1, r0, r1, True, stats=True)
+    want, wst = oracle_block(dna_set, r0, r1, r0, r1, False, ednafull, 16, 4)
+    assert pairs == (r1 - r0 + 1) ** 2
+    assert_same(got, want, gst, wst, "diag-full")
+    al.close()
+
+
+def test_double_distances(dna_set, ednafull):
+    say("double")
+    al = ca.Aligner(ednafull, 16, 4, tdistance=ca.TDIST_DOUBLE)
+    al.set_sequences(dna_set)
+    n = len(dna_set)
+    got, gst, _ = al.compute_block(0, 70, 71, n - 1, False, stats=True)
+    want, wst = oracle_block(dna_set, 0, 70, 71, n - 1, False, ednafull, 16, 4, tdouble=True)
+    assert_same(got, want, gst, wst, "double")
+    al.close()
+
+
+def test_protein_percent_similarity(protein_set, blosum62):
+    say("protein")
+    al = ca.Aligner(blosum62, 10, 1, alignment_type=ca.ALIGN_PROTEIN)
+    al.set_sequences(protein_set)
+    n = len(protein_set)
+    got, gst, _ = al.compute_block(0, n - 1, 0, n - 1, True, stats=True)
+    want, wst = oracle_block(protein_set, 0, n - 1, 0, n - 1, True, blosum62, 10, 1, nucleic=False)
+    assert_same(got, want, gst, wst, "protein")
+    al.close()
+
+
+def test_swapped_arguments_follow_block_order(dna_set, ednafull):
+    # align(s_i, s_j) and align(s_j, s_i) may differ in identity/length:
+    # the block's rows are always s1
+    say("swapped")
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(dna_set)
+    g1, s1, _ = al.compute_block(0, 40, 41, 90, False, stats=True)
+    g2, s2, _ = al.compute_block(41, 90, 0, 40, False, stats=True)
+    w1, _ = oracle_block(dna_set, 0, 40, 41, 90, False, ednafull, 16, 4)
+    w2, _ = oracle_block(dna_set, 41, 90, 0, 40, False, ednafull, 16, 4)
+    assert np.array_equal(g1, w1) and np.array_equal(g2, w2)
+    assert np.array_equal(s1[..., 0], s2[..., 0].T)  # the score is symmetric
+    al.close()
+
+
+def test_full_matrix_multi_rank_layout(dna_set, ednafull):
+    # every rank's checkerboard blocks + scatter mirror, P = 3 (Program.cs:618-677)
+    say("full matrix P=3")
+    n = 120
+    sub = dna_set.subset(range(n))
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(sub)
+    full, pairs = al.compute_full_matrix(world=3)
+    assert pairs == n * (n - 1) // 2
+    ranges = ca.range_partition(n, 3)
+    owner = np.zeros(n, dtype=int)
+    for r, (s, e) in enumerate(ranges):
+        owner[s:e + 1] = r
+    a, b = [], []
+    for x in range(n):
+        for y in range(x + 1, n):
+            ra, rb = owner[x], owner[y]
+            if ra == rb or (ra + rb) % 2 == 0:
+                a.append(y); b.append(x)  # s1 = larger index
+            else:
+                a.append(x); b.append(y)
+    d, _ = O.sw_pairs_mt(sub.residues, sub.offsets, a, b, ednafull, 16, 4, threads=THREADS)
+    want = np.zeros((n, n), dtype=np.int16)
+    want[a, b] = d
+    want[b, a] = d
+    assert np.array_equal(full, want)
+    al.close()
+
+
+def test_long_sequences(ednafull):
+    say("long sequences")
+    s = ca.synth_dna(13, 70, 3, 0.10, 0.02, 0, [1500, 2000])
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(s)
+    got, gst, _ = al.compute_block(0, 5, 6, 69, False, stats=True)
+    want, wst = oracle_block(s, 0, 5, 6, 69, False, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "long")
+    al.close()
+
+
+def test_repeated_blocks_reuse_graph(dna_set, ednafull):
+    say("repeat")
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(dna_set)
+    want, wst = oracle_block(dna_set, 0, 30, 31, 120, False, ednafull, 16, 4)
+    for _ in range(3):
+        got, gst, _ = al.compute_block(0, 30, 31, 120, False, stats=True)
+        assert_same(got, want, gst, wst, "repeat")
+        other, _, _ = al.compute_block(40, 60, 0, 39, False)
+    al.close()
