@@ -1,0 +1,214 @@
+"""Benchmark: bit-exact GCUPS of the MI355X SW-Gotoh path on the
+uneven-length 50,000-sequence DNA workload (persistent batched schedule vs the
+plain one-launch-per-block kernel), with the CPU oracle timed beside it.
+
+A step is one reference Block of the 50k workload decomposed as the
+reference does for P = 32 ranks (RangePartitioner, ~1563 x 1563 pairs,
+~1.8e12 DP cells); the steps cycle through rank 0's off-diagonal checkerboard
+blocks.  Sequences are resident in HBM before timing; distances are written
+to a device buffer (the PCIe-inclusive rate is reported separately).
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W_OPS_PER_CELL = 24            # integer VALU lane-instructions per cell of the bit-exact recurrence
+PEAK_INT32_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 3.93e13 lane-instructions/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--world", type=int, default=32, help="reference rank count used for the block decomposition")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 3}))
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(pairs shard by the reference's block checkerboard across GPUs, see DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import csharp_amd as ca
+    from csharp_amd import workloads as WL
+
+    t0 = time.time()
+    seqs = WL.sequences("c3")
+    M, o, e = WL.scoring("c3")
+    n = len(seqs)
+    L = seqs.lengths
+    log("workload c3: %d sequences, mean length %.1f (generated in %.1fs)" % (n, L.mean(), time.time() - t0))
+
+    ranges = ca.range_partition(n, args.world)
+    owned = [j for j in range(args.world) if j != 0 and ca.block_is_computed(0, j)]
+    blocks = [(ranges[0], ranges[j]) for j in owned]
+    max_rows = max(r[1] - r[0] + 1 for r, _ in blocks)
+    max_cols = max(c[1] - c[0] + 1 for _, c in blocks)
+
+    torch.cuda.init()
+    dev_out = torch.empty(max_rows * max_cols, dtype=torch.int16, device="cuda")
+    al = ca.Aligner(M, o, e, alignment_type=ca.ALIGN_NUCLEIC, tdistance=ca.TDIST_INT16,
+                    schedule=ca.SCHED_PERSISTENT)
+    al.set_sequences(seqs)
+    if args.strip:
+        al.set_schedule(ca.SCHED_PERSISTENT, args.strip)
+
+    def run_block(k):
+        (r0, r1), (c0, c1) = blocks[k % len(blocks)]
+        pairs = al.compute_block_device(r0, r1, c0, c1, False, dev_out.data_ptr(), c1 - c0 + 1)
+        al.synchronize()
+        ms, cells, strip = al.last_kernel_stats()
+        return pairs, cells, ms, strip
+
+    for k in range(args.warmup):
+        p, c, ms, strip = run_block(k)
+        log("warmup block %d: %.3e cells, kernel %.1f ms (strip %d)" % (k, c, ms, strip))
+
+    torch.cuda.synchronize()
+    tot_cells, tot_kms, tot_pairs = 0.0, 0.0, 0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        p, c, ms, strip = run_block(args.warmup + k)
+        tot_cells += c; tot_kms += ms; tot_pairs += p
+        log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (strip %d)" % (
+            k, p, c, ms, c / ms / 1e6, strip))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    gcups = tot_cells / wall / 1e9
+    kernel_gcups = tot_cells / (tot_kms / 1e3) / 1e9
+    last_block = blocks[(args.warmup + args.steps - 1) % len(blocks)]
+    result_dev = dev_out[: (last_block[0][1] - last_block[0][0] + 1) * (last_block[1][1] - last_block[1][0] + 1)]
+    result = result_dev.cpu().numpy().reshape(last_block[0][1] - last_block[0][0] + 1, -1)
+
+    # PCIe-inclusive rate of the host-memory entry point (one block)
+    (r0, r1), (c0, c1) = blocks[0]
+    th = time.perf_counter()
+    host_out, _, _ = al.compute_block(r0, r1, c0, c1, False)
+    t_host = time.perf_counter() - th
+    host_cells = WL.block_cells(L, r0, r1, c0, c1)
+
+    # plain schedule: one launch per block, natural order, fixed strip
+    plain_cells, plain_wall, plain_kms = 0.0, 0.0, 0.0
+    if args.plain_steps > 0:
+        al.set_schedule(ca.SCHED_PLAIN, 16)
+        run_block(args.warmup)  # warm
+        tp = time.perf_counter()
+        for k in range(args.plain_steps):
+            p, c, ms, _ = run_block(args.warmup + k)
+            plain_cells += c; plain_kms += ms
+            log("plain step %d: kernel %.1f ms -> %.1f GCUPS" % (k, ms, c / ms / 1e6))
+        plain_wall = time.perf_counter() - tp
+    plain_gcups = plain_cells / plain_wall / 1e9 if plain_wall > 0 else None
+
+    # CPU baseline (oracle, literal restatement of the reference aligner) on a
+    # bounded random sample of the last timed block; the same sample checks
+    # the GPU distances bit-exactly
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        (r0, r1), (c0, c1) = last_block
+        rng = np.random.default_rng(0)
+
+        def sample(k):
+            a = rng.integers(r0, r1 + 1, size=k)
+            b = rng.integers(c0, c1 + 1, size=k)
+            return a, b
+
+        a, b = sample(256)
+        tc = time.perf_counter()
+        d0, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, threads=threads)
+        dt = time.perf_counter() - tc
+        rate = float((L[a].astype(np.float64) * L[b]).sum()) / dt
+        per_pair = float(L[r0:r1 + 1].mean() * L[c0:c1 + 1].mean())
+        k = int(max(256, min(200000, args.cpu_seconds * rate / per_pair)))
+        a2, b2 = sample(k)
+        tc = time.perf_counter()
+        d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a2, b2, M, o, e, threads=threads)
+        dt = time.perf_counter() - tc
+        cells = float((L[a2].astype(np.float64) * L[b2]).sum())
+        cpu = {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
+               "sample": "%d random pairs of the last timed block (%.3e cells, %.1f s)" % (k, cells, dt)}
+        A = np.concatenate([a, a2]); B = np.concatenate([b, b2]); D = np.concatenate([d0, d])
+        got = result[A - r0, B - c0]
+        hb = blocks[0]
+        parity = {"checked_pairs": int(len(A)), "mismatches": int((got != D).sum())}
+        log("cpu baseline: %.2f GCUPS on %d threads; parity %s" % (cpu["value"], threads, parity))
+
+    achieved = kernel_gcups * 1e9 * W_OPS_PER_CELL / 1e12
+    peak = PEAK_INT32_OPS / 1e12
+    line = {
+        "metric": "bit-exact GCUPS (SW-Gotoh all-pairs, persistent batched schedule)",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": "uneven-length DNA all-pairs: 50,000 sequences (60%% N(450,50), 40%% N(1450,100)), "
+                        "EDNAFULL 16/4, PercentIdentity Int16; step = one %dx%d reference block at P=%d"
+                        % (max_rows, max_cols, args.world),
+            "sequences": n,
+            "pairs_per_step": tot_pairs // max(args.steps, 1),
+            "cells_per_step": tot_cells / max(args.steps, 1),
+            "strip_rows": strip,
+        },
+        "kernel_gcups": round(kernel_gcups, 2),
+        "plain_gcups": round(plain_gcups, 2) if plain_gcups else None,
+        "speedup_vs_plain": round(gcups / plain_gcups, 3) if plain_gcups else None,
+        "gcups_host_memory_entry": round(host_cells / t_host / 1e9, 2),
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": round(peak, 3),
+            "unit": "Tops/s (int32 VALU lane-instructions, W=%d per cell)" % W_OPS_PER_CELL,
+            "frac": round(achieved / peak, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    al.close()
+
+
+if __name__ == "__main__":
+    main()
