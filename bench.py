@@ -147,28 +147,33 @@ def main():
             b = rng.integers(c0, c1 + 1, size=k)
             return a, b
 
-        a, b = sample(256)
-        tc = time.perf_counter()
-        d0, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, threads=threads)
-        dt = time.perf_counter() - tc
-        rate = float((L[a].astype(np.float64) * L[b]).sum()) / dt
-        per_pair = float(L[r0:r1 + 1].mean() * L[c0:c1 + 1].mean())
-        k = int(max(256, min(200000, args.cpu_seconds * rate / per_pair)))
-        a2, b2 = sample(k)
-        tc = time.perf_counter()
-        d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a2, b2, M, o, e, threads=threads)
-        dt = time.perf_counter() - tc
-        cells = float((L[a2].astype(np.float64) * L[b2]).sum())
+        As, Bs, Ds = [], [], []
+        cells, dt, chunk = 0.0, 0.0, 512
+        while dt < args.cpu_seconds:
+            a, b = sample(chunk)
+            tc = time.perf_counter()
+            d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, threads=threads)
+            dt += time.perf_counter() - tc
+            cells += float((L[a].astype(np.float64) * L[b]).sum())
+            As.append(a); Bs.append(b); Ds.append(d)
+            chunk = min(chunk * 2, 16384)
+        A = np.concatenate(As); B = np.concatenate(Bs); D = np.concatenate(Ds)
         cpu = {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
-               "sample": "%d random pairs of the last timed block (%.3e cells, %.1f s)" % (k, cells, dt)}
-        A = np.concatenate([a, a2]); B = np.concatenate([b, b2]); D = np.concatenate([d0, d])
+               "sample": "%d random pairs of the last timed block (%.3e cells, %.1f s)" % (len(A), cells, dt)}
         got = result[A - r0, B - c0]
-        hb = blocks[0]
         parity = {"checked_pairs": int(len(A)), "mismatches": int((got != D).sum())}
         log("cpu baseline: %.2f GCUPS on %d threads; parity %s" % (cpu["value"], threads, parity))
 
     achieved = kernel_gcups * 1e9 * W_OPS_PER_CELL / 1e12
     peak = PEAK_INT32_OPS / 1e12
+    # HBM bytes per launch of this kernel variant from the committed rocprofv3
+    # FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), if one exists
+    traffic = None
+    try:
+        key = "sw_kernel<%d, true>|c3_block_p%d" % (strip, args.world)
+        traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(key, {}).get("traffic_bytes")
+    except (OSError, ValueError):
+        traffic = None
     line = {
         "metric": "bit-exact GCUPS (SW-Gotoh all-pairs, persistent batched schedule)",
         "value": round(gcups, 2),
@@ -201,7 +206,8 @@ def main():
             "peak": round(peak, 3),
             "unit": "Tops/s (int32 VALU lane-instructions, W=%d per cell)" % W_OPS_PER_CELL,
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "kernel": "sw_kernel<%d, persistent>" % strip,
         },
         "cpu_baseline": cpu,
         "parity": parity,
