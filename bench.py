@@ -36,6 +36,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--world", type=int, default=32, help="reference rank count used for the block decomposition")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--variant-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
@@ -84,25 +85,28 @@ def main():
     if args.strip:
         al.set_schedule(ca.SCHED_PERSISTENT, args.strip)
 
+    VNAME = {ca.SCHED_PLAIN: "plain", ca.SCHED_PERSISTENT_I32: "int32 one-pass",
+             ca.SCHED_PERSISTENT_I16: "int16 score pass + bounded int32 re-run"}
+
     def run_block(k):
         (r0, r1), (c0, c1) = blocks[k % len(blocks)]
         pairs = al.compute_block_device(r0, r1, c0, c1, False, dev_out.data_ptr(), c1 - c0 + 1)
         al.synchronize()
-        ms, cells, strip = al.last_kernel_stats()
-        return pairs, cells, ms, strip
+        ms, cells, strip, variant = al.last_kernel_stats()
+        return pairs, cells, ms, strip, variant
 
     for k in range(args.warmup):
-        p, c, ms, strip = run_block(k)
-        log("warmup block %d: %.3e cells, kernel %.1f ms (strip %d)" % (k, c, ms, strip))
+        p, c, ms, strip, var = run_block(k)
+        log("warmup block %d: %.3e cells, kernel %.1f ms (strip %d, %s)" % (k, c, ms, strip, VNAME[var]))
 
     torch.cuda.synchronize()
     tot_cells, tot_kms, tot_pairs = 0.0, 0.0, 0
     t_start = time.perf_counter()
     for k in range(args.steps):
-        p, c, ms, strip = run_block(args.warmup + k)
+        p, c, ms, strip, var = run_block(args.warmup + k)
         tot_cells += c; tot_kms += ms; tot_pairs += p
-        log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (strip %d)" % (
-            k, p, c, ms, c / ms / 1e6, strip))
+        log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (strip %d, %s)" % (
+            k, p, c, ms, c / ms / 1e6, strip, VNAME[var]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     gcups = tot_cells / wall / 1e9
@@ -118,6 +122,19 @@ def main():
     t_host = time.perf_counter() - th
     host_cells = WL.block_cells(L, r0, r1, c0, c1)
 
+    # each forced variant on one block (kernel-only GCUPS), for the record
+    variants = {}
+    if args.variant_steps > 0:
+        for sched in (ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16):
+            al.set_schedule(sched, 0)
+            run_block(args.warmup)  # capture
+            vc, vms = 0.0, 0.0
+            for k in range(args.variant_steps):
+                p, c, ms, _, _ = run_block(args.warmup + k)
+                vc += c; vms += ms
+            variants[VNAME[sched]] = round(vc / (vms / 1e3) / 1e9, 2)
+            log("variant %s: %.1f GCUPS kernel-only" % (VNAME[sched], variants[VNAME[sched]]))
+
     # plain schedule: one launch per block, natural order, fixed strip
     plain_cells, plain_wall, plain_kms = 0.0, 0.0, 0.0
     if args.plain_steps > 0:
@@ -125,7 +142,7 @@ def main():
         run_block(args.warmup)  # warm
         tp = time.perf_counter()
         for k in range(args.plain_steps):
-            p, c, ms, _ = run_block(args.warmup + k)
+            p, c, ms, _, _ = run_block(args.warmup + k)
             plain_cells += c; plain_kms += ms
             log("plain step %d: kernel %.1f ms -> %.1f GCUPS" % (k, ms, c / ms / 1e6))
         plain_wall = time.perf_counter() - tp
@@ -170,7 +187,7 @@ def main():
     # FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), if one exists
     traffic = None
     try:
-        key = "sw_kernel<%d, true>|c3_block_p%d" % (strip, args.world)
+        key = "%s|c3_block_p%d" % (VNAME[var], args.world)
         traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(key, {}).get("traffic_bytes")
     except (OSError, ValueError):
         traffic = None
@@ -197,6 +214,8 @@ def main():
             "strip_rows": strip,
         },
         "kernel_gcups": round(kernel_gcups, 2),
+        "variant": VNAME[var],
+        "variants_kernel_gcups": variants,
         "plain_gcups": round(plain_gcups, 2) if plain_gcups else None,
         "speedup_vs_plain": round(gcups / plain_gcups, 3) if plain_gcups else None,
         "gcups_host_memory_entry": round(host_cells / t_host / 1e9, 2),
@@ -207,7 +226,7 @@ def main():
             "unit": "Tops/s (int32 VALU lane-instructions, W=%d per cell)" % W_OPS_PER_CELL,
             "frac": round(achieved / peak, 4),
             "traffic": traffic,
-            "kernel": "sw_kernel<%d, persistent>" % strip,
+            "kernel": "%s (one hipGraph per block; strip %d)" % (VNAME[var], strip),
         },
         "cpu_baseline": cpu,
         "parity": parity,

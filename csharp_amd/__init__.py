@@ -33,7 +33,7 @@ DIST_PERCENT_IDENTITY, DIST_KIMURA2, DIST_JUKES_CANTOR, DIST_MINMAX_NORM = 0, 1,
 MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
 TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
-SCHED_PERSISTENT, SCHED_PLAIN = 0, 1
+SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16 = 0, 1, 2, 3
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (
@@ -93,7 +93,7 @@ def lib():
         L.swg_synchronize.argtypes = [vp]
         L.swg_set_schedule.argtypes = [vp, i32, i32]
         L.swg_last_kernel_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                                            ctypes.POINTER(i32)]
+                                            ctypes.POINTER(i32), ctypes.POINTER(i32)]
         L.swg_last_error.restype = ctypes.c_char_p
         L.swg_destroy.argtypes = [vp]
         L.swg_destroy.restype = None
@@ -302,9 +302,11 @@ class Aligner:
         _check(lib().swg_synchronize(self._h))
 
     def last_kernel_stats(self):
-        ms, cells, strip = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
-        _check(lib().swg_last_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(cells), ctypes.byref(strip)))
-        return ms.value, cells.value, strip.value
+        """(kernel ms, DP cells, strip rows, variant SWG_SCHED_*) of the last block."""
+        ms, cells, strip, var = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().swg_last_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(cells), ctypes.byref(strip),
+                                           ctypes.byref(var)))
+        return ms.value, cells.value, strip.value, var.value
 
     def compute_full_matrix(self, world=1):
         """SWMPI end to end on one process (every rank's Compute + Scatter):

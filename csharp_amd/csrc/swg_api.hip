@@ -49,17 +49,35 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Kernel variants a block can run with.
+enum Variant : int {
+    kVarPlain = 0,     // sw_kernel<16, false>: natural order, static assignment
+    kVarI32 = 1,       // sw_kernel<R, true>: one pass, int32, statistics carried
+    kVarI16 = 2,       // score16_kernel (int16x2) + group_kernel + bounded sw_kernel
+    kNumVariants = 3,
+};
+
+constexpr int kScoreTile = 2 * kWave;  // pairs per score-pass task
+
 // One set of per-block device buffers; two of them alternate so that the
 // host can plan block k+1 while block k runs.
 struct BlockBufs {
-    DevBuf<int32_t> task_row, task_tile, tile_col, tile_maxlen, queue;
-    DevBuf<BlockDesc> desc;
+    // alignment tasks (single pass, or the bounded second pass)
+    DevBuf<int32_t> task_row, task_tile, tile_col, tile_maxlen, task_rowlim, tile_lim;
+    // score pass
+    DevBuf<int32_t> s_task_row, s_task_tile, s_tile_col, s_tile_maxlen;
+    DevBuf<uint32_t> best_cell;
+    DevBuf<GroupChunk> chunks;
+    DevBuf<int32_t> nchunks;
+    DevBuf<int32_t> queue;        // [0]: first pass, [16]: second pass
+    DevBuf<BlockDesc> desc;       // [0]: output desc, [1]: score-pass desc
     std::vector<int32_t> h_task_row, h_task_tile, h_tile_col, h_tile_maxlen;
+    std::vector<GroupChunk> h_chunks;
     hipEvent_t done = nullptr;
     bool pending = false;
-    // captured graph (persistent schedule) for this buffer set
-    hipGraphExec_t graph = nullptr;
-    unsigned char graph_key[sizeof(SeqSet) + sizeof(TaskSet) + sizeof(Scratch) + 2 * sizeof(int)] = {0};
+    // one captured graph per variant, keyed by every baked launch argument
+    hipGraphExec_t graph[kNumVariants] = {nullptr, nullptr, nullptr};
+    std::vector<unsigned char> graph_key[kNumVariants];
 };
 
 }  // namespace
@@ -74,6 +92,7 @@ struct swg_ctx {
     int32_t iscores[SWG_MATRIX_SIZE * SWG_MATRIX_SIZE];
     int32_t gap_open = 0, gap_ext = 0;
     bool count_similarity = false;  // protein: PercentSimilarity
+    bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
 
     // sequences
     int32_t nseq = 0;
@@ -87,11 +106,10 @@ struct swg_ctx {
     DevBuf<int2> prof;
     int64_t pad_off = 0;
 
-    // scratch
+    // scratch: one boundary slot per resident workgroup of any variant
     DevBuf<int4> bnd;
     int64_t bnd_stride = 0;
-    int nslots = 0;
-    int slots_strip = 0;
+    int bnd_slots = 0;
 
     // outputs staged on device for the host-memory entry point
     DevBuf<uint8_t> out_stage;
@@ -102,9 +120,11 @@ struct swg_ctx {
 
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;  // 0 = autotune
-    int32_t tuned_strip = 0;
+    // autotuner cache: (ncode, length bucket) -> variant
+    std::vector<std::pair<int64_t, int>> tuned;
     double last_kernel_ms = 0.0, last_cells = 0.0;
     int32_t last_strip = 0;
+    int32_t last_variant = 0;
 };
 
 namespace {
@@ -113,21 +133,37 @@ bool is_integral(float v) { return std::isfinite(v) && std::floor(v) == v && std
 
 int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 : 2; }
 
-// Number of resident workgroups for the kernel variant and the boundary
-// buffer they need (one slot per resident wave; HBM budget-capped).
-int ensure_slots(swg_ctx* c, int strip)
+// strip heights per variant: (first kernel, second kernel)
+void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
 {
-    if (c->nslots > 0 && c->slots_strip == strip && c->bnd.p) return SWG_OK;
-    int per_cu = sw_blocks_per_cu(strip, false);
-    if (per_cu <= 0) return fail(SWG_E_DEVICE, "occupancy query failed");
-    int slots = per_cu * c->num_cu;
+    if (v == kVarPlain) { *s1 = 16; *s2 = 0; return; }
+    if (v == kVarI32) { *s1 = c->strip_rows ? c->strip_rows : 16; *s2 = 0; return; }
+    *s1 = 16;  // score pass
+    *s2 = c->strip_rows ? c->strip_rows : 16;
+}
+
+int slots_for(const swg_ctx* c, int v, int* n1, int* n2)
+{
+    int s1, s2;
+    variant_strips(c, v, &s1, &s2);
+    int a = v == kVarI16 ? score16_blocks_per_cu(s1) : sw_blocks_per_cu(s1, false);
+    int b = v == kVarI16 ? sw_blocks_per_cu(s2, false) : 0;
+    if (a <= 0 || (v == kVarI16 && b <= 0)) return fail(SWG_E_DEVICE, "occupancy query failed");
+    *n1 = a * c->num_cu;
+    *n2 = b * c->num_cu;
+    return SWG_OK;
+}
+
+// Boundary scratch for `slots` resident workgroups (HBM budget-capped).
+int ensure_scratch(swg_ctx* c, int* slots)
+{
     const int64_t stride = (int64_t)(c->maxlen + 16) * kWave;
     const double budget = 24.0 * (1u << 30);
-    while (slots > c->num_cu && (double)slots * (double)stride * sizeof(int4) > budget) slots -= c->num_cu;
-    SWG_HIP(c->bnd.reserve((size_t)slots * (size_t)stride));
+    while (*slots > c->num_cu && (double)(*slots) * (double)stride * sizeof(int4) > budget) *slots -= c->num_cu;
+    if (c->bnd.p && c->bnd_stride == stride && c->bnd_slots >= *slots) return SWG_OK;
+    SWG_HIP(c->bnd.reserve((size_t)(*slots) * (size_t)stride));
     c->bnd_stride = stride;
-    c->nslots = slots;
-    c->slots_strip = strip;
+    c->bnd_slots = *slots;
     return SWG_OK;
 }
 
@@ -136,31 +172,48 @@ int ensure_slots(swg_ctx* c, int strip)
 struct Plan {
     int64_t pairs = 0;
     double cells = 0.0;
+    int32_t ntasks = 0;   // tasks of the (first) kernel
+    int32_t ntasks2 = 0;  // bounded second pass
+    int32_t nchunks = 0;
 };
 
-// Tiles of 64 column sequences and tasks (row sequence, tile).
-//   persistent: columns sorted by length (longest first) so that the lanes of
-//     a wave do similar work; tasks ordered by estimated cost, longest first,
-//     drained from a device queue.
-//   plain: columns in natural order, tasks row-major, static assignment.
-void plan_block(swg_ctx* c, BlockBufs& B, int32_t r0, int32_t r1, int32_t c0, int32_t c1,
-                bool diag_lower, bool sorted, Plan& plan)
+struct BlockGeom {
+    int32_t r0, r1, c0, c1;
+    bool diag_lower;
+};
+
+// Rows/columns of the block sorted by length, longest first (persistent
+// variants) or natural order (plain).
+void order_block(const swg_ctx* c, const BlockGeom& g, bool sorted, std::vector<int32_t>& rows,
+                 std::vector<int32_t>& cols)
 {
     const std::vector<int32_t>& L = c->h_len;
-    auto& trow = B.h_task_row; auto& ttile = B.h_task_tile;
-    auto& tcol = B.h_tile_col; auto& tmax = B.h_tile_maxlen;
+    rows.resize((size_t)(g.r1 - g.r0 + 1));
+    std::iota(rows.begin(), rows.end(), g.r0);
+    cols.resize((size_t)(g.c1 - g.c0 + 1));
+    std::iota(cols.begin(), cols.end(), g.c0);
+    if (sorted) {
+        auto longer = [&](int32_t x, int32_t y) { return L[x] > L[y]; };
+        std::stable_sort(rows.begin(), rows.end(), longer);
+        std::stable_sort(cols.begin(), cols.end(), longer);
+    }
+}
+
+// Tiles of `width` column sequences; task = (row sequence, tile).  Off-
+// diagonal blocks share one tile set across rows; a lower-mirror diagonal
+// block gets per-row tiles of its columns j < i.  Tasks are emitted row by
+// row, rows longest first, so the queue drains long work first.
+void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, std::vector<int32_t>& trow,
+                std::vector<int32_t>& ttile, std::vector<int32_t>& tcol, std::vector<int32_t>& tmax, Plan& plan)
+{
+    const std::vector<int32_t>& L = c->h_len;
     trow.clear(); ttile.clear(); tcol.clear(); tmax.clear();
-    plan = Plan{};
-
-    std::vector<int32_t> cols((size_t)(c1 - c0 + 1));
-    std::iota(cols.begin(), cols.end(), c0);
-    if (sorted)
-        std::stable_sort(cols.begin(), cols.end(), [&](int32_t x, int32_t y) { return L[x] > L[y]; });
-
+    std::vector<int32_t> rows, cols;
+    order_block(c, g, sorted, rows, cols);
     auto emit_tile = [&](const int32_t* ids, int n) -> int32_t {
         const int32_t t = (int32_t)tmax.size();
         int32_t mx = 0;
-        for (int k = 0; k < kWave; ++k) {
+        for (int k = 0; k < width; ++k) {
             const int32_t id = k < n ? ids[k] : -1;
             tcol.push_back(id);
             if (id >= 0) mx = std::max(mx, L[id]);
@@ -168,19 +221,13 @@ void plan_block(swg_ctx* c, BlockBufs& B, int32_t r0, int32_t r1, int32_t c0, in
         tmax.push_back(mx);
         return t;
     };
-
-    std::vector<int32_t> rows((size_t)(r1 - r0 + 1));
-    std::iota(rows.begin(), rows.end(), r0);
-    if (sorted)
-        std::stable_sort(rows.begin(), rows.end(), [&](int32_t x, int32_t y) { return L[x] > L[y]; });
-
-    if (!diag_lower) {
+    plan.pairs = 0;
+    plan.cells = 0.0;
+    if (!g.diag_lower) {
         std::vector<int32_t> tiles;
         double sum_cols = 0.0;
-        for (size_t k = 0; k < cols.size(); k += kWave) {
-            const int n = (int)std::min<size_t>(kWave, cols.size() - k);
-            tiles.push_back(emit_tile(&cols[k], n));
-        }
+        for (size_t k = 0; k < cols.size(); k += width)
+            tiles.push_back(emit_tile(&cols[k], (int)std::min<size_t>(width, cols.size() - k)));
         for (int32_t x : cols) sum_cols += L[x];
         for (int32_t a : rows) {
             for (int32_t t : tiles) { trow.push_back(a); ttile.push_back(t); }
@@ -193,51 +240,50 @@ void plan_block(swg_ctx* c, BlockBufs& B, int32_t r0, int32_t r1, int32_t c0, in
         for (int32_t a : rows) {
             valid.clear();
             for (int32_t x : cols) if (x < a) valid.push_back(x);
-            for (size_t k = 0; k < valid.size(); k += kWave) {
-                const int n = (int)std::min<size_t>(kWave, valid.size() - k);
+            for (size_t k = 0; k < valid.size(); k += width) {
                 trow.push_back(a);
-                ttile.push_back(emit_tile(&valid[k], n));
+                ttile.push_back(emit_tile(&valid[k], (int)std::min<size_t>(width, valid.size() - k)));
             }
             for (int32_t x : valid) plan.cells += (double)L[a] * L[x];
             plan.pairs += (int64_t)valid.size();
         }
     }
-
-    if (sorted) {
-        // longest-processing-time-first order for the work queue
-        const size_t nt = trow.size();
-        std::vector<uint32_t> order(nt);
-        std::iota(order.begin(), order.end(), 0u);
-        std::vector<int64_t> cost(nt);
-        for (size_t k = 0; k < nt; ++k) cost[k] = (int64_t)L[trow[k]] * (int64_t)tmax[ttile[k]];
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
-        std::vector<int32_t> r2(nt), t2(nt);
-        for (size_t k = 0; k < nt; ++k) { r2[k] = trow[order[k]]; t2[k] = ttile[order[k]]; }
-        trow.swap(r2); ttile.swap(t2);
-    }
+    plan.ntasks = (int32_t)trow.size();
 }
 
-int upload_plan(swg_ctx* c, BlockBufs& B, const BlockDesc& desc)
+// Grouping chunks of the two-pass variant: for every row (same order as the
+// score-pass tasks) its partners j (contiguous column indices: the whole
+// column range, or c0..i-1 on a lower-mirror diagonal block) in chunks of
+// kGroupChunk, each emitting ceil(n/64) bounded tasks.
+void plan_chunks(const swg_ctx* c, const BlockGeom& g, std::vector<GroupChunk>& chunks, Plan& plan)
 {
-    const size_t nt = B.h_task_row.size();
-    const size_t ntile = B.h_tile_maxlen.size();
-    SWG_HIP(B.task_row.reserve(std::max<size_t>(nt, 1)));
-    SWG_HIP(B.task_tile.reserve(std::max<size_t>(nt, 1)));
-    SWG_HIP(B.tile_col.reserve(std::max<size_t>(ntile * kWave, 1)));
-    SWG_HIP(B.tile_maxlen.reserve(std::max<size_t>(ntile, 1)));
-    SWG_HIP(B.queue.reserve(64));
-    SWG_HIP(B.desc.reserve(1));
-    if (nt) {
-        SWG_HIP(hipMemcpyAsync(B.task_row.p, B.h_task_row.data(), nt * 4, hipMemcpyHostToDevice, c->stream));
-        SWG_HIP(hipMemcpyAsync(B.task_tile.p, B.h_task_tile.data(), nt * 4, hipMemcpyHostToDevice, c->stream));
+    chunks.clear();
+    std::vector<int32_t> rows, cols;
+    order_block(c, g, true, rows, cols);
+    const int64_t ld = g.c1 - g.c0 + 1;
+    int32_t base = 0;
+    for (int32_t a : rows) {
+        const int32_t last = g.diag_lower ? a - 1 : g.c1;
+        for (int32_t f = g.c0; f <= last; f += kGroupChunk) {
+            GroupChunk ch;
+            ch.row = a;
+            ch.first_col = f;
+            ch.n = std::min<int32_t>(kGroupChunk, last - f + 1);
+            ch.task_base = base;
+            ch.cell_off = (int64_t)(a - g.r0) * ld + (f - g.c0);
+            base += (ch.n + kWave - 1) / kWave;
+            chunks.push_back(ch);
+        }
     }
-    if (ntile) {
-        SWG_HIP(hipMemcpyAsync(B.tile_col.p, B.h_tile_col.data(), ntile * kWave * 4, hipMemcpyHostToDevice, c->stream));
-        SWG_HIP(hipMemcpyAsync(B.tile_maxlen.p, B.h_tile_maxlen.data(), ntile * 4, hipMemcpyHostToDevice, c->stream));
-    }
-    SWG_HIP(hipMemcpyAsync(B.desc.p, &desc, sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
-    // the host vectors must stay alive until the copies complete: callers
-    // synchronise on B.done before replanning this buffer set
+    plan.ntasks2 = base;
+    plan.nchunks = (int32_t)chunks.size();
+}
+
+template <typename T>
+int upload(hipStream_t st, DevBuf<T>& d, const std::vector<T>& h)
+{
+    SWG_HIP(d.reserve(std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) SWG_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
     return SWG_OK;
 }
 
@@ -251,21 +297,190 @@ SeqSet make_seqset(const swg_ctx* c)
     return S;
 }
 
-TaskSet make_taskset(BlockBufs& B)
+template <typename T>
+void append_key(std::vector<unsigned char>& k, const T& v)
 {
-    TaskSet T;
-    T.task_row = B.task_row.p; T.task_tile = B.task_tile.p;
-    T.tile_col = B.tile_col.p; T.tile_maxlen = B.tile_maxlen.p;
-    T.desc = B.desc.p; T.queue = B.queue.p;
-    return T;
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
 }
 
-int pick_strip(swg_ctx* c)
+// Plan, upload and launch one block (or the rows of a sample) with variant v.
+int run_variant(swg_ctx* c, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld, int32_t* dev_stats,
+                Plan& plan)
 {
-    if (c->schedule == SWG_SCHED_PLAIN) return 16;
-    if (c->strip_rows) return c->strip_rows;
-    if (c->tuned_strip) return c->tuned_strip;
-    return 16;
+    int s1, s2, n1, n2;
+    variant_strips(c, v, &s1, &s2);
+    int rc = slots_for(c, v, &n1, &n2);
+    if (rc) return rc;
+    int want = std::max(n1, n2);
+    rc = ensure_scratch(c, &want);
+    if (rc) return rc;
+    n1 = std::min(n1, want);
+    n2 = std::min(n2, want);
+
+    const hipStream_t st = c->stream;
+    BlockDesc desc[2] = {};
+    if (v == kVarI16) {
+        plan_tiles(c, g, true, kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
+        plan_chunks(c, g, B.h_chunks, plan);
+        if ((rc = upload(st, B.s_task_row, B.h_task_row)) || (rc = upload(st, B.s_task_tile, B.h_task_tile)) ||
+            (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)) ||
+            (rc = upload(st, B.chunks, B.h_chunks)))
+            return rc;
+        const size_t nt2 = std::max<int32_t>(plan.ntasks2, 1);
+        SWG_HIP(B.task_row.reserve(nt2)); SWG_HIP(B.task_tile.reserve(nt2)); SWG_HIP(B.task_rowlim.reserve(nt2));
+        SWG_HIP(B.tile_maxlen.reserve(nt2));
+        SWG_HIP(B.tile_col.reserve(nt2 * kWave)); SWG_HIP(B.tile_lim.reserve(nt2 * kWave));
+        SWG_HIP(B.best_cell.reserve(std::max<size_t>((size_t)(g.r1 - g.r0 + 1) * (size_t)(g.c1 - g.c0 + 1), 1)));
+        SWG_HIP(B.nchunks.reserve(1));
+        SWG_HIP(hipMemcpyAsync(B.nchunks.p, &plan.nchunks, sizeof(int32_t), hipMemcpyHostToDevice, st));
+        desc[0].ntasks = plan.ntasks2;
+        desc[1].ntasks = plan.ntasks;
+        desc[1].r0 = g.r0; desc[1].c0 = g.c0; desc[1].ld = g.c1 - g.c0 + 1;
+    } else {
+        plan_tiles(c, g, v != kVarPlain, kWave, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
+        if ((rc = upload(st, B.task_row, B.h_task_row)) || (rc = upload(st, B.task_tile, B.h_task_tile)) ||
+            (rc = upload(st, B.tile_col, B.h_tile_col)) || (rc = upload(st, B.tile_maxlen, B.h_tile_maxlen)))
+            return rc;
+        desc[0].ntasks = plan.ntasks;
+    }
+    desc[0].mirror = g.diag_lower ? 1 : 0;
+    desc[0].r0 = g.r0; desc[0].c0 = g.c0; desc[0].ld = ld;
+    desc[0].out = dev_out; desc[0].stats = dev_stats;
+    SWG_HIP(B.desc.reserve(2));
+    SWG_HIP(B.queue.reserve(64));
+    SWG_HIP(hipMemcpyAsync(B.desc.p, desc, sizeof(desc), hipMemcpyHostToDevice, st));
+
+    const SeqSet S = make_seqset(c);
+    Scratch W;
+    W.bnd = c->bnd.p;
+    W.bnd_stride = c->bnd_stride;
+    TaskSet T{};
+    T.task_row = B.task_row.p; T.task_tile = B.task_tile.p; T.tile_col = B.tile_col.p;
+    T.tile_maxlen = B.tile_maxlen.p; T.desc = B.desc.p; T.queue = B.queue.p;
+    T.task_rowlim = nullptr; T.tile_lim = nullptr;
+    ScoreTaskSet ST{};
+    GroupSet G{};
+    const int group_grid = c->num_cu * 4;
+    if (v == kVarI16) {
+        ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
+        ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
+        G.chunks = B.chunks.p; G.nchunks = B.nchunks.p; G.best_cell = B.best_cell.p;
+        G.task_row = B.task_row.p; G.task_tile = B.task_tile.p; G.task_rowlim = B.task_rowlim.p;
+        G.tile_col = B.tile_col.p; G.tile_lim = B.tile_lim.p; G.tile_maxlen = B.tile_maxlen.p;
+        T.queue = B.queue.p + 16;
+        T.task_rowlim = B.task_rowlim.p; T.tile_lim = B.tile_lim.p;
+    }
+
+    // capture (or re-capture) before the timing events so that host-side
+    // capture time is not counted as device time
+    if (plan.ntasks > 0 && v != kVarPlain) {
+        // every pointer and scalar baked into the graph is part of its key:
+        // re-capture whenever one changed (new sequences, grown buffers)
+        std::vector<unsigned char> key;
+        append_key(key, S); append_key(key, T); append_key(key, W); append_key(key, ST); append_key(key, G);
+        append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
+        if (!B.graph[v] || key != B.graph_key[v]) {
+            if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
+            hipGraph_t gr = nullptr;
+            SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_fill_i32(B.queue.p, 0, 32, st);
+            if (e == hipSuccess) {
+                if (v == kVarI32) {
+                    e = launch_sw(S, T, W, s1, true, n1, st);
+                } else {
+                    e = launch_score16(S, ST, W, s1, n1, st);
+                    if (e == hipSuccess) e = launch_group(G, group_grid, st);
+                    if (e == hipSuccess) e = launch_sw(S, T, W, s2, true, n2, st);
+                }
+            }
+            hipError_t e2 = hipStreamEndCapture(st, &gr);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return fail(SWG_E_DEVICE, std::string("graph capture failed: ") +
+                                              hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            hipError_t e3 = hipGraphInstantiate(&B.graph[v], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e3 != hipSuccess) { B.graph[v] = nullptr; return fail(SWG_E_DEVICE, "hipGraphInstantiate failed"); }
+            B.graph_key[v] = key;
+        }
+    }
+    SWG_HIP(hipEventRecord(c->ev_k0, st));
+    if (plan.ntasks > 0) {
+        if (v == kVarPlain) SWG_HIP(launch_sw(S, T, W, s1, false, n1, st));
+        else SWG_HIP(hipGraphLaunch(B.graph[v], st));
+    }
+    SWG_HIP(hipEventRecord(c->ev_k1, st));
+    SWG_HIP(hipEventRecord(B.done, st));
+    B.pending = true;
+    c->last_strip = v == kVarI16 ? s2 : s1;
+    c->last_variant = v;
+    return SWG_OK;
+}
+
+BlockBufs& next_bufs(swg_ctx* c, int* rc)
+{
+    BlockBufs& B = c->bufs[c->cur];
+    c->cur ^= 1;
+    *rc = SWG_OK;
+    if (B.pending) {
+        hipError_t e = hipEventSynchronize(B.done);
+        if (e != hipSuccess) *rc = fail(SWG_E_DEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+        B.pending = false;
+    }
+    return B;
+}
+
+// Autotuner: per (residue alphabet, block length bucket) the faster of the
+// int32 one-pass and the int16 two-pass variants, measured on a sample of
+// the first block that has the key (the sample's rows are recomputed by the
+// full run afterwards).
+int choose_variant(swg_ctx* c, const BlockGeom& g, void* dev_out, int64_t ld, int32_t* dev_stats, int* variant)
+{
+    if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I32 || !c->int16_ok) { *variant = kVarI32; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I16) { *variant = kVarI16; return SWG_OK; }
+    const std::vector<int32_t>& L = c->h_len;
+    double sum = 0.0;
+    int64_t cnt = 0;
+    for (int32_t i = g.r0; i <= g.r1; ++i) { sum += L[i]; ++cnt; }
+    for (int32_t i = g.c0; i <= g.c1; ++i) { sum += L[i]; ++cnt; }
+    const int bucket = cnt ? (int)std::lround(2.0 * std::log2(1.0 + sum / (double)cnt)) : 0;
+    const int64_t key = ((int64_t)c->ncode << 32) | (int64_t)bucket;
+    for (auto& kv : c->tuned)
+        if (kv.first == key) { *variant = kv.second; return SWG_OK; }
+    // sample: the last k rows against the columns that precede them (same
+    // alignment as in the block for both block kinds)
+    const int32_t rows = g.r1 - g.r0 + 1;
+    const int32_t k = std::max<int32_t>(64, rows / 16);
+    BlockGeom s;
+    s.diag_lower = false;
+    s.r1 = g.r1; s.r0 = std::max(g.r0, g.r1 - k + 1);
+    s.c0 = g.c0; s.c1 = g.diag_lower ? s.r0 - 1 : g.c1;
+    if (k >= rows || s.c1 < s.c0 || (double)(s.r1 - s.r0 + 1) * (s.c1 - s.c0 + 1) < 65536.0) {
+        *variant = kVarI16;  // too small to measure: the int16 pass is never slower on short blocks
+        return SWG_OK;
+    }
+    double rate[kNumVariants] = {0.0, 0.0, 0.0};
+    for (int v : {kVarI32, kVarI16}) {
+        for (int rep = 0; rep < 2; ++rep) {  // first run captures the graph
+            int rc;
+            BlockBufs& B = next_bufs(c, &rc);
+            if (rc) return rc;
+            Plan plan;
+            rc = run_variant(c, B, s, v, dev_out, ld, dev_stats, plan);
+            if (rc) return rc;
+            SWG_HIP(hipStreamSynchronize(c->stream));
+            B.pending = false;
+            float ms = 0.0f;
+            SWG_HIP(hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1));
+            rate[v] = plan.cells / std::max(1e-3, (double)ms);
+        }
+    }
+    *variant = rate[kVarI16] >= rate[kVarI32] ? kVarI16 : kVarI32;
+    c->tuned.push_back({key, *variant});
+    return SWG_OK;
 }
 
 // Enqueue one block on the context stream into device buffers.
@@ -275,75 +490,25 @@ int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, in
     if (!c->res.p) return fail(SWG_E_ARGUMENT, "swg_set_sequences has not been called");
     if (r0 < 0 || c0 < 0 || r1 < r0 - 1 || c1 < c0 - 1 || r1 >= c->nseq || c1 >= c->nseq)
         return fail(SWG_E_ARGUMENT, "block range outside the sequence set");
-    const bool diag_lower = is_diag && c->p.diag_mode == SWG_DIAG_LOWER_MIRROR;
-    if (diag_lower && (r0 != c0 || r1 != c1))
+    BlockGeom g;
+    g.r0 = r0; g.r1 = r1; g.c0 = c0; g.c1 = c1;
+    g.diag_lower = is_diag && c->p.diag_mode == SWG_DIAG_LOWER_MIRROR;
+    if (g.diag_lower && (r0 != c0 || r1 != c1))
         return fail(SWG_E_ARGUMENT, "a diagonal block must have equal row and column ranges");
     if (ld < (int64_t)(c1 - c0 + 1)) return fail(SWG_E_ARGUMENT, "out_ld smaller than the block width");
-
-    const bool persistent = c->schedule == SWG_SCHED_PERSISTENT;
-    const int strip = pick_strip(c);
-    int rc = ensure_slots(c, strip);
-    if (rc) return rc;
-
-    BlockBufs& B = c->bufs[c->cur];
-    c->cur ^= 1;
-    if (B.pending) { SWG_HIP(hipEventSynchronize(B.done)); B.pending = false; }
-
-    Plan plan;
-    plan_block(c, B, r0, r1, c0, c1, diag_lower, persistent, plan);
-    BlockDesc desc{};
-    desc.ntasks = (int32_t)B.h_task_row.size();
-    desc.mirror = diag_lower ? 1 : 0;
-    desc.r0 = r0; desc.c0 = c0; desc.ld = ld;
-    desc.out = dev_out; desc.stats = dev_stats;
-    rc = upload_plan(c, B, desc);
-    if (rc) return rc;
-
-    const SeqSet S = make_seqset(c);
-    const TaskSet T = make_taskset(B);
-    Scratch W; W.bnd = c->bnd.p; W.bnd_stride = c->bnd_stride;
-
-    SWG_HIP(hipEventRecord(c->ev_k0, c->stream));
-    if (desc.ntasks > 0) {
-        if (c->p.mode == SWG_MODE_NW_GLOBAL) {
-            SWG_HIP(launch_nw(S, T, W, strip, persistent, c->nslots, c->stream));
-        } else if (persistent) {
-            // the graph bakes every pointer and scalar of the launch: re-capture
-            // whenever one of them changed (new sequences, grown buffers, other strip)
-            unsigned char key[sizeof(B.graph_key)];
-            std::memset(key, 0, sizeof(key));
-            unsigned char* kp = key;
-            std::memcpy(kp, &S, sizeof(S)); kp += sizeof(S);
-            std::memcpy(kp, &T, sizeof(T)); kp += sizeof(T);
-            std::memcpy(kp, &W, sizeof(W)); kp += sizeof(W);
-            std::memcpy(kp, &strip, sizeof(int)); kp += sizeof(int);
-            std::memcpy(kp, &c->nslots, sizeof(int));
-            if (!B.graph || std::memcmp(key, B.graph_key, sizeof(key)) != 0) {
-                if (B.graph) { (void)hipGraphExecDestroy(B.graph); B.graph = nullptr; }
-                hipGraph_t g = nullptr;
-                SWG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t e1 = launch_fill_i32(B.queue.p, 0, 1, c->stream);
-                hipError_t e2 = launch_sw(S, T, W, strip, true, c->nslots, c->stream);
-                hipError_t e3 = hipStreamEndCapture(c->stream, &g);
-                if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return fail(SWG_E_DEVICE, "graph capture of the block kernel failed");
-                }
-                hipError_t e4 = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (e4 != hipSuccess) { B.graph = nullptr; return fail(SWG_E_DEVICE, "hipGraphInstantiate failed"); }
-                std::memcpy(B.graph_key, key, sizeof(key));
-            }
-            SWG_HIP(hipGraphLaunch(B.graph, c->stream));
-        } else {
-            SWG_HIP(launch_sw(S, T, W, strip, false, c->nslots, c->stream));
-        }
+    if (r1 < r0 || c1 < c0) {
+        if (pairs_done) *pairs_done = 0;
+        return SWG_OK;
     }
-    SWG_HIP(hipEventRecord(c->ev_k1, c->stream));
-    SWG_HIP(hipEventRecord(B.done, c->stream));
-    B.pending = true;
+    int variant = kVarI32;
+    int rc = choose_variant(c, g, dev_out, ld, dev_stats, &variant);
+    if (rc) return rc;
+    BlockBufs& B = next_bufs(c, &rc);
+    if (rc) return rc;
+    Plan plan;
+    rc = run_variant(c, B, g, variant, dev_out, ld, dev_stats, plan);
+    if (rc) return rc;
     c->last_cells = plan.cells;
-    c->last_strip = strip;
     if (pairs_done) *pairs_done = plan.pairs;
     return SWG_OK;
 }
@@ -364,7 +529,8 @@ int swg_create(swg_ctx** out, const swg_params* p)
         return fail(SWG_E_ARGUMENT, "bad alignment_type");
     if (p->tdistance != SWG_TDIST_INT16 && p->tdistance != SWG_TDIST_DOUBLE) return fail(SWG_E_ARGUMENT, "bad tdistance");
     if (p->diag_mode != SWG_DIAG_LOWER_MIRROR && p->diag_mode != SWG_DIAG_FULL) return fail(SWG_E_ARGUMENT, "bad diag_mode");
-    if (p->schedule != SWG_SCHED_PERSISTENT && p->schedule != SWG_SCHED_PLAIN) return fail(SWG_E_ARGUMENT, "bad schedule");
+    if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_I16)
+        return fail(SWG_E_ARGUMENT, "bad schedule");
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
         return fail(SWG_E_UNSUPPORTED, "only PercentIdentity is implemented for nucleic alignments");
     if (p->mode == SWG_MODE_NW_GLOBAL)
@@ -383,6 +549,13 @@ int swg_create(swg_ctx** out, const swg_params* p)
     c->gap_ext = (int32_t)p->gap_ext;
     c->count_similarity = p->alignment_type == SWG_ALIGN_PROTEIN;
     c->schedule = p->schedule;
+    {
+        // the int16 score pass holds score + gap_open and the pad score in int16
+        int maxabs = 0;
+        for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k) maxabs = std::max(maxabs, std::abs(c->iscores[k]));
+        c->int16_ok = maxabs + std::abs(c->gap_open) <= 8192 && std::abs(c->gap_ext) <= 8192 && c->gap_open >= 0 &&
+                      c->gap_ext >= 0;
+    }
     c->device = p->device;
 
     auto bail = [&](hipError_t e, const char* what) {
@@ -412,6 +585,7 @@ int swg_set_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offset
     SWG_HIP(hipSetDevice(c->device));
     SWG_HIP(hipStreamSynchronize(c->stream));
     for (auto& B : c->bufs) B.pending = false;
+    c->tuned.clear();
     // residue codes: distinct symbols of the set, in ASCII order, + pad
     bool seen[256] = {false};
     int32_t maxlen = 0;
@@ -480,7 +654,7 @@ int swg_set_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offset
     c->pad_off = pad_off;
     c->nseq = n;
     c->h_len.assign(lens.begin(), lens.begin() + n);
-    if (maxlen != c->maxlen) { c->bnd.release(); c->nslots = 0; }
+    if (maxlen != c->maxlen) { c->bnd.release(); c->bnd_slots = 0; }
     c->maxlen = maxlen;
     return SWG_OK;
 }
@@ -530,20 +704,25 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
 int swg_set_schedule(swg_ctx* c, int32_t schedule, int32_t strip_rows)
 {
     if (!c) return fail(SWG_E_ARGUMENT, "null argument");
-    if (schedule != SWG_SCHED_PERSISTENT && schedule != SWG_SCHED_PLAIN) return fail(SWG_E_ARGUMENT, "bad schedule");
+    if (schedule < SWG_SCHED_PERSISTENT || schedule > SWG_SCHED_PERSISTENT_I16)
+        return fail(SWG_E_ARGUMENT, "bad schedule");
     if (strip_rows != 0 && strip_rows != 8 && strip_rows != 16 && strip_rows != 32)
-        return fail(SWG_E_ARGUMENT, "strip_rows must be 0 (autotune), 8, 16 or 32");
+        return fail(SWG_E_ARGUMENT, "strip_rows must be 0 (default), 8, 16 or 32");
     c->schedule = schedule;
     c->strip_rows = strip_rows;
     return SWG_OK;
 }
 
-int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t* strip_rows)
+int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t* strip_rows, int32_t* variant)
 {
     if (!c) return fail(SWG_E_ARGUMENT, "null argument");
     if (kernel_ms) *kernel_ms = c->last_kernel_ms;
     if (cells) *cells = c->last_cells;
     if (strip_rows) *strip_rows = c->last_strip;
+    if (variant)
+        *variant = c->last_variant == kVarPlain ? SWG_SCHED_PLAIN
+                 : c->last_variant == kVarI32   ? SWG_SCHED_PERSISTENT_I32
+                                                : SWG_SCHED_PERSISTENT_I16;
     return SWG_OK;
 }
 
@@ -553,8 +732,12 @@ void swg_destroy(swg_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& B : c->bufs) {
-        if (B.graph) (void)hipGraphExecDestroy(B.graph);
+        for (auto& gexec : B.graph)
+            if (gexec) (void)hipGraphExecDestroy(gexec);
         B.task_row.release(); B.task_tile.release(); B.tile_col.release(); B.tile_maxlen.release();
+        B.task_rowlim.release(); B.tile_lim.release();
+        B.s_task_row.release(); B.s_task_tile.release(); B.s_tile_col.release(); B.s_tile_maxlen.release();
+        B.best_cell.release(); B.chunks.release(); B.nchunks.release();
         B.queue.release(); B.desc.release();
         if (B.done) (void)hipEventDestroy(B.done);
     }

@@ -41,10 +41,54 @@ struct TaskSet {
     const int32_t* task_row;
     const int32_t* task_tile;
     const int32_t* tile_col;
-    const int32_t* tile_maxlen;   // max length over the tile's lanes
+    const int32_t* tile_maxlen;   // max (limited) length over the tile's lanes
     const BlockDesc* desc;
     int32_t* queue;               // [0]: next task (persistent schedule)
+    // bounded re-run (stats pass of the two-pass schedule); null = unbounded
+    const int32_t* task_rowlim;   // DP rows 1..rowlim only
+    const int32_t* tile_lim;      // per lane: DP columns 1..lim only
 };
+
+// Score-only pass (int16 x 2 packed): task t aligns s1 = task_row[t] against
+// the 128 sequences of tile task_tile[t] (lane l: tile_col[tile*128 + l] in
+// the low half, tile_col[tile*128 + 64 + l] in the high half) and writes, per
+// pair, the first-maximum cell (bi << 16) | bj into best_cell at the pair's
+// output position, or (La << 16) | Lb when the score may have saturated.
+struct ScoreTaskSet {
+    const int32_t* task_row;
+    const int32_t* task_tile;
+    const int32_t* tile_col;
+    const int32_t* tile_maxlen;
+    const BlockDesc* desc;        // ntasks, r0, c0, ld (of best_cell)
+    int32_t* queue;
+    uint32_t* best_cell;
+};
+
+// Grouping of the score pass's best cells into bounded re-run tasks: chunk c
+// covers partners first_col .. first_col+n-1 of row `row` (contiguous in
+// best_cell at offset cell_off) and emits ceil(n/64) tasks from task_base.
+struct GroupChunk {
+    int32_t row;
+    int32_t first_col;
+    int32_t n;
+    int32_t task_base;
+    int64_t cell_off;
+};
+
+struct GroupSet {
+    const GroupChunk* chunks;
+    const int32_t* nchunks;       // device-side count (the launch grid is fixed)
+    const uint32_t* best_cell;
+    int32_t* task_row;
+    int32_t* task_tile;
+    int32_t* task_rowlim;
+    int32_t* tile_col;
+    int32_t* tile_lim;
+    int32_t* tile_maxlen;
+};
+
+constexpr int kGroupChunk = 2048;  // partners sorted together
+constexpr int kGroupSpan = 256;    // partners sharing a row bound
 
 struct Scratch {
     int4* bnd;                    // per-slot row-boundary buffer
@@ -58,8 +102,13 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
                      bool persistent, int nslots, hipStream_t stream);
 hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
                      bool persistent, int nslots, hipStream_t stream);
+// Score-only int16x2 pass; strip_rows in {16, 32}.
+hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
+                          int nslots, hipStream_t stream);
+hipError_t launch_group(const GroupSet& G, int nchunks, hipStream_t stream);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
+int score16_blocks_per_cu(int strip_rows);
 // Fill kernels (graph-capturable replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_fill_bytes_zero(void* p, int64_t nbytes, hipStream_t stream);

@@ -103,8 +103,14 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
         const int tile = uni(T.task_tile[t]);
         const int maxLb = uni(T.tile_maxlen[tile]);
         const int b = T.tile_col[(int64_t)tile * kWave + lane];
-        const int La = uni(S.len[a]);
-        const int Lb = b >= 0 ? S.len[b] : 0;
+        const int La0 = uni(S.len[a]);
+        const int Lb0 = b >= 0 ? S.len[b] : 0;
+        // bounded re-run: only the rectangle up to the known first-maximum
+        // cell is recomputed; every cell in it depends only on cells above
+        // and to the left, so its values are exact, and no cell outside the
+        // rectangle precedes the first maximum in row-major order
+        const int La = T.task_rowlim ? min(La0, uni(T.task_rowlim[t])) : La0;
+        const int Lb = T.tile_lim ? min(Lb0, T.tile_lim[(int64_t)tile * kWave + lane]) : Lb0;
         const uint8_t* s1 = S.res + S.off[a];
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (b >= 0 ? S.off[b] : S.pad_off));
         const int ncols = (maxLb + 3) & ~3;
@@ -192,9 +198,252 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
 
         if (b >= 0) {
             int score = best, Tb = bestT;
-            if (La == 0 || Lb == 0) { score = INT_MIN; Tb = 0; }
+            if (La0 == 0 || Lb0 == 0) { score = INT_MIN; Tb = 0; }
             write_distance(S, *T.desc, a, b, score, Tb);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Score-only pass, two pairs per lane in packed int16 halves.
+//
+// Same strip structure and query profile as sw_kernel, but only the score
+// recurrence (SmithWatermanGotoh.cs:154-190) and the first strict maximum
+// (:211-214) are evaluated: per row, the packed row maximum and the column of
+// its first occurrence (v_pk_* ops, one instruction for two cells).  The
+// diagonal add saturates, so a score that would exceed 32767 pins at 32767 and
+// the pair is sent to the exact int32 re-run with unbounded limits.  Every
+// other operation only lowers values, and all values stay >= -(gap_open+16384).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 v2(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ uint32_t u2(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t splat16(int x) { return ((uint32_t)x & 0xFFFFu) * 0x10001u; }
+
+constexpr int kPad16 = -16384;
+constexpr int kSat16 = 32766;  // best >= this: the pair may have saturated
+
+template <int R>
+__device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, const uint8_t* s1, int La, int i0,
+                                                int lane)
+{
+    const int ncode = S.ncode, pad = ncode - 1;
+    short* lp = reinterpret_cast<short*>(lprof);
+    for (int idx = lane; idx < ncode * R; idx += kWave) {
+        const int c2 = idx / R, r = idx % R;
+        const int row = i0 + r;
+        const int c1 = row < La ? (int)s1[row] : pad;
+        const int v = S.prof[c1 * ncode + c2].x;
+        lp[c2 * (R + 8) + r] = (short)(v < kPad16 ? kPad16 : v);
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
+{
+    static_assert(R % 8 == 0, "strip height must be a multiple of 8");
+    constexpr int PS = R / 8 + 1;  // int4 per residue code (R int16 + 8 int16 of padding)
+    __shared__ int4 lprof[kMaxCodes * PS];
+
+    const int lane = threadIdx.x;
+    int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
+    const BlockDesc* D = T.desc;
+    const int ntasks = D->ntasks;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int pad = S.ncode - 1;
+    const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e)), Z2 = v2(0u);
+    const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
+
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(T.queue, 1);
+        const int t = uni(v);
+        if (t >= ntasks) break;
+
+        const int a = uni(T.task_row[t]);
+        const int tile = uni(T.task_tile[t]);
+        const int maxLb = uni(T.tile_maxlen[tile]);
+        const int bA = T.tile_col[(int64_t)tile * (2 * kWave) + lane];
+        const int bB = T.tile_col[(int64_t)tile * (2 * kWave) + kWave + lane];
+        const int La = uni(S.len[a]);
+        const int LbA = bA >= 0 ? S.len[bA] : 0;
+        const int LbB = bB >= 0 ? S.len[bB] : 0;
+        const uint8_t* s1 = S.res + S.off[a];
+        const uint32_t* wA = reinterpret_cast<const uint32_t*>(S.res + (bA >= 0 ? S.off[bA] : S.pad_off));
+        const uint32_t* wB = reinterpret_cast<const uint32_t*>(S.res + (bB >= 0 ? S.off[bB] : S.pad_off));
+        const int ncols = (maxLb + 3) & ~3;
+
+        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2((int)T0, (int)NEG2);
+
+        int bestA = INT_MIN, bestB = INT_MIN;
+        uint32_t cellA = 0, cellB = 0;
+        for (int i0 = 0; i0 < La; i0 += R) {
+            __syncthreads();
+            stage_profile16<R>(lprof, S, s1, La, i0, lane);
+            __syncthreads();
+
+            uint32_t tr[R], Hr[R], rb[R], rj[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = NEG2; rb[r] = M1; rj[r] = 0u; }
+            uint32_t dtb = T0;
+
+            uint32_t nwa = wA[0], nwb = wB[0];
+            int2 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
+            int2 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
+            for (int j = 1; j <= ncols; j += 4) {
+                const uint32_t cwa = nwa, cwb = nwb;
+                int2 cb[4] = {nb0, nb1, nb2, nb3};
+                nwa = wA[(j + 3) >> 2];
+                nwb = wB[(j + 3) >> 2];
+                const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
+                nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int jj = j + q;
+                    int ca = (int)((cwa >> (8 * q)) & 0xFFu);
+                    int cc = (int)((cwb >> (8 * q)) & 0xFFu);
+                    ca = jj > LbA ? pad : ca;
+                    cc = jj > LbB ? pad : cc;
+                    uint32_t tu = (uint32_t)cb[q].x, Gu = (uint32_t)cb[q].y;
+                    uint32_t dt = dtb;
+                    dtb = tu;
+                    const uint32_t JJ = splat16(jj);
+                    const int4* pa = lprof + ca * PS;
+                    const int4* pc = lprof + cc * PS;
+#pragma unroll
+                    for (int r8 = 0; r8 < R / 8; ++r8) {
+                        const int4 xa = pa[r8], xc = pc[r8];
+                        const uint32_t wa4[4] = {(uint32_t)xa.x, (uint32_t)xa.y, (uint32_t)xa.z, (uint32_t)xa.w};
+                        const uint32_t wc4[4] = {(uint32_t)xc.x, (uint32_t)xc.y, (uint32_t)xc.z, (uint32_t)xc.w};
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int r = 8 * r8 + k;
+                            // rows r of pair A (low half) and pair B (high half)
+                            const uint32_t sp = __builtin_amdgcn_perm(wc4[k >> 1], wa4[k >> 1],
+                                                                      (k & 1) ? 0x07060302u : 0x05040100u);
+                            const s16x2 G = __builtin_elementwise_max(v2(Gu) - E2, v2(tu));
+                            const s16x2 H = __builtin_elementwise_max(v2(Hr[r]) - E2, v2(tr[r]));
+                            const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G),
+                                                                      __builtin_elementwise_max(H, Z2));
+                            const s16x2 tn = V - O2;
+                            // V > row best (no wrap: row best >= -1, V >= 0): the sign of
+                            // (row best - V) per half, spread to a 16-bit mask, selects
+                            // the column with one bitfield insert
+                            const uint32_t d = u2(v2(rb[r]) - V);
+                            uint32_t mk;
+                            asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
+                            rj[r] = (mk & JJ) | (~mk & rj[r]);
+                            rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), V));
+                            dt = tr[r];
+                            tr[r] = u2(tn);
+                            Hr[r] = u2(H);
+                            tu = u2(tn);
+                            Gu = u2(G);
+                        }
+                    }
+                    bnd[(int64_t)jj * kWave + lane] = make_int2((int)tu, (int)Gu);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (i0 + r < La) {
+                    const int va = (short)(rb[r] & 0xFFFFu), vb = (short)(rb[r] >> 16);
+                    const uint32_t row = (uint32_t)(i0 + r + 1) << 16;
+                    if (va > bestA) { bestA = va; cellA = row | (rj[r] & 0xFFFFu); }
+                    if (vb > bestB) { bestB = vb; cellB = row | (rj[r] >> 16); }
+                }
+            }
+        }
+        if (bA >= 0) {
+            const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
+            T.best_cell[(int64_t)(a - D->r0) * D->ld + (bA - D->c0)] = c;
+        }
+        if (bB >= 0) {
+            const uint32_t c = (La == 0 || LbB == 0 || bestB >= kSat16) ? (((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
+            T.best_cell[(int64_t)(a - D->r0) * D->ld + (bB - D->c0)] = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grouping: for one row sequence, sort its partners by first-maximum row and,
+// inside spans of kGroupSpan, by first-maximum column, then emit 64-lane tiles
+// (longest first) with per-lane column limits and a per-task row limit.  The
+// bounded re-run's cost per task is rowlim x max(lim), so grouping pairs with
+// similar best cells keeps it close to the area the pairs actually need.
+__device__ __forceinline__ void bitonic_u64(uint64_t* key, int n, int seg, int tid, int nthr)
+{
+    for (int k = 2; k <= seg; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < n; i += nthr) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (k == seg) || ((i & k) == 0);
+                    const uint64_t x = key[i], y = key[ixj];
+                    if ((x > y) == up) { key[i] = y; key[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = max(v, __shfl_xor(v, s, kWave));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void group_kernel(GroupSet G)
+{
+    __shared__ uint64_t key[kGroupChunk];
+    const int nchunks = *G.nchunks;
+    const int tid = threadIdx.x;
+    for (int ci = blockIdx.x; ci < nchunks; ci += (int)gridDim.x) {
+    __syncthreads();
+    const GroupChunk C = G.chunks[ci];
+    int N = kWave;
+    while (N < C.n) N <<= 1;
+    for (int i = tid; i < N; i += 256) {
+        uint64_t k = ~0ull;
+        if (i < C.n) k = ((uint64_t)G.best_cell[C.cell_off + i] << 32) | (uint32_t)i;  // (bi, bj, partner)
+        key[i] = k;
+    }
+    __syncthreads();
+    bitonic_u64(key, N, N, tid, 256);
+    for (int i = tid; i < N; i += 256) {
+        const uint64_t k = key[i];
+        if (k != ~0ull) {
+            const uint64_t cell = k >> 32;
+            key[i] = ((cell & 0xFFFFull) << 48) | ((cell >> 16) << 32) | (k & 0xFFFFFFFFull);  // (bj, bi, partner)
+        }
+    }
+    __syncthreads();
+    bitonic_u64(key, N, N < kGroupSpan ? N : kGroupSpan, tid, 256);
+
+    const int ntiles = (C.n + kWave - 1) / kWave;
+    const int lane = tid & (kWave - 1);
+    for (int k = tid / kWave; k < ntiles; k += 256 / kWave) {
+        const uint64_t kk = key[(ntiles - 1 - k) * kWave + lane];
+        int col = -1, lim = 0, bi = 0;
+        if (kk != ~0ull) {
+            col = C.first_col + (int)(kk & 0xFFFFFFFFull);
+            lim = (int)((kk >> 48) & 0xFFFFull);
+            bi = (int)((kk >> 32) & 0xFFFFull);
+        }
+        const int task = C.task_base + k;
+        G.tile_col[(int64_t)task * kWave + lane] = col;
+        G.tile_lim[(int64_t)task * kWave + lane] = lim;
+        const int mlim = wave_max(lim), mbi = wave_max(bi);
+        if (lane == 0) {
+            G.task_row[task] = C.row;
+            G.task_tile[task] = task;
+            G.task_rowlim[task] = mbi;
+            G.tile_maxlen[task] = mlim;
+        }
+    }
     }
 }
 
@@ -227,6 +476,35 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 hipError_t launch_nw(const SeqSet&, const TaskSet&, const Scratch&, int, bool, int, hipStream_t)
 {
     return hipErrorNotSupported;
+}
+
+hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
+                          hipStream_t st)
+{
+    switch (strip_rows) {
+    case 16: hipLaunchKernelGGL(score16_kernel<16>, dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
+    case 32: hipLaunchKernelGGL(score16_kernel<32>, dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group(const GroupSet& G, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(group_kernel, dim3(grid), dim3(256), 0, st, G);
+    return hipGetLastError();
+}
+
+int score16_blocks_per_cu(int strip_rows)
+{
+    int nb = 0;
+    hipError_t err = hipErrorInvalidValue;
+    switch (strip_rows) {
+    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16>, kWave, 0); break;
+    case 32: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32>, kWave, 0); break;
+    default: break;
+    }
+    return err == hipSuccess ? nb : 0;
 }
 
 int sw_blocks_per_cu(int strip_rows, bool /*nw*/)

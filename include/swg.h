@@ -58,8 +58,11 @@ extern "C" {
 #define SWG_DIAG_LOWER_MIRROR 0
 #define SWG_DIAG_FULL         1
 /* schedules */
-#define SWG_SCHED_PERSISTENT 0   /* length-bucketed tiles, device work queue, hipGraph per block, autotuned */
-#define SWG_SCHED_PLAIN      1   /* one launch per block, natural order, fixed strip height, int32 */
+#define SWG_SCHED_PERSISTENT     0  /* length-bucketed tiles, device work queue, hipGraph per block,
+                                       variant (int32 one-pass / int16 two-pass) autotuned */
+#define SWG_SCHED_PLAIN          1  /* one launch per block, natural order, fixed strip height, int32 */
+#define SWG_SCHED_PERSISTENT_I32 2  /* persistent, forced int32 one-pass variant */
+#define SWG_SCHED_PERSISTENT_I16 3  /* persistent, forced int16 score pass + bounded int32 re-run */
 
 #define SWG_MATRIX_SIZE 127      /* ScoringMatrix.SIZE, Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:34 */
 #define SWG_MAX_SEQ_LEN 32767    /* gap-size arrays are short in the reference (SmithWatermanGotoh.cs:83-84) */
@@ -115,21 +118,24 @@ int  swg_compute_block(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                        void* out, int64_t out_ld, swg_pair_stats* stats, int64_t* pairs_done);
 
 /* Same as swg_compute_block, but `dev_out`/`dev_stats` are device pointers
- * and the call only enqueues work on the context's stream (`stream`, a
- * hipStream_t, or NULL for the context's own).  Used for device-resident
- * pipelines; completes at swg_synchronize(). */
+ * and the call only enqueues work on the context's own stream (the first block
+ * of a new length class also runs the autotuner's sample synchronously).
+ * Used for device-resident pipelines; completes at swg_synchronize(). */
 int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                               int32_t col_start, int32_t col_end, int32_t is_diagonal,
                               void* dev_out, int64_t out_ld, swg_pair_stats* dev_stats,
                               int64_t* pairs_done);
 int  swg_synchronize(swg_ctx* ctx);
 
-/* Schedule control (persistent vs plain, strip height, forced int mode);
- * strip_rows 0 = autotune. */
+/* Schedule control (SWG_SCHED_*) and strip height of the int32 alignment
+ * kernel (8, 16 or 32; 0 = default 16). */
 int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
-/* Timing of the last block's dominant kernel (HIP events on the stream it
- * ran on): milliseconds and DP cells. */
-int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows);
+/* Timing of the last block's kernels (HIP events on the context stream
+ * around its graph or launch): milliseconds, DP cells, strip height of the
+ * int32 alignment kernel and the variant that ran (1 plain, 2 int32 one-pass,
+ * 3 int16 score pass + bounded int32 re-run: the SWG_SCHED_* value). */
+int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows,
+                           int32_t* variant);
 
 const char* swg_last_error(void);
 void swg_destroy(swg_ctx* ctx);

@@ -77,8 +77,13 @@ def protein_set():
     return ca.SequenceSet.from_strings([s.seq(i) for i in range(len(s))] + extra)
 
 
-@pytest.mark.parametrize("schedule,strip", [(ca.SCHED_PERSISTENT, 16), (ca.SCHED_PLAIN, 16),
-                                            (ca.SCHED_PERSISTENT, 8), (ca.SCHED_PERSISTENT, 32)])
+SCHEDULES = [(ca.SCHED_PERSISTENT, 0), (ca.SCHED_PLAIN, 16), (ca.SCHED_PERSISTENT_I32, 8),
+             (ca.SCHED_PERSISTENT_I32, 16), (ca.SCHED_PERSISTENT_I32, 32), (ca.SCHED_PERSISTENT_I16, 16),
+             (ca.SCHED_PERSISTENT_I16, 32)]
+BLOCK_SCHEDULES = [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16]
+
+
+@pytest.mark.parametrize("schedule,strip", SCHEDULES)
 def test_offdiag_block_int16(dna_set, ednafull, schedule, strip):
     say("offdiag int16 schedule=%d strip=%d" % (schedule, strip))
     al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
@@ -93,7 +98,7 @@ def test_offdiag_block_int16(dna_set, ednafull, schedule, strip):
     al.close()
 
 
-@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN])
+@pytest.mark.parametrize("schedule", BLOCK_SCHEDULES)
 def test_diagonal_block_lower_mirror(dna_set, ednafull, schedule):
     say("diag lower-mirror schedule=%d" % schedule)
     al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
@@ -109,9 +114,10 @@ def test_diagonal_block_lower_mirror(dna_set, ednafull, schedule):
     al.close()
 
 
-def test_diagonal_block_full_swms(dna_set, ednafull):
-    say("diag full")
-    al = ca.Aligner(ednafull, 16, 4, diag_mode=ca.DIAG_FULL)
+@pytest.mark.parametrize("schedule", BLOCK_SCHEDULES)
+def test_diagonal_block_full_swms(dna_set, ednafull, schedule):
+    say("diag full schedule=%d" % schedule)
+    al = ca.Aligner(ednafull, 16, 4, diag_mode=ca.DIAG_FULL, schedule=schedule)
     al.set_sequences(dna_set)
     r0, r1 = 100, 170
     got, gst, pairs = al.compute_block(r0, r1, r0, r1, True, stats=True)
@@ -132,9 +138,10 @@ def test_double_distances(dna_set, ednafull):
     al.close()
 
 
-def test_protein_percent_similarity(protein_set, blosum62):
-    say("protein")
-    al = ca.Aligner(blosum62, 10, 1, alignment_type=ca.ALIGN_PROTEIN)
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16])
+def test_protein_percent_similarity(protein_set, blosum62, schedule):
+    say("protein schedule=%d" % schedule)
+    al = ca.Aligner(blosum62, 10, 1, alignment_type=ca.ALIGN_PROTEIN, schedule=schedule)
     al.set_sequences(protein_set)
     n = len(protein_set)
     got, gst, _ = al.compute_block(0, n - 1, 0, n - 1, True, stats=True)
@@ -187,10 +194,11 @@ def test_full_matrix_multi_rank_layout(dna_set, ednafull):
     al.close()
 
 
-def test_long_sequences(ednafull):
-    say("long sequences")
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16])
+def test_long_sequences(ednafull, schedule):
+    say("long sequences schedule=%d" % schedule)
     s = ca.synth_dna(13, 70, 3, 0.10, 0.02, 0, [1500, 2000])
-    al = ca.Aligner(ednafull, 16, 4)
+    al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
     al.set_sequences(s)
     got, gst, _ = al.compute_block(0, 5, 6, 69, False, stats=True)
     want, wst = oracle_block(s, 0, 5, 6, 69, False, ednafull, 16, 4)
@@ -207,4 +215,57 @@ def test_repeated_blocks_reuse_graph(dna_set, ednafull):
         got, gst, _ = al.compute_block(0, 30, 31, 120, False, stats=True)
         assert_same(got, want, gst, wst, "repeat")
         other, _, _ = al.compute_block(40, 60, 0, 39, False)
+    al.close()
+
+
+def test_int16_saturation_reruns_exactly():
+    # match +100: identical 400-residue sequences score 40000 > 32767, so the
+    # int16 score pass saturates and those pairs must be re-run in int32
+    say("saturation")
+    M = np.zeros((127, 127), dtype=np.float32)
+    for x in b"ACGT":
+        for y in b"ACGT":
+            M[x, y] = 100 if x == y else -60
+    s = ca.synth_dna(14, 40, 2, 0.02, 0.005, 0, [300, 420])
+    seqs = ca.SequenceSet.from_strings([s.seq(i) for i in range(40)] + [s.seq(0), s.seq(0), b"ACGT" * 100])
+    for sched in (ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_I32):
+        al = ca.Aligner(M, 150, 20, schedule=sched)
+        al.set_sequences(seqs)
+        n = len(seqs)
+        got, gst, _ = al.compute_block(0, n - 1, 0, n - 1, True, stats=True)
+        want, wst = oracle_block(seqs, 0, n - 1, 0, n - 1, True, M, 150, 20)
+        assert wst[..., 0].max() > 32767
+        assert_same(got, want, gst, wst, "saturation sched=%d" % sched)
+        al.close()
+
+
+def test_many_partners_multi_chunk(ednafull):
+    # a row with more than 2048 partners is grouped in several chunks
+    say("multi-chunk")
+    s = ca.synth_dna(15, 2300, 30, 0.10, 0.02, 0, [8, 60])
+    for sched in (ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT):
+        al = ca.Aligner(ednafull, 16, 4, schedule=sched)
+        al.set_sequences(s)
+        got, gst, _ = al.compute_block(0, 7, 8, 2299, False, stats=True)
+        want, wst = oracle_block(s, 0, 7, 8, 2299, False, ednafull, 16, 4)
+        assert_same(got, want, gst, wst, "multi-chunk sched=%d" % sched)
+        got, gst, _ = al.compute_block(2200, 2299, 2200, 2299, True, stats=True)
+        want, wst = oracle_block(s, 2200, 2299, 2200, 2299, True, ednafull, 16, 4)
+        assert_same(got, want, gst, wst, "multi-chunk diag sched=%d" % sched)
+        al.close()
+
+
+def test_autotuned_blocks_match(ednafull):
+    # a block large enough for the autotuner's sample; both variants' results
+    # must agree with the oracle regardless of the choice
+    say("autotune")
+    s = ca.synth_dna(16, 700, 20, 0.10, 0.02, 0, [60, 200])
+    al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT)
+    al.set_sequences(s)
+    got, gst, _ = al.compute_block(0, 349, 350, 699, False, stats=True)
+    want, wst = oracle_block(s, 0, 349, 350, 699, False, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "autotune offdiag")
+    got, gst, _ = al.compute_block(350, 699, 350, 699, True, stats=True)
+    want, wst = oracle_block(s, 350, 699, 350, 699, True, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "autotune diag")
     al.close()
