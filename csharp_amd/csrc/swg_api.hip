@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -80,6 +81,14 @@ struct BlockBufs {
     std::vector<unsigned char> graph_key[kNumVariants];
 };
 
+// Autotuner record of one key (see choose_variant).
+struct TuneEntry {
+    int64_t key = 0;
+    int chosen = -1;
+    bool tried[kNumVariants] = {false, false, false};
+    double rate[kNumVariants] = {0.0, 0.0, 0.0};
+};
+
 }  // namespace
 
 struct swg_ctx {
@@ -120,8 +129,8 @@ struct swg_ctx {
 
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;  // 0 = autotune
-    // autotuner cache: (ncode, length bucket) -> variant
-    std::vector<std::pair<int64_t, int>> tuned;
+    // autotuner cache (entries are never removed while the deque lives)
+    std::deque<TuneEntry> tuned;
     double last_kernel_ms = 0.0, last_cells = 0.0;
     int32_t last_strip = 0;
     int32_t last_variant = 0;
@@ -432,54 +441,49 @@ BlockBufs& next_bufs(swg_ctx* c, int* rc)
     return B;
 }
 
-// Autotuner: per (residue alphabet, block length bucket) the faster of the
-// int32 one-pass and the int16 two-pass variants, measured on a sample of
-// the first block that has the key (the sample's rows are recomputed by the
-// full run afterwards).
-int choose_variant(swg_ctx* c, const BlockGeom& g, void* dev_out, int64_t ld, int32_t* dev_stats, int* variant)
+// Autotuner: per key (residue alphabet, mean-length class, block-size class)
+// the faster of the int32 one-pass and the int16 two-pass variants.  The
+// first blocks of a key run each candidate once, synchronously, on real work
+// (nothing is recomputed); the variant with the higher cells/ms wins.
+constexpr double kTuneMinCells = 5e10;  // smaller blocks run the default variant untimed
+
+int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
 {
-    if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I32 || !c->int16_ok) { *variant = kVarI32; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I16) { *variant = kVarI16; return SWG_OK; }
     const std::vector<int32_t>& L = c->h_len;
     double sum = 0.0;
     int64_t cnt = 0;
     for (int32_t i = g.r0; i <= g.r1; ++i) { sum += L[i]; ++cnt; }
     for (int32_t i = g.c0; i <= g.c1; ++i) { sum += L[i]; ++cnt; }
-    const int bucket = cnt ? (int)std::lround(2.0 * std::log2(1.0 + sum / (double)cnt)) : 0;
-    const int64_t key = ((int64_t)c->ncode << 32) | (int64_t)bucket;
-    for (auto& kv : c->tuned)
-        if (kv.first == key) { *variant = kv.second; return SWG_OK; }
-    // sample: the last k rows against the columns that precede them (same
-    // alignment as in the block for both block kinds)
-    const int32_t rows = g.r1 - g.r0 + 1;
-    const int32_t k = std::max<int32_t>(64, rows / 16);
-    BlockGeom s;
-    s.diag_lower = false;
-    s.r1 = g.r1; s.r0 = std::max(g.r0, g.r1 - k + 1);
-    s.c0 = g.c0; s.c1 = g.diag_lower ? s.r0 - 1 : g.c1;
-    if (k >= rows || s.c1 < s.c0 || (double)(s.r1 - s.r0 + 1) * (s.c1 - s.c0 + 1) < 65536.0) {
-        *variant = kVarI16;  // too small to measure: the int16 pass is never slower on short blocks
-        return SWG_OK;
+    const int len_class = cnt ? (int)std::lround(2.0 * std::log2(1.0 + sum / (double)cnt)) : 0;
+    const double pairs = (double)(g.r1 - g.r0 + 1) * (double)(g.c1 - g.c0 + 1) * (g.diag_lower ? 0.5 : 1.0);
+    const int size_class = (int)std::lround(std::log2(1.0 + pairs));
+    return ((int64_t)c->ncode << 40) | ((int64_t)len_class << 20) | (int64_t)size_class;
+}
+
+int choose_variant(swg_ctx* c, const BlockGeom& g, int* variant, TuneEntry** trial)
+{
+    *trial = nullptr;
+    if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I32 || !c->int16_ok) { *variant = kVarI32; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I16) { *variant = kVarI16; return SWG_OK; }
+    const int64_t key = tune_key(c, g);
+    TuneEntry* t = nullptr;
+    for (auto& e : c->tuned)
+        if (e.key == key) t = &e;
+    if (!t) {
+        c->tuned.emplace_back();
+        t = &c->tuned.back();
+        t->key = key;
     }
-    double rate[kNumVariants] = {0.0, 0.0, 0.0};
-    for (int v : {kVarI32, kVarI16}) {
-        for (int rep = 0; rep < 2; ++rep) {  // first run captures the graph
-            int rc;
-            BlockBufs& B = next_bufs(c, &rc);
-            if (rc) return rc;
-            Plan plan;
-            rc = run_variant(c, B, s, v, dev_out, ld, dev_stats, plan);
-            if (rc) return rc;
-            SWG_HIP(hipStreamSynchronize(c->stream));
-            B.pending = false;
-            float ms = 0.0f;
-            SWG_HIP(hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1));
-            rate[v] = plan.cells / std::max(1e-3, (double)ms);
+    if (t->chosen >= 0) { *variant = t->chosen; return SWG_OK; }
+    for (int v : {kVarI16, kVarI32}) {
+        if (!t->tried[v]) {
+            *variant = v;
+            *trial = t;
+            return SWG_OK;
         }
     }
-    *variant = rate[kVarI16] >= rate[kVarI32] ? kVarI16 : kVarI32;
-    c->tuned.push_back({key, *variant});
+    *variant = kVarI16;
     return SWG_OK;
 }
 
@@ -501,7 +505,8 @@ int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, in
         return SWG_OK;
     }
     int variant = kVarI32;
-    int rc = choose_variant(c, g, dev_out, ld, dev_stats, &variant);
+    TuneEntry* trial = nullptr;
+    int rc = choose_variant(c, g, &variant, &trial);
     if (rc) return rc;
     BlockBufs& B = next_bufs(c, &rc);
     if (rc) return rc;
@@ -509,6 +514,17 @@ int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, in
     rc = run_variant(c, B, g, variant, dev_out, ld, dev_stats, plan);
     if (rc) return rc;
     c->last_cells = plan.cells;
+    if (trial && plan.cells >= kTuneMinCells) {
+        SWG_HIP(hipStreamSynchronize(c->stream));
+        B.pending = false;
+        float ms = 0.0f;
+        SWG_HIP(hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1));
+        c->last_kernel_ms = ms;
+        TuneEntry& t = *trial;
+        t.tried[variant] = true;
+        t.rate[variant] = plan.cells / std::max(1e-3, (double)ms);
+        if (t.tried[kVarI16] && t.tried[kVarI32]) t.chosen = t.rate[kVarI16] >= t.rate[kVarI32] ? kVarI16 : kVarI32;
+    }
     if (pairs_done) *pairs_done = plan.pairs;
     return SWG_OK;
 }
