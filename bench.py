@@ -41,6 +41,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-rates", default="10,30,100,300,1000")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -64,6 +67,10 @@ def main():
     import torch
     import csharp_amd as ca
     from csharp_amd import workloads as WL
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve_bench(args)}), flush=True)
+        return
 
     t0 = time.time()
     seqs = WL.sequences("c3")
@@ -180,6 +187,9 @@ def main():
         parity = {"checked_pairs": int(len(A)), "mismatches": int((got != D).sum())}
         log("cpu baseline: %.2f GCUPS on %d threads; parity %s" % (cpu["value"], threads, parity))
 
+    al.close()
+    serving = None if args.no_serve else serve_bench(args)
+
     achieved = kernel_gcups * 1e9 * W_OPS_PER_CELL / 1e12
     peak = PEAK_INT32_OPS / 1e12
     # HBM bytes per launch of this kernel variant from the committed rocprofv3
@@ -229,9 +239,39 @@ def main():
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    al.close()
+
+
+def serve_bench(args):
+    """Serving workload: a device-resident panel of 20,000 DNA sequences
+    (lengths 300-1500) and Poisson query arrivals (lengths 50-2000) at each
+    rate; p50/p99 latency (submit -> distances on host) and GCUPS."""
+    import csharp_amd as ca
+    from csharp_amd import workloads as WL
+    from csharp_amd.serving import run_poisson
+
+    panel = WL.sequences("c5_panel")
+    qset = WL.sequences("c5_queries")
+    M, o, e = WL.scoring("c5_panel")
+    srv = ca.Server(panel, M, o, e, max_query_len=2000, max_batch=8, lanes=3, hbm_budget=2 << 30, stats=False)
+    # warm-up: scratch allocation and graph capture on every lane
+    warm = [srv.submit(qset.seq(i)) for i in range(6)]
+    for t in warm:
+        srv.wait(t)
+        srv.release(t)
+    out = {"panel": len(panel), "max_batch": 8, "lanes": 3, "results": []}
+    base = 6
+    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+        n = int(min(200, max(20, rate * 2)))
+        qs = [qset.seq((base + k) % len(qset)) for k in range(n)]
+        base += n
+        r = run_poisson(srv, qs, rate, seed=int(rate))
+        log("serving %.0f q/s: %s" % (rate, r))
+        out["results"].append(r)
+    srv.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -39,6 +39,8 @@ MATRIX_SIZE = 127
 ABI_SYMBOLS = (
     "swg_create", "swg_set_sequences", "swg_compute_block", "swg_compute_block_device",
     "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
+    "swg_serve_create", "swg_serve_submit", "swg_serve_wait", "swg_serve_times", "swg_serve_release",
+    "swg_serve_stats", "swg_serve_destroy",
 )
 HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
@@ -97,6 +99,17 @@ def lib():
         L.swg_last_error.restype = ctypes.c_char_p
         L.swg_destroy.argtypes = [vp]
         L.swg_destroy.restype = None
+        L.swg_serve_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(swg_params), vp, vp, i32, i32, i32, i32,
+                                       i64, i32]
+        L.swg_serve_submit.argtypes = [vp, vp, i32, vp, vp, ctypes.POINTER(i64)]
+        L.swg_serve_wait.argtypes = [vp, i64, i32, ctypes.POINTER(i32)]
+        L.swg_serve_times.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.swg_serve_release.argtypes = [vp, i64]
+        L.swg_serve_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.swg_serve_destroy.argtypes = [vp]
+        L.swg_serve_destroy.restype = None
         L._typed = True
     return L
 
@@ -335,3 +348,73 @@ def write_swms_block(path, project, rank, sysname, fasta, row_block, col_block, 
                                        arr, _ptr(data), data.size, data.itemsize)
     if rc != SWG_OK:
         raise SwgError(rc, "cannot write %s" % path)
+
+
+# ---------------------------------------------------------------- serving
+class Server:
+    """Continuous-batching query server over a device-resident panel
+    (swg_serve_* in include/swg.h).  submit() returns a ticket; the result
+    arrays are filled when the query completes."""
+
+    def __init__(self, panel, scores, gap_open, gap_ext, alignment_type=ALIGN_NUCLEIC,
+                 distance_fn=DIST_PERCENT_IDENTITY, tdistance=TDIST_INT16, max_query_len=2000, max_batch=16,
+                 lanes=2, hbm_budget=1 << 30, stats=True, device=0):
+        self._scores = np.ascontiguousarray(scores, dtype=np.float32).reshape(MATRIX_SIZE, MATRIX_SIZE)
+        p = swg_params()
+        p.mode, p.alignment_type, p.distance_fn = MODE_SW_LOCAL, alignment_type, distance_fn
+        p.tdistance, p.diag_mode, p.schedule = tdistance, DIAG_FULL, SCHED_PERSISTENT
+        p.gap_open, p.gap_ext = float(gap_open), float(gap_ext)
+        p.scores = self._scores.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        p.device = device
+        self.npanel = len(panel)
+        self.dtype = np.float64 if tdistance == TDIST_DOUBLE else np.int16
+        self.want_stats = stats
+        self._panel = panel
+        h = ctypes.c_void_p()
+        _check(lib().swg_serve_create(ctypes.byref(h), ctypes.byref(p), _ptr(panel.residues), _ptr(panel.offsets),
+                                      self.npanel, max_query_len, max_batch, lanes, int(hbm_budget), int(stats)))
+        self._h = h
+        self._bufs = {}
+
+    def submit(self, query):
+        q = np.frombuffer(bytes(query), dtype=np.uint8) if len(query) else np.zeros(1, np.uint8)
+        out = np.zeros(self.npanel, dtype=self.dtype)
+        st = np.zeros((self.npanel, 4), dtype=np.int32) if self.want_stats else None
+        t = ctypes.c_int64()
+        _check(lib().swg_serve_submit(self._h, _ptr(q), len(query), _ptr(out), _ptr(st), ctypes.byref(t)))
+        self._bufs[t.value] = (out, st, q)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        done = ctypes.c_int32()
+        _check(lib().swg_serve_wait(self._h, ticket, timeout_ms, ctypes.byref(done)))
+        return bool(done.value)
+
+    def result(self, ticket):
+        out, st, _ = self._bufs[ticket]
+        return out, st
+
+    def times(self, ticket):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().swg_serve_times(self._h, ticket, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def release(self, ticket):
+        _check(lib().swg_serve_release(self._h, ticket))
+        self._bufs.pop(ticket, None)
+
+    def stats(self):
+        n, cells, now = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().swg_serve_stats(self._h, ctypes.byref(n), ctypes.byref(cells), ctypes.byref(now)))
+        return n.value, cells.value, now.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swg_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

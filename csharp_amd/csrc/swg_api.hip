@@ -89,14 +89,26 @@ struct TuneEntry {
     double rate[kNumVariants] = {0.0, 0.0, 0.0};
 };
 
+// An execution lane: a stream, two alternating buffer sets and a boundary
+// scratch partition.  A context has one lane; a server has several lanes whose
+// batches run concurrently on the device, each on its share of the CUs.
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BlockBufs bufs[2];
+    int cur = 0;
+    DevBuf<int4> bnd;
+    int64_t bnd_stride = 0;
+    int bnd_slots = 0;
+    int share = 1;  // the lane's persistent grids use 1/share of the resident workgroups
+};
+
 }  // namespace
 
 struct swg_ctx {
     swg_params p{};
     int device = 0;
     int num_cu = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;
 
     int32_t iscores[SWG_MATRIX_SIZE * SWG_MATRIX_SIZE];
     int32_t gap_open = 0, gap_ext = 0;
@@ -107,6 +119,7 @@ struct swg_ctx {
     int32_t nseq = 0;
     int32_t maxlen = 0;
     std::vector<int32_t> h_len;
+    std::vector<int64_t> h_off;
     int16_t code_of[256];
     int32_t ncode = 0;  // incl. pad
     DevBuf<uint8_t> res;
@@ -115,17 +128,12 @@ struct swg_ctx {
     DevBuf<int2> prof;
     int64_t pad_off = 0;
 
-    // scratch: one boundary slot per resident workgroup of any variant
-    DevBuf<int4> bnd;
-    int64_t bnd_stride = 0;
-    int bnd_slots = 0;
+    Lane main;      // the context's own execution lane
+    int32_t slot_maxlen = 0;  // longest sequence the scratch must hold (>= maxlen)
 
     // outputs staged on device for the host-memory entry point
     DevBuf<uint8_t> out_stage;
     DevBuf<int32_t> stats_stage;
-
-    BlockBufs bufs[2];
-    int cur = 0;
 
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;  // 0 = autotune
@@ -151,28 +159,31 @@ void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
     *s2 = c->strip_rows ? c->strip_rows : 16;
 }
 
-int slots_for(const swg_ctx* c, int v, int* n1, int* n2)
+int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
 {
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
     int a = v == kVarI16 ? score16_blocks_per_cu(s1) : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? sw_blocks_per_cu(s2, false) : 0;
     if (a <= 0 || (v == kVarI16 && b <= 0)) return fail(SWG_E_DEVICE, "occupancy query failed");
-    *n1 = a * c->num_cu;
-    *n2 = b * c->num_cu;
+    *n1 = std::max(1, a * c->num_cu / ln.share);
+    *n2 = std::max(1, b * c->num_cu / ln.share);
     return SWG_OK;
 }
 
-// Boundary scratch for `slots` resident workgroups (HBM budget-capped).
-int ensure_scratch(swg_ctx* c, int* slots)
+// Boundary scratch of a lane for `slots` resident workgroups (HBM
+// budget-capped: 24 GB over all lanes).
+int ensure_scratch(swg_ctx* c, Lane& ln, int* slots)
 {
-    const int64_t stride = (int64_t)(c->maxlen + 16) * kWave;
-    const double budget = 24.0 * (1u << 30);
-    while (*slots > c->num_cu && (double)(*slots) * (double)stride * sizeof(int4) > budget) *slots -= c->num_cu;
-    if (c->bnd.p && c->bnd_stride == stride && c->bnd_slots >= *slots) return SWG_OK;
-    SWG_HIP(c->bnd.reserve((size_t)(*slots) * (size_t)stride));
-    c->bnd_stride = stride;
-    c->bnd_slots = *slots;
+    const int64_t stride = (int64_t)(std::max(c->maxlen, c->slot_maxlen) + 16) * kWave;
+    const double budget = 24.0 * (1u << 30) / ln.share;
+    const int step = std::max(1, c->num_cu / ln.share);
+    while (*slots > step && (double)(*slots) * (double)stride * sizeof(int4) > budget) *slots -= step;
+    if (ln.bnd.p && ln.bnd_stride == stride && ln.bnd_slots >= *slots) return SWG_OK;
+    if (ln.bnd_stride != stride) ln.bnd.release();
+    SWG_HIP(ln.bnd.reserve((size_t)(*slots) * (size_t)stride));
+    ln.bnd_stride = stride;
+    ln.bnd_slots = *slots;
     return SWG_OK;
 }
 
@@ -314,20 +325,20 @@ void append_key(std::vector<unsigned char>& k, const T& v)
 }
 
 // Plan, upload and launch one block (or the rows of a sample) with variant v.
-int run_variant(swg_ctx* c, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld, int32_t* dev_stats,
-                Plan& plan)
+int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld,
+                int32_t* dev_stats, Plan& plan)
 {
     int s1, s2, n1, n2;
     variant_strips(c, v, &s1, &s2);
-    int rc = slots_for(c, v, &n1, &n2);
+    int rc = slots_for(c, ln, v, &n1, &n2);
     if (rc) return rc;
     int want = std::max(n1, n2);
-    rc = ensure_scratch(c, &want);
+    rc = ensure_scratch(c, ln, &want);
     if (rc) return rc;
     n1 = std::min(n1, want);
     n2 = std::min(n2, want);
 
-    const hipStream_t st = c->stream;
+    const hipStream_t st = ln.stream;
     BlockDesc desc[2] = {};
     if (v == kVarI16) {
         plan_tiles(c, g, true, kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
@@ -362,15 +373,15 @@ int run_variant(swg_ctx* c, BlockBufs& B, const BlockGeom& g, int v, void* dev_o
 
     const SeqSet S = make_seqset(c);
     Scratch W;
-    W.bnd = c->bnd.p;
-    W.bnd_stride = c->bnd_stride;
+    W.bnd = ln.bnd.p;
+    W.bnd_stride = ln.bnd_stride;
     TaskSet T{};
     T.task_row = B.task_row.p; T.task_tile = B.task_tile.p; T.tile_col = B.tile_col.p;
     T.tile_maxlen = B.tile_maxlen.p; T.desc = B.desc.p; T.queue = B.queue.p;
     T.task_rowlim = nullptr; T.tile_lim = nullptr;
     ScoreTaskSet ST{};
     GroupSet G{};
-    const int group_grid = c->num_cu * 4;
+    const int group_grid = std::max(1, c->num_cu * 4 / ln.share);
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
@@ -415,12 +426,12 @@ int run_variant(swg_ctx* c, BlockBufs& B, const BlockGeom& g, int v, void* dev_o
             B.graph_key[v] = key;
         }
     }
-    SWG_HIP(hipEventRecord(c->ev_k0, st));
+    SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
         if (v == kVarPlain) SWG_HIP(launch_sw(S, T, W, s1, false, n1, st));
         else SWG_HIP(hipGraphLaunch(B.graph[v], st));
     }
-    SWG_HIP(hipEventRecord(c->ev_k1, st));
+    SWG_HIP(hipEventRecord(ln.ev1, st));
     SWG_HIP(hipEventRecord(B.done, st));
     B.pending = true;
     c->last_strip = v == kVarI16 ? s2 : s1;
@@ -428,10 +439,10 @@ int run_variant(swg_ctx* c, BlockBufs& B, const BlockGeom& g, int v, void* dev_o
     return SWG_OK;
 }
 
-BlockBufs& next_bufs(swg_ctx* c, int* rc)
+BlockBufs& next_bufs(Lane& ln, int* rc)
 {
-    BlockBufs& B = c->bufs[c->cur];
-    c->cur ^= 1;
+    BlockBufs& B = ln.bufs[ln.cur];
+    ln.cur ^= 1;
     *rc = SWG_OK;
     if (B.pending) {
         hipError_t e = hipEventSynchronize(B.done);
@@ -487,8 +498,8 @@ int choose_variant(swg_ctx* c, const BlockGeom& g, int* variant, TuneEntry** tri
     return SWG_OK;
 }
 
-// Enqueue one block on the context stream into device buffers.
-int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
+// Enqueue one block on a lane's stream into device buffers.
+int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
                   void* dev_out, int64_t ld, int32_t* dev_stats, int64_t* pairs_done)
 {
     if (!c->res.p) return fail(SWG_E_ARGUMENT, "swg_set_sequences has not been called");
@@ -508,17 +519,17 @@ int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, in
     TuneEntry* trial = nullptr;
     int rc = choose_variant(c, g, &variant, &trial);
     if (rc) return rc;
-    BlockBufs& B = next_bufs(c, &rc);
+    BlockBufs& B = next_bufs(ln, &rc);
     if (rc) return rc;
     Plan plan;
-    rc = run_variant(c, B, g, variant, dev_out, ld, dev_stats, plan);
+    rc = run_variant(c, ln, B, g, variant, dev_out, ld, dev_stats, plan);
     if (rc) return rc;
     c->last_cells = plan.cells;
     if (trial && plan.cells >= kTuneMinCells) {
-        SWG_HIP(hipStreamSynchronize(c->stream));
+        SWG_HIP(hipStreamSynchronize(ln.stream));
         B.pending = false;
         float ms = 0.0f;
-        SWG_HIP(hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1));
+        SWG_HIP(hipEventElapsedTime(&ms, ln.ev0, ln.ev1));
         c->last_kernel_ms = ms;
         TuneEntry& t = *trial;
         t.tried[variant] = true;
@@ -526,6 +537,105 @@ int enqueue_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, in
         if (t.tried[kVarI16] && t.tried[kVarI32]) t.chosen = t.rate[kVarI16] >= t.rate[kVarI32] ? kVarI16 : kVarI32;
     }
     if (pairs_done) *pairs_done = plan.pairs;
+    return SWG_OK;
+}
+
+// Residue coding + device upload of a sequence set, optionally followed by
+// `nslots` empty slots of capacity `slot_cap` residues (serving queries are
+// written into them later).  With `matrix_symbols`, every symbol that has a
+// non-zero score also gets a code so that later queries can use it.
+int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, int32_t n, int32_t nslots,
+                   int32_t slot_cap, bool matrix_symbols)
+{
+    if (!c || n < 0 || nslots < 0 || slot_cap < 0 || (n > 0 && (!residues || !offsets)))
+        return fail(SWG_E_ARGUMENT, "null argument");
+    if (slot_cap > SWG_MAX_SEQ_LEN) return fail(SWG_E_CAPACITY, "query capacity above 32767 residues");
+    SWG_HIP(hipSetDevice(c->device));
+    SWG_HIP(hipStreamSynchronize(c->main.stream));
+    for (auto& B : c->main.bufs) B.pending = false;
+    c->tuned.clear();
+    // residue codes: distinct symbols of the set (and of the matrix), in ASCII order, + pad
+    bool seen[256] = {false};
+    int32_t maxlen = slot_cap;
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t L = offsets[i + 1] - offsets[i];
+        if (L < 0) return fail(SWG_E_ARGUMENT, "offsets must be non-decreasing");
+        if (L > SWG_MAX_SEQ_LEN) return fail(SWG_E_CAPACITY, "sequence longer than 32767 residues");
+        maxlen = std::max<int32_t>(maxlen, (int32_t)L);
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) seen[residues[k]] = true;
+    }
+    if (matrix_symbols)
+        for (int x = 0; x < SWG_MATRIX_SIZE; ++x)
+            for (int y = 0; y < SWG_MATRIX_SIZE; ++y)
+                if (c->iscores[x * SWG_MATRIX_SIZE + y] != 0) seen[x] = seen[y] = true;
+    int32_t ncode = 0;
+    for (int s = 0; s < 256; ++s) {
+        c->code_of[s] = -1;
+        if (seen[s]) {
+            if (s >= SWG_MATRIX_SIZE) return fail(SWG_E_SYMBOL, "residue symbol >= 127 (the reference indexes a 127x127 matrix)");
+            c->code_of[s] = (int16_t)ncode++;
+        }
+    }
+    if (ncode + 1 > kMaxCodes) return fail(SWG_E_CAPACITY, "more than 63 distinct residue symbols");
+    const int32_t pad = ncode;
+    c->ncode = ncode + 1;
+    std::vector<uint8_t> sym_of(c->ncode, 0);
+    for (int s = 0; s < 256; ++s) if (c->code_of[s] >= 0) sym_of[c->code_of[s]] = (uint8_t)s;
+
+    // profile table (score + gap_open, increment of the packed count word)
+    std::vector<int2> prof((size_t)c->ncode * c->ncode);
+    for (int x = 0; x < c->ncode; ++x)
+        for (int y = 0; y < c->ncode; ++y) {
+            int2 v;
+            if (x == pad || y == pad) { v.x = kPadScore; v.y = 0; }
+            else {
+                const int sx = sym_of[x], sy = sym_of[y];
+                const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
+                int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
+                if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
+                v.x = sc + c->gap_open;
+                v.y = 0x10000 + cnt;
+            }
+            prof[(size_t)x * c->ncode + y] = v;
+        }
+
+    // residue codes, each sequence 4-byte aligned, then the slots, then a
+    // tail of pad codes
+    const int32_t ntot = n + nslots;
+    std::vector<int64_t> off(std::max(ntot, 1));
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        off[i] = total;
+        total += ((offsets[i + 1] - offsets[i]) + 3) & ~int64_t(3);
+    }
+    for (int32_t q = 0; q < nslots; ++q) {
+        off[n + q] = total;
+        total += ((int64_t)slot_cap + 3) & ~int64_t(3);
+    }
+    const int64_t pad_off = total;
+    total += maxlen + 64;
+    std::vector<uint8_t> codes((size_t)total, (uint8_t)pad);
+    std::vector<int32_t> lens(std::max(ntot, 1), 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t L = offsets[i + 1] - offsets[i];
+        lens[i] = (int32_t)L;
+        for (int64_t k = 0; k < L; ++k) codes[off[i] + k] = (uint8_t)c->code_of[residues[offsets[i] + k]];
+    }
+    SWG_HIP(c->res.reserve(codes.size()));
+    SWG_HIP(c->off.reserve(off.size()));
+    SWG_HIP(c->len.reserve(lens.size()));
+    SWG_HIP(c->prof.reserve(prof.size()));
+    SWG_HIP(hipMemcpy(c->res.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
+    SWG_HIP(hipMemcpy(c->off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+    SWG_HIP(hipMemcpy(c->len.p, lens.data(), lens.size() * 4, hipMemcpyHostToDevice));
+    SWG_HIP(hipMemcpy(c->prof.p, prof.data(), prof.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c->pad_off = pad_off;
+    c->nseq = ntot;
+    c->h_len.assign(lens.begin(), lens.begin() + ntot);
+    c->h_off.assign(off.begin(), off.begin() + ntot);
+    if (maxlen != c->maxlen) { c->main.bnd.release(); c->main.bnd_slots = 0; }
+    c->maxlen = maxlen;
+    c->slot_maxlen = slot_cap;
     return SWG_OK;
 }
 
@@ -585,11 +695,11 @@ int swg_create(swg_ctx** out, const swg_params* p)
     e = hipGetDeviceProperties(&prop, c->device);
     if (e != hipSuccess) return bail(e, "hipGetDeviceProperties");
     c->num_cu = prop.multiProcessorCount;
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    e = hipStreamCreateWithFlags(&c->main.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(e, "hipStreamCreate");
-    if ((e = hipEventCreate(&c->ev_k0)) != hipSuccess) return bail(e, "hipEventCreate");
-    if ((e = hipEventCreate(&c->ev_k1)) != hipSuccess) return bail(e, "hipEventCreate");
-    for (auto& B : c->bufs)
+    if ((e = hipEventCreate(&c->main.ev0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&c->main.ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    for (auto& B : c->main.bufs)
         if ((e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     *out = c;
     return SWG_OK;
@@ -597,82 +707,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
 
 int swg_set_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, int32_t n)
 {
-    if (!c || n < 0 || (n > 0 && (!residues || !offsets))) return fail(SWG_E_ARGUMENT, "null argument");
-    SWG_HIP(hipSetDevice(c->device));
-    SWG_HIP(hipStreamSynchronize(c->stream));
-    for (auto& B : c->bufs) B.pending = false;
-    c->tuned.clear();
-    // residue codes: distinct symbols of the set, in ASCII order, + pad
-    bool seen[256] = {false};
-    int32_t maxlen = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        const int64_t L = offsets[i + 1] - offsets[i];
-        if (L < 0) return fail(SWG_E_ARGUMENT, "offsets must be non-decreasing");
-        if (L > SWG_MAX_SEQ_LEN) return fail(SWG_E_CAPACITY, "sequence longer than 32767 residues");
-        maxlen = std::max<int32_t>(maxlen, (int32_t)L);
-        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) seen[residues[k]] = true;
-    }
-    int32_t ncode = 0;
-    for (int s = 0; s < 256; ++s) {
-        c->code_of[s] = -1;
-        if (seen[s]) {
-            if (s >= SWG_MATRIX_SIZE) return fail(SWG_E_SYMBOL, "residue symbol >= 127 (the reference indexes a 127x127 matrix)");
-            c->code_of[s] = (int16_t)ncode++;
-        }
-    }
-    if (ncode + 1 > kMaxCodes) return fail(SWG_E_CAPACITY, "more than 63 distinct residue symbols");
-    const int32_t pad = ncode;
-    c->ncode = ncode + 1;
-    std::vector<uint8_t> sym_of(c->ncode, 0);
-    for (int s = 0; s < 256; ++s) if (c->code_of[s] >= 0) sym_of[c->code_of[s]] = (uint8_t)s;
-
-    // profile table (score + gap_open, increment of the packed count word)
-    std::vector<int2> prof((size_t)c->ncode * c->ncode);
-    for (int x = 0; x < c->ncode; ++x)
-        for (int y = 0; y < c->ncode; ++y) {
-            int2 v;
-            if (x == pad || y == pad) { v.x = kPadScore; v.y = 0; }
-            else {
-                const int sx = sym_of[x], sy = sym_of[y];
-                const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
-                int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
-                if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
-                v.x = sc + c->gap_open;
-                v.y = 0x10000 + cnt;
-            }
-            prof[(size_t)x * c->ncode + y] = v;
-        }
-
-    // residue codes, each sequence 4-byte aligned, tail of pad codes
-    std::vector<int64_t> off(std::max(n, 1));
-    int64_t total = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        off[i] = total;
-        total += ((offsets[i + 1] - offsets[i]) + 3) & ~int64_t(3);
-    }
-    const int64_t pad_off = total;
-    total += maxlen + 64;
-    std::vector<uint8_t> codes((size_t)total, (uint8_t)pad);
-    std::vector<int32_t> lens(std::max(n, 1), 0);
-    for (int32_t i = 0; i < n; ++i) {
-        const int64_t L = offsets[i + 1] - offsets[i];
-        lens[i] = (int32_t)L;
-        for (int64_t k = 0; k < L; ++k) codes[off[i] + k] = (uint8_t)c->code_of[residues[offsets[i] + k]];
-    }
-    SWG_HIP(c->res.reserve(codes.size()));
-    SWG_HIP(c->off.reserve(off.size()));
-    SWG_HIP(c->len.reserve(lens.size()));
-    SWG_HIP(c->prof.reserve(prof.size()));
-    SWG_HIP(hipMemcpy(c->res.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
-    SWG_HIP(hipMemcpy(c->off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice));
-    SWG_HIP(hipMemcpy(c->len.p, lens.data(), lens.size() * 4, hipMemcpyHostToDevice));
-    SWG_HIP(hipMemcpy(c->prof.p, prof.data(), prof.size() * sizeof(int2), hipMemcpyHostToDevice));
-    c->pad_off = pad_off;
-    c->nseq = n;
-    c->h_len.assign(lens.begin(), lens.begin() + n);
-    if (maxlen != c->maxlen) { c->bnd.release(); c->bnd_slots = 0; }
-    c->maxlen = maxlen;
-    return SWG_OK;
+    return load_sequences(c, residues, offsets, n, 0, 0, false);
 }
 
 int swg_compute_block_device(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
@@ -680,16 +715,17 @@ int swg_compute_block_device(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int
 {
     if (!c || !dev_out) return fail(SWG_E_ARGUMENT, "null argument");
     SWG_HIP(hipSetDevice(c->device));
-    return enqueue_block(c, r0, r1, c0, c1, is_diag, dev_out, ld, reinterpret_cast<int32_t*>(dev_stats), pairs_done);
+    return enqueue_block(c, c->main, r0, r1, c0, c1, is_diag, dev_out, ld, reinterpret_cast<int32_t*>(dev_stats),
+                         pairs_done);
 }
 
 int swg_synchronize(swg_ctx* c)
 {
     if (!c) return fail(SWG_E_ARGUMENT, "null argument");
-    SWG_HIP(hipStreamSynchronize(c->stream));
-    for (auto& B : c->bufs) B.pending = false;
+    SWG_HIP(hipStreamSynchronize(c->main.stream));
+    for (auto& B : c->main.bufs) B.pending = false;
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last_kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, c->main.ev0, c->main.ev1) == hipSuccess) c->last_kernel_ms = ms;
     return SWG_OK;
 }
 
@@ -705,15 +741,15 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
     const size_t out_bytes = (size_t)(rows * cols * es + 3) & ~size_t(3);
     SWG_HIP(c->out_stage.reserve(out_bytes));
     if (stats) SWG_HIP(c->stats_stage.reserve((size_t)rows * cols * 4));
-    SWG_HIP(launch_fill_bytes_zero(c->out_stage.p, (int64_t)out_bytes, c->stream));
-    if (stats) SWG_HIP(launch_fill_i32(c->stats_stage.p, 0, rows * cols * 4, c->stream));
-    int rc = enqueue_block(c, r0, r1, c0, c1, is_diag, c->out_stage.p, cols, stats ? c->stats_stage.p : nullptr,
-                           pairs_done);
+    SWG_HIP(launch_fill_bytes_zero(c->out_stage.p, (int64_t)out_bytes, c->main.stream));
+    if (stats) SWG_HIP(launch_fill_i32(c->stats_stage.p, 0, rows * cols * 4, c->main.stream));
+    int rc = enqueue_block(c, c->main, r0, r1, c0, c1, is_diag, c->out_stage.p, cols,
+                           stats ? c->stats_stage.p : nullptr, pairs_done);
     if (rc) return rc;
     SWG_HIP(hipMemcpy2DAsync(out, (size_t)out_ld * es, c->out_stage.p, (size_t)cols * es, (size_t)cols * es,
-                             (size_t)rows, hipMemcpyDeviceToHost, c->stream));
+                             (size_t)rows, hipMemcpyDeviceToHost, c->main.stream));
     if (stats)
-        SWG_HIP(hipMemcpyAsync(stats, c->stats_stage.p, (size_t)rows * cols * 16, hipMemcpyDeviceToHost, c->stream));
+        SWG_HIP(hipMemcpyAsync(stats, c->stats_stage.p, (size_t)rows * cols * 16, hipMemcpyDeviceToHost, c->main.stream));
     return swg_synchronize(c);
 }
 
@@ -746,8 +782,8 @@ void swg_destroy(swg_ctx* c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& B : c->bufs) {
+    if (c->main.stream) (void)hipStreamSynchronize(c->main.stream);
+    for (auto& B : c->main.bufs) {
         for (auto& gexec : B.graph)
             if (gexec) (void)hipGraphExecDestroy(gexec);
         B.task_row.release(); B.task_tile.release(); B.tile_col.release(); B.tile_maxlen.release();
@@ -757,12 +793,390 @@ void swg_destroy(swg_ctx* c)
         B.queue.release(); B.desc.release();
         if (B.done) (void)hipEventDestroy(B.done);
     }
-    c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->bnd.release();
+    c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
     c->out_stage.release(); c->stats_stage.release();
-    if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
-    if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->main.ev0) (void)hipEventDestroy(c->main.ev0);
+    if (c->main.ev1) (void)hipEventDestroy(c->main.ev1);
+    if (c->main.stream) (void)hipStreamDestroy(c->main.stream);
     delete c;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving: continuous batching of query sequences against a device-resident
+// panel.  A scheduler thread owns every HIP call of the server.  Each lane
+// (stream + buffers + its own boundary scratch) runs one batch at a time:
+// when a lane is free, the pending queries (FIFO) that fit the lane's batch
+// size and the HBM budget are admitted, written into query slots of the
+// resident sequence store, aligned against the whole panel as one block
+// (rows = the batch's slots, columns = the panel) and copied back through
+// pinned memory.  Lanes run concurrently, so a batch admitted while another
+// is on the GPU fills the device as the earlier one drains.  Every pair is an
+// independent alignment, so a query's distances do not depend on the batch
+// it shared.
+// ===========================================================================
+
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct ServeRequest {
+    int64_t ticket = 0;
+    std::vector<uint8_t> codes;   // residue codes
+    void* out = nullptr;          // caller's host buffer: npanel TDistance
+    swg_pair_stats* stats = nullptr;
+    Clock::time_point t_submit, t_admit, t_done;
+    int state = 0;                // 0 pending, 1 running, 2 done, < 0 error code
+    int64_t footprint = 0;
+};
+
+struct ServeLane {
+    Lane lane;
+    std::vector<std::shared_ptr<ServeRequest>> batch;
+    DevBuf<uint8_t> dout;
+    DevBuf<int32_t> dstats;
+    uint8_t* h_out = nullptr;     // pinned staging
+    int32_t* h_stats = nullptr;
+    uint8_t* h_codes = nullptr;   // pinned staging for query residues + lengths
+    hipEvent_t done = nullptr;
+    bool active = false;
+    int64_t footprint = 0;
+    int32_t slot_base = 0;        // first query slot (sequence index) of this lane
+};
+
+}  // namespace
+
+struct swg_server {
+    swg_ctx* c = nullptr;
+    int32_t npanel = 0, max_batch = 0, max_qlen = 0, nlanes = 0;
+    bool want_stats = true;
+    int64_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<ServeLane>> lanes;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<ServeRequest>> pending;
+    std::unordered_map<int64_t, std::shared_ptr<ServeRequest>> tickets;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    int error = SWG_OK;
+    std::string error_msg;
+    std::thread worker;
+    Clock::time_point t0;
+    int64_t completed = 0;
+    double cells = 0.0;
+};
+
+namespace {
+
+int64_t query_footprint(const swg_server* s, int32_t qlen)
+{
+    // distances + stats + best cells + bounded-pass tiles per panel pair, + residues
+    const int64_t per_pair = elem_size(s->c) + (s->want_stats ? 16 : 0) + 4 + 9;
+    return per_pair * s->npanel + qlen;
+}
+
+// Start a batch on lane `L` (worker thread, lock held by the caller only
+// while taking requests).
+int launch_batch(swg_server* s, ServeLane& L)
+{
+    swg_ctx* c = s->c;
+    const int nb = (int)L.batch.size();
+    const hipStream_t st = L.lane.stream;
+    // query residues into the lane's slots, lengths into the length table
+    int64_t pos = 0;
+    std::vector<int32_t> lens(nb);
+    for (int k = 0; k < nb; ++k) {
+        const auto& q = L.batch[k];
+        std::memcpy(L.h_codes + pos, q->codes.data(), q->codes.size());
+        lens[k] = (int32_t)q->codes.size();
+        c->h_len[L.slot_base + k] = lens[k];
+        pos += ((int64_t)s->max_qlen + 3) & ~int64_t(3);
+    }
+    std::memcpy(L.h_codes + pos, lens.data(), nb * sizeof(int32_t));
+    for (int k = 0; k < nb; ++k) {
+        const int64_t stride = ((int64_t)s->max_qlen + 3) & ~int64_t(3);
+        SWG_HIP(hipMemcpyAsync(c->res.p + c->h_off[L.slot_base + k], L.h_codes + k * stride, lens[k],
+                               hipMemcpyHostToDevice, st));
+    }
+    SWG_HIP(hipMemcpyAsync(c->len.p + L.slot_base, L.h_codes + pos, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    int64_t pairs = 0;
+    int rc = enqueue_block(c, L.lane, L.slot_base, L.slot_base + nb - 1, 0, s->npanel - 1, 0, L.dout.p, s->npanel,
+                           s->want_stats ? L.dstats.p : nullptr, &pairs);
+    if (rc) return rc;
+    s->cells += c->last_cells;
+    SWG_HIP(hipMemcpyAsync(L.h_out, L.dout.p, (size_t)nb * s->npanel * elem_size(c), hipMemcpyDeviceToHost, st));
+    if (s->want_stats)
+        SWG_HIP(hipMemcpyAsync(L.h_stats, L.dstats.p, (size_t)nb * s->npanel * 16, hipMemcpyDeviceToHost, st));
+    SWG_HIP(hipEventRecord(L.done, st));
+    L.active = true;
+    return SWG_OK;
+}
+
+void finish_batch(swg_server* s, ServeLane& L)
+{
+    const int es = elem_size(s->c);
+    const auto now = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t k = 0; k < L.batch.size(); ++k) {
+        auto& q = L.batch[k];
+        std::memcpy(q->out, L.h_out + k * (size_t)s->npanel * es, (size_t)s->npanel * es);
+        if (q->stats && s->want_stats)
+            std::memcpy(q->stats, L.h_stats + k * (size_t)s->npanel * 4, (size_t)s->npanel * 16);
+        q->t_done = now;
+        q->state = 2;
+        ++s->completed;
+    }
+    s->used -= L.footprint;
+    L.footprint = 0;
+    L.batch.clear();
+    L.active = false;
+    for (auto& B : L.lane.bufs) B.pending = false;
+}
+
+void serve_loop(swg_server* s)
+{
+    (void)hipSetDevice(s->c->device);
+    for (;;) {
+        bool progressed = false;
+        // completions
+        for (auto& Lp : s->lanes) {
+            ServeLane& L = *Lp;
+            if (!L.active) continue;
+            hipError_t q = hipEventQuery(L.done);
+            if (q == hipSuccess) { finish_batch(s, L); progressed = true; }
+            else if (q != hipErrorNotReady) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->error = SWG_E_DEVICE;
+                s->error_msg = std::string("batch failed: ") + hipGetErrorString(q);
+                for (auto& r : L.batch) r->state = SWG_E_DEVICE;
+                L.batch.clear();
+                L.active = false;
+                s->cv.notify_all();
+            }
+        }
+        // admissions: FIFO, one batch per free lane
+        for (auto& Lp : s->lanes) {
+            ServeLane& L = *Lp;
+            if (L.active) continue;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                while (!s->pending.empty() && (int)L.batch.size() < s->max_batch) {
+                    auto& q = s->pending.front();
+                    if (s->used + L.footprint + q->footprint > s->budget && !L.batch.empty()) break;
+                    if (s->used + L.footprint + q->footprint > s->budget && s->used > 0) break;
+                    L.footprint += q->footprint;
+                    q->t_admit = Clock::now();
+                    q->state = 1;
+                    L.batch.push_back(q);
+                    s->pending.pop_front();
+                }
+                s->used += L.footprint;
+            }
+            if (L.batch.empty()) continue;
+            int rc = launch_batch(s, L);
+            progressed = true;
+            if (rc) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->error = rc;
+                s->error_msg = swg_last_error();
+                for (auto& r : L.batch) r->state = rc;
+                s->used -= L.footprint;
+                L.footprint = 0;
+                L.batch.clear();
+            }
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (progressed) s->cv.notify_all();
+        bool any_active = false;
+        for (auto& Lp : s->lanes) any_active = any_active || Lp->active;
+        if (s->stop && !any_active) break;
+        if (!progressed) {
+            if (any_active) s->cv.wait_for(lk, std::chrono::microseconds(100));
+            else s->cv.wait_for(lk, std::chrono::milliseconds(5), [&] { return s->stop || !s->pending.empty(); });
+        }
+    }
+}
+
+double ms_since(const swg_server* s, Clock::time_point t)
+{
+    return std::chrono::duration<double, std::milli>(t - s->t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel_residues,
+                     const int64_t* panel_offsets, int32_t npanel, int32_t max_query_len, int32_t max_batch,
+                     int32_t lanes, int64_t hbm_budget_bytes, int32_t want_stats)
+{
+    if (!out || !p || npanel < 1 || max_query_len < 1 || max_batch < 1 || lanes < 1 || lanes > 8 ||
+        hbm_budget_bytes <= 0)
+        return fail(SWG_E_ARGUMENT, "bad server arguments");
+    *out = nullptr;
+    swg_params q = *p;
+    q.schedule = SWG_SCHED_PERSISTENT_I32;  // replaced below when the int16 pass applies
+    swg_ctx* c = nullptr;
+    int rc = swg_create(&c, &q);
+    if (rc) return rc;
+    if (c->int16_ok) c->schedule = SWG_SCHED_PERSISTENT_I16;
+    rc = load_sequences(c, panel_residues, panel_offsets, npanel, lanes * max_batch, max_query_len, true);
+    if (rc) { swg_destroy(c); return rc; }
+    auto* s = new swg_server();
+    s->c = c;
+    s->npanel = npanel; s->max_batch = max_batch; s->max_qlen = max_query_len; s->nlanes = lanes;
+    s->want_stats = want_stats != 0;
+    s->budget = hbm_budget_bytes;
+    s->t0 = Clock::now();
+    auto bail = [&](const char* what, hipError_t e) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        swg_serve_destroy(s);
+        return fail(SWG_E_DEVICE, m);
+    };
+    const int es = elem_size(c);
+    const size_t nout = (size_t)max_batch * npanel;
+    for (int i = 0; i < lanes; ++i) {
+        auto L = std::make_unique<ServeLane>();
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&L->lane.stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+        if ((e = hipEventCreate(&L->lane.ev0)) != hipSuccess) return bail("event", e);
+        if ((e = hipEventCreate(&L->lane.ev1)) != hipSuccess) return bail("event", e);
+        if ((e = hipEventCreateWithFlags(&L->done, hipEventDisableTiming)) != hipSuccess) return bail("event", e);
+        for (auto& B : L->lane.bufs)
+            if ((e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming)) != hipSuccess) return bail("event", e);
+        if ((e = L->dout.reserve(nout * es)) != hipSuccess) return bail("hipMalloc", e);
+        if (s->want_stats && (e = L->dstats.reserve(nout * 4)) != hipSuccess) return bail("hipMalloc", e);
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_out), nout * es, 0)) != hipSuccess) return bail("hipHostMalloc", e);
+        if (s->want_stats &&
+            (e = hipHostMalloc(reinterpret_cast<void**>(&L->h_stats), nout * 16, 0)) != hipSuccess)
+            return bail("hipHostMalloc", e);
+        const size_t codes_bytes = (size_t)max_batch * (((size_t)max_query_len + 3) & ~size_t(3)) + max_batch * 4;
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_codes), codes_bytes, 0)) != hipSuccess)
+            return bail("hipHostMalloc", e);
+        L->slot_base = npanel + i * max_batch;
+        s->lanes.push_back(std::move(L));
+    }
+    s->worker = std::thread(serve_loop, s);
+    *out = s;
+    return SWG_OK;
+}
+
+int swg_serve_submit(swg_server* s, const uint8_t* query, int32_t len, void* out, swg_pair_stats* stats,
+                     int64_t* ticket)
+{
+    if (!s || (!query && len > 0) || !out || !ticket || len < 0) return fail(SWG_E_ARGUMENT, "null argument");
+    if (len > s->max_qlen) return fail(SWG_E_CAPACITY, "query longer than the server's max_query_len");
+    auto r = std::make_shared<ServeRequest>();
+    r->codes.resize(len);
+    for (int32_t k = 0; k < len; ++k) {
+        const int16_t code = s->c->code_of[query[k]];
+        if (code < 0) return fail(SWG_E_SYMBOL, "query symbol outside the panel/matrix alphabet");
+        r->codes[k] = (uint8_t)code;
+    }
+    r->out = out;
+    r->stats = stats;
+    r->footprint = query_footprint(s, len);
+    if (r->footprint > s->budget) return fail(SWG_E_CAPACITY, "one query exceeds the HBM budget");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(SWG_E_ARGUMENT, "server is stopping");
+    r->ticket = s->next_ticket++;
+    r->t_submit = Clock::now();
+    s->tickets[r->ticket] = r;
+    s->pending.push_back(r);
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return SWG_OK;
+}
+
+int swg_serve_wait(swg_server* s, int64_t ticket, int32_t timeout_ms, int32_t* done)
+{
+    if (!s || !done) return fail(SWG_E_ARGUMENT, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SWG_E_ARGUMENT, "unknown ticket");
+    auto r = it->second;
+    auto ready = [&] { return r->state == 2 || r->state < 0; };
+    if (timeout_ms < 0) s->cv.wait(lk, ready);
+    else s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    *done = r->state == 2 ? 1 : 0;
+    if (r->state < 0) return fail(r->state, "query failed: " + s->error_msg);
+    return SWG_OK;
+}
+
+int swg_serve_times(swg_server* s, int64_t ticket, double* submit_ms, double* admit_ms, double* done_ms)
+{
+    if (!s) return fail(SWG_E_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SWG_E_ARGUMENT, "unknown ticket");
+    const auto& r = *it->second;
+    if (submit_ms) *submit_ms = ms_since(s, r.t_submit);
+    if (admit_ms) *admit_ms = r.state >= 1 ? ms_since(s, r.t_admit) : -1.0;
+    if (done_ms) *done_ms = r.state == 2 ? ms_since(s, r.t_done) : -1.0;
+    return SWG_OK;
+}
+
+int swg_serve_release(swg_server* s, int64_t ticket)
+{
+    if (!s) return fail(SWG_E_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SWG_E_ARGUMENT, "unknown ticket");
+    if (it->second->state == 0 || it->second->state == 1) return fail(SWG_E_ARGUMENT, "query still in flight");
+    s->tickets.erase(it);
+    return SWG_OK;
+}
+
+int swg_serve_stats(swg_server* s, int64_t* completed, double* cells, double* now_ms)
+{
+    if (!s) return fail(SWG_E_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (completed) *completed = s->completed;
+    if (cells) *cells = s->cells;
+    if (now_ms) *now_ms = ms_since(s, Clock::now());
+    return SWG_OK;
+}
+
+void swg_serve_destroy(swg_server* s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->c ? s->c->device : 0);
+    for (auto& Lp : s->lanes) {
+        ServeLane& L = *Lp;
+        if (L.lane.stream) (void)hipStreamSynchronize(L.lane.stream);
+        for (auto& B : L.lane.bufs) {
+            for (auto& g : B.graph) if (g) (void)hipGraphExecDestroy(g);
+            B.task_row.release(); B.task_tile.release(); B.tile_col.release(); B.tile_maxlen.release();
+            B.task_rowlim.release(); B.tile_lim.release();
+            B.s_task_row.release(); B.s_task_tile.release(); B.s_tile_col.release(); B.s_tile_maxlen.release();
+            B.best_cell.release(); B.chunks.release(); B.nchunks.release(); B.queue.release(); B.desc.release();
+            if (B.done) (void)hipEventDestroy(B.done);
+        }
+        L.lane.bnd.release();
+        L.dout.release(); L.dstats.release();
+        if (L.h_out) (void)hipHostFree(L.h_out);
+        if (L.h_stats) (void)hipHostFree(L.h_stats);
+        if (L.h_codes) (void)hipHostFree(L.h_codes);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.lane.ev0) (void)hipEventDestroy(L.lane.ev0);
+        if (L.lane.ev1) (void)hipEventDestroy(L.lane.ev1);
+        if (L.lane.stream) (void)hipStreamDestroy(L.lane.stream);
+    }
+    if (s->c) swg_destroy(s->c);
+    delete s;
 }
 
 }  // extern "C"

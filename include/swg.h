@@ -140,6 +140,38 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
 const char* swg_last_error(void);
 void swg_destroy(swg_ctx* ctx);
 
+/* ---- serving (no counterpart in the reference, which only runs batch
+ * jobs): query sequences aligned SW-Gotoh against a device-resident panel
+ * with continuous batching.  s1 = the query, s2 = each panel sequence; the
+ * distances follow the context semantics above.  Thread-safe: any host thread
+ * may submit and wait; one internal thread issues all device work. ---- */
+typedef struct swg_server swg_server;
+
+/* panel: residues/offsets[npanel+1] as swg_set_sequences.  Queries may use
+ * the panel's symbols and every symbol the scoring matrix scores.  Up to
+ * `lanes` batches of at most `max_batch` queries (each <= max_query_len
+ * residues) are in flight at once; a batch is admitted only while the summed
+ * per-query device footprint (distances, statistics, best cells and task
+ * tiles: ~(sizeof(TDistance)+16+13) bytes per panel pair) stays within
+ * hbm_budget_bytes. */
+int  swg_serve_create(swg_server** server, const swg_params* p, const uint8_t* panel_residues,
+                      const int64_t* panel_offsets, int32_t npanel, int32_t max_query_len,
+                      int32_t max_batch, int32_t lanes, int64_t hbm_budget_bytes, int32_t want_stats);
+/* Queue one query; `out` (npanel TDistance) and `stats` (npanel entries or
+ * NULL) are written when it completes and must stay valid until then. */
+int  swg_serve_submit(swg_server* server, const uint8_t* query, int32_t len, void* out,
+                      swg_pair_stats* stats, int64_t* ticket);
+/* Wait up to timeout_ms (-1: forever) for a ticket; *done = 1 when finished. */
+int  swg_serve_wait(swg_server* server, int64_t ticket, int32_t timeout_ms, int32_t* done);
+/* Submit, admission and completion times of a ticket in ms since the server
+ * started (-1 while not reached). */
+int  swg_serve_times(swg_server* server, int64_t ticket, double* submit_ms, double* admit_ms, double* done_ms);
+/* Forget a finished ticket. */
+int  swg_serve_release(swg_server* server, int64_t ticket);
+/* Completed queries, DP cells issued and the server clock (ms). */
+int  swg_serve_stats(swg_server* server, int64_t* completed, double* cells, double* now_ms);
+void swg_serve_destroy(swg_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -269,3 +269,39 @@ def test_autotuned_blocks_match(ednafull):
     want, wst = oracle_block(s, 350, 699, 350, 699, True, ednafull, 16, 4)
     assert_same(got, want, gst, wst, "autotune diag")
     al.close()
+
+
+def test_serving_same_alone_and_in_mix(ednafull):
+    # each query's distances must be identical when it runs alone and when it
+    # shares batches with others, and equal to the oracle's
+    say("serving alone vs mix")
+    panel = ca.synth_dna(21, 300, 10, 0.10, 0.02, 0, [30, 250])
+    qs = ca.synth_dna(22, 24, 10, 0.10, 0.02, 0, [1, 400])
+    queries = [qs.seq(i) for i in range(len(qs))] + [b"", b"NACGU"]
+    srv = ca.Server(panel, ednafull, 16, 4, max_query_len=400, max_batch=5, lanes=2, hbm_budget=64 << 20)
+    alone = []
+    for q in queries:
+        t = srv.submit(q)
+        assert srv.wait(t, 60000)
+        out, st = srv.result(t)
+        alone.append((out.copy(), st.copy()))
+        srv.release(t)
+    tickets = [srv.submit(q) for q in queries]
+    for t in tickets:
+        assert srv.wait(t, 60000)
+    for k, t in enumerate(tickets):
+        out, st = srv.result(t)
+        assert np.array_equal(out, alone[k][0]), "query %d differs in the mix" % k
+        assert np.array_equal(st, alone[k][1])
+    # oracle: s1 = the query, s2 = each panel sequence
+    allseq = ca.SequenceSet.from_strings([panel.seq(i) for i in range(len(panel))] + queries)
+    npan = len(panel)
+    for k in range(len(queries)):
+        a = np.full(npan, npan + k)
+        b = np.arange(npan)
+        d, st = O.sw_pairs_mt(allseq.residues, allseq.offsets, a, b, ednafull, 16, 4, threads=THREADS,
+                              want_stats=True)
+        out, gst = srv.result(tickets[k])
+        assert np.array_equal(out, d), "query %d vs oracle" % k
+        assert np.array_equal(gst[:, :3], st[:, :3])
+    srv.close()
