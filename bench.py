@@ -255,7 +255,8 @@ def serve_bench(args):
     panel = WL.sequences("c5_panel")
     qset = WL.sequences("c5_queries")
     M, o, e = WL.scoring("c5_panel")
-    srv = ca.Server(panel, M, o, e, max_query_len=2000, max_batch=8, lanes=3, hbm_budget=2 << 30, stats=False)
+    qmax = int(qset.lengths.max())
+    srv = ca.Server(panel, M, o, e, max_query_len=qmax, max_batch=8, lanes=3, hbm_budget=2 << 30, stats=False)
     # warm-up: scratch allocation and graph capture on every lane
     warm = [srv.submit(qset.seq(i)) for i in range(6)]
     for t in warm:

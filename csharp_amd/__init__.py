@@ -33,7 +33,7 @@ DIST_PERCENT_IDENTITY, DIST_KIMURA2, DIST_JUKES_CANTOR, DIST_MINMAX_NORM = 0, 1,
 MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
 TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
-SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16 = 0, 1, 2, 3
+SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (

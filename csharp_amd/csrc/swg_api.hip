@@ -55,7 +55,8 @@ enum Variant : int {
     kVarPlain = 0,     // sw_kernel<16, false>: natural order, static assignment
     kVarI32 = 1,       // sw_kernel<R, true>: one pass, int32, statistics carried
     kVarI16 = 2,       // score16_kernel (int16x2) + group_kernel + bounded sw_kernel
-    kNumVariants = 3,
+    kVarLpp = 3,       // lpp16_kernel: 16 lanes per pair, 4 pairs per wave
+    kNumVariants = 4,
 };
 
 constexpr int kScoreTile = 2 * kWave;  // pairs per score-pass task
@@ -77,7 +78,7 @@ struct BlockBufs {
     hipEvent_t done = nullptr;
     bool pending = false;
     // one captured graph per variant, keyed by every baked launch argument
-    hipGraphExec_t graph[kNumVariants] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graph[kNumVariants] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<unsigned char> graph_key[kNumVariants];
 };
 
@@ -85,8 +86,8 @@ struct BlockBufs {
 struct TuneEntry {
     int64_t key = 0;
     int chosen = -1;
-    bool tried[kNumVariants] = {false, false, false};
-    double rate[kNumVariants] = {0.0, 0.0, 0.0};
+    bool tried[kNumVariants] = {false, false, false, false};
+    double rate[kNumVariants] = {0.0, 0.0, 0.0, 0.0};
 };
 
 // An execution lane: a stream, two alternating buffer sets and a boundary
@@ -114,6 +115,7 @@ struct swg_ctx {
     int32_t gap_open = 0, gap_ext = 0;
     bool count_similarity = false;  // protein: PercentSimilarity
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
+    bool lpp_ok = false;            // non-negative penalties (pre-column cells of the 16-lane kernel)
 
     // sequences
     int32_t nseq = 0;
@@ -154,6 +156,7 @@ int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 
 void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
 {
     if (v == kVarPlain) { *s1 = 16; *s2 = 0; return; }
+    if (v == kVarLpp) { *s1 = c->strip_rows == 8 ? 8 : 4; *s2 = 0; return; }
     if (v == kVarI32) { *s1 = c->strip_rows ? c->strip_rows : 16; *s2 = 0; return; }
     *s1 = 16;  // score pass
     *s2 = c->strip_rows ? c->strip_rows : 16;
@@ -163,7 +166,9 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
 {
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
-    int a = v == kVarI16 ? score16_blocks_per_cu(s1) : sw_blocks_per_cu(s1, false);
+    int a = v == kVarI16 ? score16_blocks_per_cu(s1)
+          : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
+                         : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? sw_blocks_per_cu(s2, false) : 0;
     if (a <= 0 || (v == kVarI16 && b <= 0)) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
@@ -358,7 +363,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         desc[1].ntasks = plan.ntasks;
         desc[1].r0 = g.r0; desc[1].c0 = g.c0; desc[1].ld = g.c1 - g.c0 + 1;
     } else {
-        plan_tiles(c, g, v != kVarPlain, kWave, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
+        plan_tiles(c, g, v != kVarPlain, v == kVarLpp ? kLppTile : kWave, B.h_task_row, B.h_task_tile, B.h_tile_col,
+                   B.h_tile_maxlen, plan);
         if ((rc = upload(st, B.task_row, B.h_task_row)) || (rc = upload(st, B.task_tile, B.h_task_tile)) ||
             (rc = upload(st, B.tile_col, B.h_tile_col)) || (rc = upload(st, B.tile_maxlen, B.h_tile_maxlen)))
             return rc;
@@ -408,6 +414,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (e == hipSuccess) {
                 if (v == kVarI32) {
                     e = launch_sw(S, T, W, s1, true, n1, st);
+                } else if (v == kVarLpp) {
+                    e = launch_lpp16(S, T, W, s1, true, n1, st);
                 } else {
                     e = launch_score16(S, ST, W, s1, n1, st);
                     if (e == hipSuccess) e = launch_group(G, group_grid, st);
@@ -471,12 +479,30 @@ int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
     return ((int64_t)c->ncode << 40) | ((int64_t)len_class << 20) | (int64_t)size_class;
 }
 
-int choose_variant(swg_ctx* c, const BlockGeom& g, int* variant, TuneEntry** trial)
+// Too few row sequences to fill the GPU with one-lane-per-pair waves: the
+// 16-lanes-per-pair kernel gives each pair 16 lanes instead.
+bool thin_block(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
+{
+    const double rows = g.r1 - g.r0 + 1, cols = g.c1 - g.c0 + 1;
+    const double waves = g.diag_lower ? rows * std::ceil(cols / 2.0 / kWave) : rows * std::ceil(cols / kWave);
+    int n1 = 0, n2 = 0;
+    if (slots_for(c, ln, kVarI32, &n1, &n2) != SWG_OK) return false;
+    return waves < 2.0 * n1;
+}
+
+int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant, TuneEntry** trial)
 {
     *trial = nullptr;
     if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I32 || !c->int16_ok) { *variant = kVarI32; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I16) { *variant = kVarI16; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) {
+        if (!c->lpp_ok) return fail(SWG_E_UNSUPPORTED, "the 16-lanes-per-pair kernel needs non-negative gap penalties");
+        *variant = kVarLpp;
+        return SWG_OK;
+    }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *variant = kVarI32; return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *variant = kVarI16; return SWG_OK; }
+    if (c->lpp_ok && thin_block(c, ln, g)) { *variant = kVarLpp; return SWG_OK; }
+    if (!c->int16_ok) { *variant = kVarI32; return SWG_OK; }
     const int64_t key = tune_key(c, g);
     TuneEntry* t = nullptr;
     for (auto& e : c->tuned)
@@ -517,7 +543,7 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
     }
     int variant = kVarI32;
     TuneEntry* trial = nullptr;
-    int rc = choose_variant(c, g, &variant, &trial);
+    int rc = choose_variant(c, ln, g, &variant, &trial);
     if (rc) return rc;
     BlockBufs& B = next_bufs(ln, &rc);
     if (rc) return rc;
@@ -655,7 +681,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
         return fail(SWG_E_ARGUMENT, "bad alignment_type");
     if (p->tdistance != SWG_TDIST_INT16 && p->tdistance != SWG_TDIST_DOUBLE) return fail(SWG_E_ARGUMENT, "bad tdistance");
     if (p->diag_mode != SWG_DIAG_LOWER_MIRROR && p->diag_mode != SWG_DIAG_FULL) return fail(SWG_E_ARGUMENT, "bad diag_mode");
-    if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_I16)
+    if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPP16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
         return fail(SWG_E_UNSUPPORTED, "only PercentIdentity is implemented for nucleic alignments");
@@ -681,6 +707,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
         for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k) maxabs = std::max(maxabs, std::abs(c->iscores[k]));
         c->int16_ok = maxabs + std::abs(c->gap_open) <= 8192 && std::abs(c->gap_ext) <= 8192 && c->gap_open >= 0 &&
                       c->gap_ext >= 0;
+        c->lpp_ok = c->gap_open >= 0 && c->gap_ext >= 0;
     }
     c->device = p->device;
 
@@ -756,7 +783,7 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
 int swg_set_schedule(swg_ctx* c, int32_t schedule, int32_t strip_rows)
 {
     if (!c) return fail(SWG_E_ARGUMENT, "null argument");
-    if (schedule < SWG_SCHED_PERSISTENT || schedule > SWG_SCHED_PERSISTENT_I16)
+    if (schedule < SWG_SCHED_PERSISTENT || schedule > SWG_SCHED_PERSISTENT_LPP16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
     if (strip_rows != 0 && strip_rows != 8 && strip_rows != 16 && strip_rows != 32)
         return fail(SWG_E_ARGUMENT, "strip_rows must be 0 (default), 8, 16 or 32");
@@ -774,6 +801,7 @@ int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t*
     if (variant)
         *variant = c->last_variant == kVarPlain ? SWG_SCHED_PLAIN
                  : c->last_variant == kVarI32   ? SWG_SCHED_PERSISTENT_I32
+                 : c->last_variant == kVarLpp   ? SWG_SCHED_PERSISTENT_LPP16
                                                 : SWG_SCHED_PERSISTENT_I16;
     return SWG_OK;
 }
@@ -906,6 +934,14 @@ int launch_batch(swg_server* s, ServeLane& L)
                                hipMemcpyHostToDevice, st));
     }
     SWG_HIP(hipMemcpyAsync(c->len.p + L.slot_base, L.h_codes + pos, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    // batches too small to fill the GPU one lane per pair use 16 lanes per
+    // pair; larger ones the int16 two-pass (or int32) variant -- no timing
+    // trials on the serving path
+    BlockGeom g;
+    g.r0 = L.slot_base; g.r1 = L.slot_base + nb - 1; g.c0 = 0; g.c1 = s->npanel - 1; g.diag_lower = false;
+    c->schedule = (c->lpp_ok && thin_block(c, L.lane, g)) ? SWG_SCHED_PERSISTENT_LPP16
+                : c->int16_ok                            ? SWG_SCHED_PERSISTENT_I16
+                                                         : SWG_SCHED_PERSISTENT_I32;
     int64_t pairs = 0;
     int rc = enqueue_block(c, L.lane, L.slot_base, L.slot_base + nb - 1, 0, s->npanel - 1, 0, L.dout.p, s->npanel,
                            s->want_stats ? L.dstats.p : nullptr, &pairs);
@@ -1022,11 +1058,10 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
         return fail(SWG_E_ARGUMENT, "bad server arguments");
     *out = nullptr;
     swg_params q = *p;
-    q.schedule = SWG_SCHED_PERSISTENT_I32;  // replaced below when the int16 pass applies
+    q.schedule = SWG_SCHED_PERSISTENT_I32;
     swg_ctx* c = nullptr;
     int rc = swg_create(&c, &q);
     if (rc) return rc;
-    if (c->int16_ok) c->schedule = SWG_SCHED_PERSISTENT_I16;
     rc = load_sequences(c, panel_residues, panel_offsets, npanel, lanes * max_batch, max_query_len, true);
     if (rc) { swg_destroy(c); return rc; }
     auto* s = new swg_server();

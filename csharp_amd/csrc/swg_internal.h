@@ -106,6 +106,12 @@ hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
                           int nslots, hipStream_t stream);
 hipError_t launch_group(const GroupSet& G, int nchunks, hipStream_t stream);
+// Sixteen-lanes-per-pair kernel: tiles of kLppTile column sequences;
+// strip_rows (rows per lane) in {4, 8}.
+constexpr int kLppTile = 4;
+hipError_t launch_lpp16(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool persistent,
+                        int nslots, hipStream_t stream);
+int lpp16_blocks_per_cu(int strip_rows, int ncode);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
 int score16_blocks_per_cu(int strip_rows);

@@ -205,6 +205,174 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
 }
 
 // ---------------------------------------------------------------------------
+// Sixteen lanes per pair: a wave aligns s1 (uniform) against 4 sequences s2,
+// one per 16-lane DPP row.  Lane k of a group owns R consecutive DP rows of a
+// stripe of 16*R rows and runs one column behind lane k-1 (anti-diagonal
+// wavefront): at step t it computes column t-k+1 of its rows, taking the row
+// above its first row (V-o, G, T, TG) and the column residue from lane k-1
+// with a DPP row shift, while lane 0 reads the previous stripe's bottom row
+// from the boundary buffer and lane 15 writes this stripe's.  Four times the
+// waves per sequence s1 of the one-lane-per-pair kernel, for latency when few
+// sequences s1 are in flight.
+//
+// Lanes before their first column ("pre-columns", j < 1) run pad cells whose
+// value is 0 with T = 0, i.e. the state of DP column 0 (this needs
+// gap_open >= 0 and gap_ext >= 0); lanes past the last column run pad cells,
+// which can never hold a new row maximum (see sw_kernel).
+constexpr int kLppPairs = 4;   // pairs per wave
+constexpr int kLpp = 16;       // lanes per pair
+
+__device__ __forceinline__ int row_shr1(int old, int src)
+{
+    return __builtin_amdgcn_update_dpp(old, src, 0x111, 0xF, 0xF, false);  // row_shr:1
+}
+
+template <int R, bool PERSIST>
+__global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch W)
+{
+    constexpr int H = kLpp * R;   // stripe height
+    constexpr int PS = H / 2 + 1; // int4 per residue code in the stripe profile
+    extern __shared__ int4 dyn_lds[];
+    int4* lprof = dyn_lds;                                             // ncode * PS int4
+    int2* lmerge = reinterpret_cast<int2*>(dyn_lds + S.ncode * PS);    // 4 groups * H (best, T)
+
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, k = lane & 15;
+    int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
+    const int ntasks = T.desc->ntasks;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int pad = S.ncode - 1;
+
+    for (int iter = 0;; ++iter) {
+        int t_task;
+        if (PERSIST) {
+            int v = 0;
+            if (lane == 0) v = atomicAdd(T.queue, 1);
+            t_task = uni(v);
+        } else {
+            t_task = blockIdx.x + iter * (int)gridDim.x;
+        }
+        if (t_task >= ntasks) break;
+
+        const int a = uni(T.task_row[t_task]);
+        const int tile = uni(T.task_tile[t_task]);
+        const int maxLb = uni(T.tile_maxlen[tile]);
+        const int b = T.tile_col[(int64_t)tile * kLppPairs + g];
+        const int La = uni(S.len[a]);
+        const int Lb = b >= 0 ? S.len[b] : 0;
+        const uint8_t* s1 = S.res + S.off[a];
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (b >= 0 ? S.off[b] : S.pad_off));
+        const int nsteps = (maxLb + (kLpp - 1) + 3) & ~3;  // lane 15 reaches column maxLb
+
+        // row 0 of the DP (SmithWatermanGotoh.cs:131-139), also beyond the last
+        // column so that lane 0 always reads defined values
+        for (int j = 1 + lane; j <= nsteps + 8; j += kWave)
+            for (int q = 0; q < kLppPairs; ++q) bnd[(int64_t)j * kLppPairs + q] = make_int4(-o, kNeg, 0, 0);
+        __syncthreads();  // other lanes read these columns
+
+        int best = INT_MIN, bestT = 0;
+        for (int i0 = 0; i0 < La; i0 += H) {
+            __syncthreads();
+            // stage the stripe profile: lprof[c2][row], rows past s1 use the pad code
+            {
+                int2* lp = reinterpret_cast<int2*>(lprof);
+                for (int idx = lane; idx < S.ncode * H; idx += kWave) {
+                    const int c2 = idx / H, r = idx % H;
+                    const int row = i0 + r;
+                    const int c1 = row < La ? (int)s1[row] : pad;
+                    lp[c2 * (H + 2) + r] = S.prof[c1 * S.ncode + c2];
+                }
+            }
+            __syncthreads();
+
+            int tr[R], Tr[R], Hr[R], THr[R], rb[R], rT[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
+            }
+            int ot = -o, oG = kNeg, oT = 0, oTG = 0;  // this lane's last-row output of the previous step
+            int pt = -o, pT = 0;                      // up-values received at the previous step (diagonal)
+            int cprev = pad;
+
+            uint32_t nw = s2w[0];
+            int4 nb0 = bnd[1 * kLppPairs + g], nb1 = bnd[2 * kLppPairs + g];
+            int4 nb2 = bnd[3 * kLppPairs + g], nb3 = bnd[4 * kLppPairs + g];
+            for (int t = 0; t < nsteps; t += 4) {
+                const uint32_t cw = nw;
+                int4 cb[4] = {nb0, nb1, nb2, nb3};
+                nw = s2w[(t + 4) >> 2];
+                const int64_t nxt = (int64_t)(t + 5) * kLppPairs + g;
+                nb0 = bnd[nxt]; nb1 = bnd[nxt + kLppPairs]; nb2 = bnd[nxt + 2 * kLppPairs];
+                nb3 = bnd[nxt + 3 * kLppPairs];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int tt = t + q;
+                    int c0 = (int)((cw >> (8 * q)) & 0xFFu);
+                    c0 = tt + 1 > Lb ? pad : c0;
+                    const int c2 = row_shr1(c0, cprev);
+                    cprev = c2;
+                    int tu = row_shr1(cb[q].x, ot), Gu = row_shr1(cb[q].y, oG);
+                    int Tu = row_shr1(cb[q].z, oT), TGu = row_shr1(cb[q].w, oTG);
+                    int dt = pt, dT = pT;
+                    pt = tu; pT = Tu;
+                    const int4* pr = lprof + c2 * PS + k * (R / 2);
+#pragma unroll
+                    for (int r2 = 0; r2 < R / 2; ++r2) {
+                        const int4 pp = pr[r2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int r = 2 * r2 + h;
+                            const int sc = h ? pp.z : pp.x;
+                            const int inc = h ? pp.w : pp.y;
+                            const int g1 = Gu - e;
+                            const bool cg = g1 > tu;
+                            const int G = cg ? g1 : tu;
+                            const int TG = (cg ? TGu : Tu) + 0x10000;
+                            const int h1 = Hr[r] - e;
+                            const bool ch = h1 > tr[r];
+                            Hr[r] = ch ? h1 : tr[r];
+                            THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                            const int f = dt + sc;
+                            int V = max(max(f, G), Hr[r]);
+                            V = max(V, 0);
+                            const int Td = dT + inc;
+                            int x = (V == G) ? TG : THr[r];
+                            x = (V == f) ? Td : x;
+                            const int Tn = (V == 0) ? 0 : x;
+                            const bool cb2 = V > rb[r];
+                            rb[r] = cb2 ? V : rb[r];
+                            rT[r] = cb2 ? Tn : rT[r];
+                            const int tn = V - o;
+                            dt = tr[r]; dT = Tr[r];
+                            tr[r] = tn; Tr[r] = Tn;
+                            tu = tn; Gu = G; Tu = Tn; TGu = TG;
+                        }
+                    }
+                    ot = tu; oG = Gu; oT = Tu; oTG = TGu;
+                    const int j = tt - k + 1;
+                    if (k == kLpp - 1 && j >= 1) bnd[(int64_t)j * kLppPairs + g] = make_int4(ot, oG, oT, oTG);
+                }
+            }
+            // first strict maximum in row-major order over the stripe's rows
+#pragma unroll
+            for (int r = 0; r < R; ++r) lmerge[g * H + k * R + r] = make_int2(rb[r], rT[r]);
+            __syncthreads();
+            if (k == 0) {
+                for (int r = 0; r < H && i0 + r < La; ++r) {
+                    const int2 m = lmerge[g * H + r];
+                    if (m.x > best) { best = m.x; bestT = m.y; }
+                }
+            }
+        }
+        if (k == 0 && b >= 0) {
+            int score = best, Tb = bestT;
+            if (La == 0 || Lb == 0) { score = INT_MIN; Tb = 0; }
+            write_distance(S, *T.desc, a, b, score, Tb);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Score-only pass, two pairs per lane in packed int16 halves.
 //
 // Same strip structure and query profile as sw_kernel, but only the score
@@ -476,6 +644,43 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 hipError_t launch_nw(const SeqSet&, const TaskSet&, const Scratch&, int, bool, int, hipStream_t)
 {
     return hipErrorNotSupported;
+}
+
+size_t lpp16_lds_bytes(int strip_rows, int ncode)
+{
+    const int H = kLpp * strip_rows;
+    return (size_t)ncode * (H / 2 + 1) * sizeof(int4) + (size_t)kLppPairs * H * sizeof(int2);
+}
+
+hipError_t launch_lpp16(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool persistent,
+                        int nslots, hipStream_t st)
+{
+    const size_t lds = lpp16_lds_bytes(strip_rows, S.ncode);
+    switch (strip_rows) {
+    case 4:
+        if (persistent) hipLaunchKernelGGL((lpp16_kernel<4, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((lpp16_kernel<4, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    case 8:
+        if (persistent) hipLaunchKernelGGL((lpp16_kernel<8, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((lpp16_kernel<8, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int lpp16_blocks_per_cu(int strip_rows, int ncode)
+{
+    int nb = 0;
+    hipError_t err = hipErrorInvalidValue;
+    const size_t lds = lpp16_lds_bytes(strip_rows, ncode);
+    switch (strip_rows) {
+    case 4: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lpp16_kernel<4, true>, kWave, lds); break;
+    case 8: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lpp16_kernel<8, true>, kWave, lds); break;
+    default: break;
+    }
+    return err == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,

@@ -63,6 +63,8 @@ extern "C" {
 #define SWG_SCHED_PLAIN          1  /* one launch per block, natural order, fixed strip height, int32 */
 #define SWG_SCHED_PERSISTENT_I32 2  /* persistent, forced int32 one-pass variant */
 #define SWG_SCHED_PERSISTENT_I16 3  /* persistent, forced int16 score pass + bounded int32 re-run */
+#define SWG_SCHED_PERSISTENT_LPP16 4 /* persistent, 16 lanes per pair (anti-diagonal, DPP row shifts),
+                                       4 pairs per wave: latency when few row sequences are in flight */
 
 #define SWG_MATRIX_SIZE 127      /* ScoringMatrix.SIZE, Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:34 */
 #define SWG_MAX_SEQ_LEN 32767    /* gap-size arrays are short in the reference (SmithWatermanGotoh.cs:83-84) */
@@ -128,12 +130,14 @@ int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
 int  swg_synchronize(swg_ctx* ctx);
 
 /* Schedule control (SWG_SCHED_*) and strip height of the int32 alignment
- * kernel (8, 16 or 32; 0 = default 16). */
+ * kernel (8, 16 or 32; 0 = default 16; for SWG_SCHED_PERSISTENT_LPP16, 8
+ * selects 8 rows per lane, anything else 4). */
 int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the
  * int32 alignment kernel and the variant that ran (1 plain, 2 int32 one-pass,
- * 3 int16 score pass + bounded int32 re-run: the SWG_SCHED_* value). */
+ * 3 int16 score pass + bounded int32 re-run, 4 sixteen lanes per pair: the
+ * SWG_SCHED_* value). */
 int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows,
                            int32_t* variant);
 

@@ -79,8 +79,9 @@ def protein_set():
 
 SCHEDULES = [(ca.SCHED_PERSISTENT, 0), (ca.SCHED_PLAIN, 16), (ca.SCHED_PERSISTENT_I32, 8),
              (ca.SCHED_PERSISTENT_I32, 16), (ca.SCHED_PERSISTENT_I32, 32), (ca.SCHED_PERSISTENT_I16, 16),
-             (ca.SCHED_PERSISTENT_I16, 32)]
-BLOCK_SCHEDULES = [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16]
+             (ca.SCHED_PERSISTENT_I16, 32), (ca.SCHED_PERSISTENT_LPP16, 0), (ca.SCHED_PERSISTENT_LPP16, 8)]
+BLOCK_SCHEDULES = [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16,
+                   ca.SCHED_PERSISTENT_LPP16]
 
 
 @pytest.mark.parametrize("schedule,strip", SCHEDULES)
@@ -138,7 +139,7 @@ def test_double_distances(dna_set, ednafull):
     al.close()
 
 
-@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16])
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16])
 def test_protein_percent_similarity(protein_set, blosum62, schedule):
     say("protein schedule=%d" % schedule)
     al = ca.Aligner(blosum62, 10, 1, alignment_type=ca.ALIGN_PROTEIN, schedule=schedule)
@@ -194,7 +195,7 @@ def test_full_matrix_multi_rank_layout(dna_set, ednafull):
     al.close()
 
 
-@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16])
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16])
 def test_long_sequences(ednafull, schedule):
     say("long sequences schedule=%d" % schedule)
     s = ca.synth_dna(13, 70, 3, 0.10, 0.02, 0, [1500, 2000])
@@ -278,7 +279,7 @@ def test_serving_same_alone_and_in_mix(ednafull):
     panel = ca.synth_dna(21, 300, 10, 0.10, 0.02, 0, [30, 250])
     qs = ca.synth_dna(22, 24, 10, 0.10, 0.02, 0, [1, 400])
     queries = [qs.seq(i) for i in range(len(qs))] + [b"", b"NACGU"]
-    srv = ca.Server(panel, ednafull, 16, 4, max_query_len=400, max_batch=5, lanes=2, hbm_budget=64 << 20)
+    srv = ca.Server(panel, ednafull, 16, 4, max_query_len=600, max_batch=5, lanes=2, hbm_budget=64 << 20)
     alone = []
     for q in queries:
         t = srv.submit(q)
