@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"],
+                    help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity check")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000")
@@ -70,6 +72,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_bench(args)}), flush=True)
+        return
+    if args.workload != "c3":
+        print(json.dumps(workload_run(args)), flush=True)
         return
 
     t0 = time.time()
@@ -242,6 +247,100 @@ def main():
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble):
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, nucleic=nucleic, tdist_double=tdouble,
+                         threads=threads)
+    return d, time.perf_counter() - t, threads
+
+
+def workload_run(args):
+    """C1: 1,000 DNA all pairs -> full Double matrix (SWMPI layout), every pair
+    checked against the oracle.  C2: 10,000 proteins, the full Int16 matrix
+    (PercentSimilarity), sampled pairs checked.  C4: rank 0 of 100,000 DNA at
+    P = 4: its 25,000-row stripe, diagonal block (lower-mirror) and the three
+    blocks right of it, streamed to host and written as SWMS block files."""
+    import hashlib
+    import tempfile
+    import csharp_amd as ca
+    from csharp_amd import workloads as WL
+
+    name = args.workload
+    cfg = WL.CONFIGS[name]
+    seqs = WL.sequences(name)
+    M, o, e = WL.scoring(name)
+    n = len(seqs)
+    L = seqs.lengths.astype(np.float64)
+    nucleic = cfg["alignment_type"] == ca.ALIGN_NUCLEIC
+    tdouble = cfg["tdistance"] == ca.TDIST_DOUBLE
+    al = ca.Aligner(M, o, e, alignment_type=cfg["alignment_type"], tdistance=cfg["tdistance"])
+    al.set_sequences(seqs)
+    rng = np.random.default_rng(7)
+    out = {"workload": name, "sequences": n}
+    if name in ("c1", "c2"):
+        t = time.perf_counter()
+        full, pairs = al.compute_full_matrix(world=1)
+        wall = time.perf_counter() - t
+        cells = float((np.cumsum(L)[:-1] * L[1:]).sum())
+        out.update(pairs=pairs, cells=cells, seconds=round(wall, 3), gcups=round(cells / wall / 1e9, 1),
+                   sha256=hashlib.sha256(full.tobytes()).hexdigest())
+        if name == "c1":
+            a, b = np.tril_indices(n, -1)  # every pair, s1 = the larger index (diagonal block)
+        else:
+            a = rng.integers(1, n, size=20000)
+            b = (rng.random(20000) * a).astype(np.int64)
+        d, dt, th = oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble)
+        got = full[a, b]
+        same = (got == d) | (np.isnan(got) & np.isnan(d)) if tdouble else (got == d)
+        sym = bool(np.array_equal(full[a, b], full[b, a], equal_nan=True)) if tdouble else bool(
+            np.array_equal(full[a, b], full[b, a]))
+        out["parity"] = {"checked_pairs": int(len(a)), "mismatches": int((~same).sum()), "mirror_symmetric": sym}
+        out["cpu_baseline"] = {"value": round(float((L[a] * L[b]).sum()) / dt / 1e9, 3), "unit": "GCUPS",
+                               "cores": th, "kind": "port", "sample": "%d pairs" % len(a)}
+    else:
+        world, rank = 4, 0
+        ranges = ca.range_partition(n, world)
+        (r0, r1) = ranges[rank]
+        tmp = tempfile.mkdtemp(prefix="c4_blocks_")
+        labels = ["seq%d" % i for i in range(n)]
+        hashes, tot_pairs, tot_cells = {}, 0, 0.0
+        t = time.perf_counter()
+        samples = []
+        for j in range(world):
+            c0, c1 = ranges[j]
+            blk, _, pairs = al.compute_block(r0, r1, c0, c1, j == rank)
+            path = os.path.join(tmp, "%d-%d.bin" % (rank, j))
+            ca.write_swms_block(path, "c4", rank, "mi355x", "c4.fasta", rank, j, (r0, r1), (c0, c1),
+                                [labels[r0], labels[r1], labels[c0], labels[c1]], blk)
+            hashes["%d-%d" % (rank, j)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
+            os.remove(path)
+            tot_pairs += pairs
+            tot_cells += WL.block_cells(L, r0, r1, c0, c1, diag_lower=(j == rank))
+            ka = rng.integers(r0, r1 + 1, size=1500)
+            kb = rng.integers(c0, c1 + 1, size=1500)
+            if j == rank:
+                keep = ka != kb
+                ka, kb = ka[keep], kb[keep]
+                ka, kb = np.maximum(ka, kb), np.minimum(ka, kb)
+            samples.append((ka, kb, blk[ka - r0, kb - c0].copy()))
+            log("c4 block %d-%d: %d pairs, %.1f s elapsed" % (rank, j, pairs, time.perf_counter() - t))
+            del blk
+        wall = time.perf_counter() - t
+        os.rmdir(tmp)
+        A = np.concatenate([x[0] for x in samples]); B = np.concatenate([x[1] for x in samples])
+        G = np.concatenate([x[2] for x in samples])
+        d, dt, th = oracle_sample(seqs, A, B, M, o, e, nucleic, tdouble)
+        out.update(rank=rank, world=world, pairs=tot_pairs, cells=tot_cells, seconds=round(wall, 2),
+                   gcups_end_to_end=round(tot_cells / wall / 1e9, 1), block_files_sha256=hashes,
+                   parity={"checked_pairs": int(len(A)), "mismatches": int((G != d).sum())},
+                   cpu_baseline={"value": round(float((L[A] * L[B]).sum()) / dt / 1e9, 3), "unit": "GCUPS",
+                                 "cores": th, "kind": "port", "sample": "%d pairs" % len(A)})
+    al.close()
+    return out
 
 
 def serve_bench(args):

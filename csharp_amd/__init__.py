@@ -40,12 +40,12 @@ ABI_SYMBOLS = (
     "swg_create", "swg_set_sequences", "swg_compute_block", "swg_compute_block_device",
     "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
     "swg_serve_create", "swg_serve_submit", "swg_serve_wait", "swg_serve_times", "swg_serve_release",
-    "swg_serve_stats", "swg_serve_destroy",
+    "swg_serve_stats", "swg_serve_destroy", "swg_set_option",
 )
 HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
     "salsa_write_matrix", "salsa_write_swms_block", "salsa_load_scoring_matrix",
-    "salsa_synth_dna", "salsa_synth_protein",
+    "salsa_synth_dna", "salsa_synth_protein", "salsa_swms_compute_rank", "salsa_read_restart_file",
 )
 
 
@@ -108,6 +108,7 @@ def lib():
         L.swg_serve_release.argtypes = [vp, i64]
         L.swg_serve_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
+        L.swg_set_option.argtypes = [vp, i32, i64]
         L.swg_serve_destroy.argtypes = [vp]
         L.swg_serve_destroy.restype = None
         L._typed = True
@@ -132,6 +133,12 @@ def host():
         H.salsa_synth_dna.restype = i64
         H.salsa_synth_protein.argtypes = [ctypes.c_uint64, i32, i32, vp, vp, i64, vp]
         H.salsa_synth_protein.restype = i64
+        H.salsa_swms_compute_rank.argtypes = [vp, i32, i32, i32, i32, i32, ctypes.c_char_p, ctypes.c_char_p,
+                                              ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                              ctypes.POINTER(ctypes.c_char_p), i32, i32, ctypes.POINTER(i32),
+                                              ctypes.POINTER(i64)]
+        H.salsa_read_restart_file.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                              ctypes.POINTER(i32)]
         H._typed = True
     return H
 
@@ -290,6 +297,9 @@ class Aligner:
     def set_schedule(self, schedule, strip_rows=0):
         _check(lib().swg_set_schedule(self._h, schedule, strip_rows))
 
+    def set_stream_chunk_bytes(self, nbytes):
+        _check(lib().swg_set_option(self._h, 1, int(nbytes)))
+
     def compute_block(self, row_start, row_end, col_start, col_end, is_diagonal, stats=False, out=None):
         """ComputeBlock: returns (distances[rows, cols], stats or None, pairs)."""
         rows, cols = row_end - row_start + 1, col_end - col_start + 1
@@ -330,6 +340,24 @@ class Aligner:
         rc = host().salsa_compute_full_matrix(self._h, n, world, _ptr(full), full.itemsize, ctypes.byref(pairs))
         _check(rc)
         return full, pairs.value
+
+    def swms_compute_rank(self, world, rank, block_dir, done_dir=None, last_saved_block=-1,
+                          block_write_frequency=1, project="salsa", sysname="mi355x", fasta="input.fasta",
+                          labels=None, stop_after=-1):
+        """SWMS Compute(lastSavedBlock) + WriteBlocks + done file for one rank;
+        returns (return code, last saved block, pairs)."""
+        n = len(self.seqs)
+        labels = labels or ["seq%d" % i for i in range(n)]
+        arr = (ctypes.c_char_p * n)(*[x.encode() for x in labels])
+        saved, pairs = ctypes.c_int32(), ctypes.c_int64()
+        rc = host().salsa_swms_compute_rank(self._h, n, world, rank, last_saved_block, block_write_frequency,
+                                            block_dir.encode(), done_dir.encode() if done_dir else None,
+                                            project.encode(), sysname.encode(), fasta.encode(), arr,
+                                            np.dtype(self.dtype).itemsize, stop_after, ctypes.byref(saved),
+                                            ctypes.byref(pairs))
+        if rc not in (SWG_OK, SWG_E_CAPACITY):
+            _check(rc)
+        return rc, saved.value, pairs.value
 
     def compute_rank(self, world, rank):
         n = len(self.seqs)
@@ -418,3 +446,12 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def read_restart_file(path, process_rank):
+    """ReadRestartFile (Salsa.SmithWatermanMS/Program.cs:245-262): (world, rank, lastSavedBlock)."""
+    w, r, ls = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    rc = host().salsa_read_restart_file(path.encode(), process_rank, ctypes.byref(w), ctypes.byref(r), ctypes.byref(ls))
+    if rc != SWG_OK:
+        raise SwgError(rc, "cannot read restart file %s" % path)
+    return w.value, r.value, ls.value

@@ -226,3 +226,112 @@ int salsa_load_scoring_matrix(const char* path, float* scores)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// SWMS: Compute(lastSavedBlock) + WriteBlocks + done file
+// (Salsa.SmithWatermanMS/Program.cs:1164-1227, 1081-1160, 188-201)
+
+#include <memory>
+
+namespace {
+
+struct UnsavedBlock {
+    int32_t j;
+    std::vector<uint8_t> data;
+};
+
+int write_unsaved(std::vector<UnsavedBlock>& unsaved, int32_t rank, const std::vector<int32_t>& rs,
+                  const std::vector<int32_t>& re, const char* block_dir, const char* project, const char* sysname,
+                  const char* fasta, const char* const* labels, int32_t elem)
+{
+    // WriteBlocks walks j from lastUnsavedBlock down to lastSavedBlock+1
+    for (auto it = unsaved.rbegin(); it != unsaved.rend(); ++it) {
+        const int32_t j = it->j;
+        const std::string path = std::string(block_dir) + "/" + std::to_string(rank) + "-" + std::to_string(j) + ".bin";
+        const char* names[4] = {labels ? labels[rs[rank]] : "", labels ? labels[re[rank]] : "",
+                                labels ? labels[rs[j]] : "", labels ? labels[re[j]] : ""};
+        const int64_t nelem = (int64_t)(re[rank] - rs[rank] + 1) * (re[j] - rs[j] + 1);
+        int rc = salsa_write_swms_block(path.c_str(), project, rank, sysname, fasta, rank, j, rs[rank], re[rank],
+                                        rs[j], re[j], names, it->data.data(), nelem, elem);
+        if (rc) return rc;
+    }
+    unsaved.clear();
+    return SWG_OK;
+}
+
+}  // namespace
+
+extern "C" int salsa_swms_compute_rank(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank, int32_t last_saved_block,
+                                       int32_t block_write_frequency, const char* block_dir, const char* done_dir,
+                                       const char* project, const char* sysname, const char* fasta,
+                                       const char* const* labels, int32_t elem, int32_t stop_after,
+                                       int32_t* last_saved, int64_t* pairs)
+{
+    if (!ctx || !block_dir || world < 1 || rank < 0 || rank >= world || (elem != 2 && elem != 8))
+        return SWG_E_ARGUMENT;
+    std::vector<int32_t> rs(world), re(world);
+    int rc = salsa_range_partition(0, n, world, rs.data(), re.data());
+    if (rc) return rc;
+    int64_t total = 0;
+    int32_t computed = 0, unsaved_count = 0, saved = last_saved_block;
+    std::vector<UnsavedBlock> unsaved;
+    const int64_t rows = re[rank] - rs[rank] + 1;
+    for (int32_t j = last_saved_block + 1; j < world; ++j) {
+        if (!salsa_block_is_computed(rank, j)) continue;
+        const int64_t cols = re[j] - rs[j] + 1;
+        UnsavedBlock ub;
+        ub.j = j;
+        ub.data.assign((size_t)(rows * cols * elem), 0);
+        int64_t done = 0;
+        rc = swg_compute_block(ctx, rs[rank], re[rank], rs[j], re[j], rank == j ? 1 : 0, ub.data.data(), cols,
+                               nullptr, &done);
+        if (rc) return rc;
+        total += done;
+        unsaved.push_back(std::move(ub));
+        ++computed;
+        ++unsaved_count;
+        if (block_write_frequency > 0 && unsaved_count == block_write_frequency) {
+            rc = write_unsaved(unsaved, rank, rs, re, block_dir, project, sysname, fasta, labels, elem);
+            if (rc) return rc;
+            unsaved_count = 0;
+            saved = j;
+        }
+        if (stop_after >= 0 && computed >= stop_after) {
+            if (last_saved) *last_saved = saved;
+            if (pairs) *pairs = total;
+            return SWG_E_CAPACITY;
+        }
+    }
+    if (block_write_frequency > 0 && unsaved_count > 0) {
+        rc = write_unsaved(unsaved, rank, rs, re, block_dir, project, sysname, fasta, labels, elem);
+        if (rc) return rc;
+        saved = world - 1;
+    }
+    if (done_dir) {
+        const std::string path = std::string(done_dir) + "/" + std::to_string(rank) + ".txt";
+        FILE* f = std::fopen(path.c_str(), "w");
+        if (!f) return SWG_E_ARGUMENT;
+        std::fprintf(f, "Done: Rank %d done.\n", rank);
+        std::fclose(f);
+    }
+    if (last_saved) *last_saved = saved;
+    if (pairs) *pairs = total;
+    return SWG_OK;
+}
+
+extern "C" int salsa_read_restart_file(const char* path, int32_t process_rank, int32_t* world, int32_t* rank,
+                                       int32_t* last_saved_block)
+{
+    FILE* f = path ? std::fopen(path, "r") : nullptr;
+    if (!f || !world || !rank || !last_saved_block) { if (f) std::fclose(f); return SWG_E_ARGUMENT; }
+    int w = 0;
+    int ok = std::fscanf(f, " %d", &w) == 1;
+    int r = -1, ls = -1;
+    for (int32_t k = 0; ok && k <= process_rank; ++k) ok = std::fscanf(f, " %d %d", &r, &ls) == 2;
+    std::fclose(f);
+    if (!ok) return SWG_E_ARGUMENT;
+    *world = w;
+    *rank = r;
+    *last_saved_block = ls;
+    return SWG_OK;
+}

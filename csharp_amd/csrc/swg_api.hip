@@ -136,6 +136,14 @@ struct swg_ctx {
     // outputs staged on device for the host-memory entry point
     DevBuf<uint8_t> out_stage;
     DevBuf<int32_t> stats_stage;
+    // streaming of large blocks: row chunks computed into alternating device
+    // buffers while a copy stream moves the previous chunk to pinned memory
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_comp[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+    DevBuf<uint8_t> stream_dev[2];
+    uint8_t* stream_host[2] = {nullptr, nullptr};
+    size_t stream_host_bytes = 0;
+    size_t stream_chunk_bytes = size_t(64) << 20;
 
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;  // 0 = autotune
@@ -756,6 +764,79 @@ int swg_synchronize(swg_ctx* c)
     return SWG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// A block whose output exceeds one chunk is computed in row chunks: chunk k
+// runs on the compute stream into device buffer k%2 while the copy stream
+// moves chunk k-1 to pinned memory and the host copies chunk k-2 into `out`.
+int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, void* out, int64_t out_ld,
+                           int64_t* pairs_done)
+{
+    const int es = elem_size(c);
+    const int64_t cols = c1 - c0 + 1;
+    const int64_t row_bytes = cols * es;
+    const int32_t chunk_rows = (int32_t)std::max<int64_t>(1, (int64_t)c->stream_chunk_bytes / row_bytes);
+    const size_t chunk_bytes = (size_t)chunk_rows * row_bytes;
+    if (!c->copy_stream) {
+        SWG_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b) {
+            SWG_HIP(hipEventCreateWithFlags(&c->ev_comp[b], hipEventDisableTiming));
+            SWG_HIP(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
+            SWG_HIP(hipEventRecord(c->ev_copy[b], c->copy_stream));
+        }
+    }
+    if (c->stream_host_bytes < chunk_bytes) {
+        for (int b = 0; b < 2; ++b) {
+            if (c->stream_host[b]) { SWG_HIP(hipHostFree(c->stream_host[b])); c->stream_host[b] = nullptr; }
+            SWG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->stream_host[b]), chunk_bytes, 0));
+        }
+        c->stream_host_bytes = chunk_bytes;
+    }
+    for (int b = 0; b < 2; ++b) SWG_HIP(c->stream_dev[b].reserve(chunk_bytes));
+    const int32_t nchunks = (int32_t)((r1 - r0 + 1 + chunk_rows - 1) / chunk_rows);
+    int64_t total_pairs = 0;
+    double cells = 0.0;
+    auto consume = [&](int32_t k) -> int {
+        const int b = k & 1;
+        const int32_t a0 = r0 + k * chunk_rows, a1 = std::min(r1, a0 + chunk_rows - 1);
+        SWG_HIP(hipEventSynchronize(c->ev_copy[b]));
+        uint8_t* dst = static_cast<uint8_t*>(out) + (int64_t)(a0 - r0) * out_ld * es;
+        for (int32_t i = 0; i < a1 - a0 + 1; ++i)
+            std::memcpy(dst + (int64_t)i * out_ld * es, c->stream_host[b] + (int64_t)i * row_bytes, row_bytes);
+        return SWG_OK;
+    };
+    for (int32_t k = 0; k < nchunks; ++k) {
+        const int b = k & 1;
+        const int32_t a0 = r0 + k * chunk_rows, a1 = std::min(r1, a0 + chunk_rows - 1);
+        // device buffer b is free once chunk k-2's copy finished
+        SWG_HIP(hipStreamWaitEvent(c->main.stream, c->ev_copy[b], 0));
+        int64_t p = 0;
+        int rc = enqueue_block(c, c->main, a0, a1, c0, c1, 0, c->stream_dev[b].p, cols, nullptr, &p);
+        if (rc) return rc;
+        total_pairs += p;
+        cells += c->last_cells;
+        SWG_HIP(hipEventRecord(c->ev_comp[b], c->main.stream));
+        SWG_HIP(hipStreamWaitEvent(c->copy_stream, c->ev_comp[b], 0));
+        SWG_HIP(hipMemcpyAsync(c->stream_host[b], c->stream_dev[b].p, (size_t)(a1 - a0 + 1) * row_bytes,
+                               hipMemcpyDeviceToHost, c->copy_stream));
+        SWG_HIP(hipEventRecord(c->ev_copy[b], c->copy_stream));
+        if (k >= 1 && (rc = consume(k - 1))) return rc;
+    }
+    int rc = consume(nchunks - 1);
+    if (rc) return rc;
+    SWG_HIP(hipStreamSynchronize(c->main.stream));
+    for (auto& B : c->main.bufs) B.pending = false;
+    c->last_cells = cells;
+    if (pairs_done) *pairs_done = total_pairs;
+    return SWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
                       void* out, int64_t out_ld, swg_pair_stats* stats, int64_t* pairs_done)
 {
@@ -765,6 +846,10 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
     const int64_t rows = r1 - r0 + 1, cols = c1 - c0 + 1;
     if (out_ld < cols) return fail(SWG_E_ARGUMENT, "out_ld smaller than the block width");
     const int es = elem_size(c);
+    const bool mirror = is_diag && c->p.diag_mode == SWG_DIAG_LOWER_MIRROR;
+    if (!mirror && !stats && (size_t)(rows * cols * es) > c->stream_chunk_bytes && c->res.p && r0 >= 0 && c0 >= 0 &&
+        r1 < c->nseq && c1 < c->nseq)
+        return compute_block_streamed(c, r0, r1, c0, c1, out, out_ld, pairs_done);
     const size_t out_bytes = (size_t)(rows * cols * es + 3) & ~size_t(3);
     SWG_HIP(c->out_stage.reserve(out_bytes));
     if (stats) SWG_HIP(c->stats_stage.reserve((size_t)rows * cols * 4));
@@ -790,6 +875,17 @@ int swg_set_schedule(swg_ctx* c, int32_t schedule, int32_t strip_rows)
     c->schedule = schedule;
     c->strip_rows = strip_rows;
     return SWG_OK;
+}
+
+int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
+{
+    if (!c) return fail(SWG_E_ARGUMENT, "null argument");
+    if (option == SWG_OPT_STREAM_CHUNK_BYTES) {
+        if (value < 1) return fail(SWG_E_ARGUMENT, "chunk size must be positive");
+        c->stream_chunk_bytes = (size_t)value;
+        return SWG_OK;
+    }
+    return fail(SWG_E_ARGUMENT, "unknown option");
 }
 
 int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t* strip_rows, int32_t* variant)
@@ -823,6 +919,14 @@ void swg_destroy(swg_ctx* c)
     }
     c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
     c->out_stage.release(); c->stats_stage.release();
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (int b = 0; b < 2; ++b) {
+        c->stream_dev[b].release();
+        if (c->stream_host[b]) (void)hipHostFree(c->stream_host[b]);
+        if (c->ev_comp[b]) (void)hipEventDestroy(c->ev_comp[b]);
+        if (c->ev_copy[b]) (void)hipEventDestroy(c->ev_copy[b]);
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->main.ev0) (void)hipEventDestroy(c->main.ev0);
     if (c->main.ev1) (void)hipEventDestroy(c->main.ev1);
     if (c->main.stream) (void)hipStreamDestroy(c->main.stream);

@@ -57,6 +57,29 @@ int salsa_write_swms_block(const char* path, const char* project, int32_t rank, 
                            int32_t row_start, int32_t row_end, int32_t col_start, int32_t col_end,
                            const char* const* names, const void* data, int64_t nelem, int32_t elem_size);
 
+/* SWMS rank driver: MPIComputeLowerTriangularBlocked.Compute(lastSavedBlock)
+ * with WriteBlocks and the done file (Salsa.SmithWatermanMS/Program.cs:
+ * 1164-1227, 1081-1160, 188-201).  Computes the rank's checkerboard blocks j >
+ * last_saved_block in order, keeps unsaved blocks in memory, writes
+ * "<block_dir>/<rank>-<j>.bin" for every unsaved owned block each time
+ * block_write_frequency blocks have been computed (and for the rest at the
+ * end), then "<done_dir>/<rank>.txt" ("Done: Rank <rank> done.").  labels[n]
+ * name the sequences in the block headers.  stop_after >= 0 stops after that
+ * many computed blocks without the final flush or done file (failure
+ * injection for restart tests; returns SWG_E_CAPACITY).  *blocks_saved gets
+ * the last saved block index (the restart file's lastSavedBlock). */
+int salsa_swms_compute_rank(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank, int32_t last_saved_block,
+                            int32_t block_write_frequency, const char* block_dir, const char* done_dir,
+                            const char* project, const char* sysname, const char* fasta,
+                            const char* const* labels, int32_t elem_size, int32_t stop_after,
+                            int32_t* last_saved, int64_t* pairs);
+
+/* ReadRestartFile (Salsa.SmithWatermanMS/Program.cs:245-262): the first line
+ * is the world size; line 1+k holds "<original rank> <last saved block>" for
+ * the k-th restarted process. */
+int salsa_read_restart_file(const char* path, int32_t process_rank, int32_t* world, int32_t* rank,
+                            int32_t* last_saved_block);
+
 /* ScoringMatrix.Load from a matrix text file into a 127x127 float table. */
 int salsa_load_scoring_matrix(const char* path, float* scores);
 

@@ -133,6 +133,12 @@ int  swg_synchronize(swg_ctx* ctx);
  * kernel (8, 16 or 32; 0 = default 16; for SWG_SCHED_PERSISTENT_LPP16, 8
  * selects 8 rows per lane, anything else 4). */
 int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
+/* Tuning knobs: SWG_OPT_STREAM_CHUNK_BYTES = device output per streamed row
+ * chunk of swg_compute_block (blocks larger than one chunk are computed chunk
+ * by chunk while the previous chunk is copied to pinned host memory;
+ * default 64 MiB). */
+#define SWG_OPT_STREAM_CHUNK_BYTES 1
+int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the
  * int32 alignment kernel and the variant that ran (1 plain, 2 int32 one-pass,

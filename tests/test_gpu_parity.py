@@ -306,3 +306,58 @@ def test_serving_same_alone_and_in_mix(ednafull):
         assert np.array_equal(out, d), "query %d vs oracle" % k
         assert np.array_equal(gst[:, :3], st[:, :3])
     srv.close()
+
+
+def test_streamed_block_matches(dna_set, ednafull):
+    # a block larger than the stream chunk is computed chunk by chunk with the
+    # copy of chunk k-1 overlapping chunk k
+    say("streamed block")
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(dna_set)
+    al.set_stream_chunk_bytes(2 * 50 * 7)   # 7 rows of 50 columns per chunk
+    n = len(dna_set)
+    got, _, pairs = al.compute_block(0, 99, n - 50, n - 1, False)
+    want, _ = oracle_block(dna_set, 0, 99, n - 50, n - 1, False, ednafull, 16, 4)
+    assert pairs == 100 * 50
+    assert np.array_equal(got, want)
+    al.close()
+
+
+def test_swms_restart_reproduces_block_files(tmp_path, ednafull):
+    # SWMS: an interrupted rank restarted from its last saved block writes the
+    # same block files as an uninterrupted run (Program.cs:1164-1227, 245-262)
+    say("swms restart")
+    s = ca.synth_dna(23, 160, 10, 0.10, 0.02, 0, [20, 120])
+    labels = ["s%03d" % i for i in range(160)]
+    full = tmp_path / "full"; full.mkdir()
+    part = tmp_path / "part"; part.mkdir()
+    done = tmp_path / "done"; done.mkdir()
+    al = ca.Aligner(ednafull, 16, 4, diag_mode=ca.DIAG_FULL)
+    al.set_sequences(s)
+    for rank in range(4):
+        rc, saved, pairs = al.swms_compute_rank(4, rank, str(full), str(done), labels=labels)
+        assert rc == ca.SWG_OK and saved == 3
+    # rank 0 owns blocks 0, 1, 3: stop after one block, restart from the restart file
+    rc, saved, _ = al.swms_compute_rank(4, 0, str(part), None, labels=labels, stop_after=1)
+    assert rc == ca.SWG_E_CAPACITY and saved == 0
+    restart = tmp_path / "restart.txt"
+    restart.write_text("4\n0\t%d\n" % saved)
+    world, rank, last = ca.read_restart_file(str(restart), 0)
+    assert (world, rank, last) == (4, 0, 0)
+    rc, saved, _ = al.swms_compute_rank(world, rank, str(part), str(done), last_saved_block=last, labels=labels)
+    assert rc == ca.SWG_OK
+    for j in (0, 1, 3):
+        a = (full / ("0-%d.bin" % j)).read_bytes()
+        b = (part / ("0-%d.bin" % j)).read_bytes()
+        assert a == b, "block 0-%d differs after restart" % j
+    assert not (full / "0-2.bin").exists()
+    assert (done / "0.txt").read_text() == "Done: Rank 0 done.\n"
+    # the block payload equals ComputeBlock's output (full diagonal block, SWMS semantics)
+    ranges = ca.range_partition(160, 4)
+    (r0, r1), (c0, c1) = ranges[0], ranges[1]
+    got, _, _ = al.compute_block(r0, r1, c0, c1, False)
+    payload = (full / "0-1.bin").read_bytes()[-got.nbytes:]
+    assert payload == got.tobytes()
+    want, _ = oracle_block(s, r0, r1, c0, c1, False, ednafull, 16, 4)
+    assert np.array_equal(got, want)
+    al.close()
