@@ -116,6 +116,7 @@ struct swg_ctx {
     bool count_similarity = false;  // protein: PercentSimilarity
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
     bool lpp_ok = false;            // non-negative penalties (pre-column cells of the 16-lane kernel)
+    bool nw = false;                // global alignment (NeedlemanWunschTPL)
 
     // sequences
     int32_t nseq = 0;
@@ -163,6 +164,7 @@ int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 
 // strip heights per variant: (first kernel, second kernel)
 void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
 {
+    if (c->nw) { *s1 = c->strip_rows == 8 ? 8 : 16; *s2 = 0; return; }
     if (v == kVarPlain) { *s1 = 16; *s2 = 0; return; }
     if (v == kVarLpp) { *s1 = c->strip_rows == 8 ? 8 : 4; *s2 = 0; return; }
     if (v == kVarI32) { *s1 = c->strip_rows ? c->strip_rows : 16; *s2 = 0; return; }
@@ -174,7 +176,8 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
 {
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
-    int a = v == kVarI16 ? score16_blocks_per_cu(s1)
+    int a = c->nw        ? nw_blocks_per_cu(s1)
+          : v == kVarI16 ? score16_blocks_per_cu(s1)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? sw_blocks_per_cu(s2, false) : 0;
@@ -188,7 +191,8 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
 // budget-capped: 24 GB over all lanes).
 int ensure_scratch(swg_ctx* c, Lane& ln, int* slots)
 {
-    const int64_t stride = (int64_t)(std::max(c->maxlen, c->slot_maxlen) + 16) * kWave;
+    // the global kernel keeps two more words per column (second half of the slot)
+    const int64_t stride = (int64_t)(std::max(c->maxlen, c->slot_maxlen) + 16) * kWave * (c->nw ? 2 : 1);
     const double budget = 24.0 * (1u << 30) / ln.share;
     const int step = std::max(1, c->num_cu / ln.share);
     while (*slots > step && (double)(*slots) * (double)stride * sizeof(int4) > budget) *slots -= step;
@@ -327,6 +331,7 @@ SeqSet make_seqset(const swg_ctx* c)
     S.prof = c->prof.p; S.ncode = c->ncode;
     S.gap_open = c->gap_open; S.gap_ext = c->gap_ext;
     S.tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
+    S.nw_region = c->nw && !c->count_similarity ? 1 : 0;
     return S;
 }
 
@@ -420,7 +425,9 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             hipError_t e = launch_fill_i32(B.queue.p, 0, 32, st);
             if (e == hipSuccess) {
-                if (v == kVarI32) {
+                if (c->nw) {
+                    e = launch_nw(S, T, W, s1, true, n1, st);
+                } else if (v == kVarI32) {
                     e = launch_sw(S, T, W, s1, true, n1, st);
                 } else if (v == kVarLpp) {
                     e = launch_lpp16(S, T, W, s1, true, n1, st);
@@ -444,7 +451,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     }
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
-        if (v == kVarPlain) SWG_HIP(launch_sw(S, T, W, s1, false, n1, st));
+        if (v == kVarPlain) SWG_HIP(c->nw ? launch_nw(S, T, W, s1, false, n1, st) : launch_sw(S, T, W, s1, false, n1, st));
         else SWG_HIP(hipGraphLaunch(B.graph[v], st));
     }
     SWG_HIP(hipEventRecord(ln.ev1, st));
@@ -502,6 +509,13 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant,
 {
     *trial = nullptr;
     if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
+    if (c->nw) {
+        // one global kernel (int32, one lane per pair, persistent)
+        if (c->schedule == SWG_SCHED_PERSISTENT_I16 || c->schedule == SWG_SCHED_PERSISTENT_LPP16)
+            return fail(SWG_E_UNSUPPORTED, "the global alignment has only the int32 one-pass kernel");
+        *variant = kVarI32;
+        return SWG_OK;
+    }
     if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) {
         if (!c->lpp_ok) return fail(SWG_E_UNSUPPORTED, "the 16-lanes-per-pair kernel needs non-negative gap penalties");
         *variant = kVarLpp;
@@ -598,6 +612,15 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
         maxlen = std::max<int32_t>(maxlen, (int32_t)L);
         for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) seen[residues[k]] = true;
     }
+    if (c->nw) {
+        // global scores have no 0 floor: every value must stay far from the
+        // int32 stand-in for -Infinity
+        int64_t maxabs = 0;
+        for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k) maxabs = std::max<int64_t>(maxabs, std::abs(c->iscores[k]));
+        const int64_t bound = 2 * ((int64_t)maxlen + 8) * (std::abs((int64_t)c->gap_open) + std::abs((int64_t)c->gap_ext) + maxabs);
+        if (bound >= (int64_t(1) << 28))
+            return fail(SWG_E_CAPACITY, "global alignment scores could exceed the int32 range at this sequence length");
+    }
     if (matrix_symbols)
         for (int x = 0; x < SWG_MATRIX_SIZE; ++x)
             for (int y = 0; y < SWG_MATRIX_SIZE; ++y)
@@ -627,6 +650,9 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
                 const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
                 int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
                 if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
+                // global protein distance: non-gap columns scoring > 0 only
+                // (Salsa.NeedlemanWunschTPL/Program.cs:583-604)
+                if (c->nw && c->count_similarity) cnt = sc > 0 ? 1 : 0;
                 v.x = sc + c->gap_open;
                 v.y = 0x10000 + cnt;
             }
@@ -693,8 +719,6 @@ int swg_create(swg_ctx** out, const swg_params* p)
         return fail(SWG_E_ARGUMENT, "bad schedule");
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
         return fail(SWG_E_UNSUPPORTED, "only PercentIdentity is implemented for nucleic alignments");
-    if (p->mode == SWG_MODE_NW_GLOBAL)
-        return fail(SWG_E_UNSUPPORTED, "the global (NW) kernel is not available in this build");
     if (!is_integral(p->gap_open) || !is_integral(p->gap_ext))
         return fail(SWG_E_NONINTEGRAL, "gap penalties must be integers below 32768 for the integer kernels");
     for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k)
@@ -708,6 +732,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
     c->gap_open = (int32_t)p->gap_open;
     c->gap_ext = (int32_t)p->gap_ext;
     c->count_similarity = p->alignment_type == SWG_ALIGN_PROTEIN;
+    c->nw = p->mode == SWG_MODE_NW_GLOBAL;
     c->schedule = p->schedule;
     {
         // the int16 score pass holds score + gap_open and the pad score in int16
@@ -1043,7 +1068,8 @@ int launch_batch(swg_server* s, ServeLane& L)
     // trials on the serving path
     BlockGeom g;
     g.r0 = L.slot_base; g.r1 = L.slot_base + nb - 1; g.c0 = 0; g.c1 = s->npanel - 1; g.diag_lower = false;
-    c->schedule = (c->lpp_ok && thin_block(c, L.lane, g)) ? SWG_SCHED_PERSISTENT_LPP16
+    c->schedule = c->nw                                  ? SWG_SCHED_PERSISTENT_I32
+                : (c->lpp_ok && thin_block(c, L.lane, g)) ? SWG_SCHED_PERSISTENT_LPP16
                 : c->int16_ok                            ? SWG_SCHED_PERSISTENT_I16
                                                          : SWG_SCHED_PERSISTENT_I32;
     int64_t pairs = 0;

@@ -33,6 +33,7 @@ struct SeqSet {
     int32_t ncode;         // includes the pad code (= ncode-1)
     int32_t gap_open, gap_ext;
     int32_t tdist;         // 0 int16, 1 double
+    int32_t nw_region;     // global variant: distance over the aligned region (nucleic identity)
 };
 
 // Task lists: task t aligns s1 = task_row[t] against the 64 sequences of
@@ -115,6 +116,7 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
 int score16_blocks_per_cu(int strip_rows);
+int nw_blocks_per_cu(int strip_rows);
 // Fill kernels (graph-capturable replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_fill_bytes_zero(void* p, int64_t nbytes, hipStream_t stream);

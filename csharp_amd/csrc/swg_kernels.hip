@@ -34,12 +34,10 @@ __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlan
 //   Double: (double)(1.0f - pct)
 // The double quotient rounded to float is the correctly rounded binary32
 // quotient (53 >= 2*24+2).  Built with -ffp-contract=off.
-__device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc& D, int a, int b,
-                                               int score, int Tb)
+__device__ __forceinline__ void write_fraction(const SeqSet& S, const BlockDesc& D, int a, int b, uint32_t cnt,
+                                               uint32_t den, int4 st)
 {
-    const uint32_t cols = (uint32_t)Tb >> 16;
-    const uint32_t cnt = (uint32_t)Tb & 0xFFFFu;
-    const float pct = (float)((double)cnt / (double)cols);
+    const float pct = (float)((double)cnt / (double)den);
     const float rr = 1.0f - pct;
     const int64_t pos = (int64_t)(a - D.r0) * D.ld + (b - D.c0);
     const int64_t mpos = (int64_t)(b - D.r0) * D.ld + (a - D.c0);
@@ -55,9 +53,16 @@ __device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc&
         if (D.mirror) static_cast<short*>(D.out)[mpos] = sv;
     }
     if (D.stats) {
-        reinterpret_cast<int4*>(D.stats)[pos] = make_int4(score, (int)cols, (int)cnt, 0);
-        if (D.mirror) reinterpret_cast<int4*>(D.stats)[mpos] = make_int4(score, (int)cols, (int)cnt, 0);
+        reinterpret_cast<int4*>(D.stats)[pos] = st;
+        if (D.mirror) reinterpret_cast<int4*>(D.stats)[mpos] = st;
     }
+}
+
+__device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc& D, int a, int b,
+                                               int score, int Tb)
+{
+    const uint32_t cols = (uint32_t)Tb >> 16, cnt = (uint32_t)Tb & 0xFFFFu;
+    write_fraction(S, D, a, b, cnt, cols, make_int4(score, (int)cols, (int)cnt, 0));
 }
 
 // Stage the strip's query profile: lprof[c2][r] = prof[s1[i0+r]][c2] for
@@ -373,6 +378,156 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
 }
 
 // ---------------------------------------------------------------------------
+// Global (Needleman-Wunsch) alignment, one lane per pair, same strip layout
+// as sw_kernel.  Recurrence of SmithWatermanGotoh.Construct without the 0
+// floor, borders V[i][0] = -(o+(i-1)e) and V[0][j] = -(o+(j-1)e) as single gap
+// runs back to the origin, result at (m, n), DIAGONAL > UP > LEFT.
+// The distance of Salsa.NeedlemanWunschTPL/Program.cs:269-300 needs, besides
+// (columns, count), the aligned region [first non-gap, last non-gap]: its
+// length is columns - lead - trail, where lead is the border run the path
+// starts with and trail the run of same-direction gap columns it ends with.
+// A second packed word carries them: T2 = (lead << 16) | (up << 15) | run,
+// `run` = length of the gap run the traceback starts with (up = vertical).
+constexpr uint32_t kUpBit = 0x8000u;
+
+__device__ __forceinline__ int nw_border(int o, int e, int k) { return -(o + (k - 1) * e); }
+
+template <int R, bool PERSIST>
+__global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
+{
+    __shared__ int4 lprof[kMaxCodes * (R / 2 + 1)];
+    const int lane = threadIdx.x;
+    // per slot: (t, G, T1, TG1) then (T2, TG2), each [column][lane]
+    int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
+    int2* __restrict__ bnd2 = reinterpret_cast<int2*>(bnd + W.bnd_stride / 2);
+    const int ntasks = T.desc->ntasks;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int pad = S.ncode - 1;
+
+    for (int iter = 0;; ++iter) {
+        int t;
+        if (PERSIST) {
+            int v = 0;
+            if (lane == 0) v = atomicAdd(T.queue, 1);
+            t = uni(v);
+        } else {
+            t = blockIdx.x + iter * (int)gridDim.x;
+        }
+        if (t >= ntasks) break;
+
+        const int a = uni(T.task_row[t]);
+        const int tile = uni(T.task_tile[t]);
+        const int maxLb = uni(T.tile_maxlen[tile]);
+        const int b = T.tile_col[(int64_t)tile * kWave + lane];
+        const int La = uni(S.len[a]);
+        const int Lb = b >= 0 ? S.len[b] : 0;
+        const uint8_t* s1 = S.res + S.off[a];
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (b >= 0 ? S.off[b] : S.pad_off));
+        const int ncols = (maxLb + 3) & ~3;
+
+        // row 0: a horizontal gap run of j columns from the origin
+        for (int j = 1; j <= ncols; ++j) {
+            bnd[(int64_t)j * kWave + lane] = make_int4(nw_border(o, e, j) - o, kNeg, j << 16, 0);
+            bnd2[(int64_t)j * kWave + lane] = make_int2((j << 16) | j, 0);
+        }
+
+        // result cell (La, Lb): border cells when a sequence is empty
+        int capV = 0;
+        uint32_t cap1 = 0, cap2 = 0;
+        if (La == 0 && Lb > 0) { capV = nw_border(o, e, Lb); cap1 = (uint32_t)Lb << 16; cap2 = ((uint32_t)Lb << 16) | Lb; }
+        if (Lb == 0 && La > 0) {
+            capV = nw_border(o, e, La); cap1 = (uint32_t)La << 16; cap2 = ((uint32_t)La << 16) | kUpBit | La;
+        }
+
+        for (int i0 = 0; i0 < La; i0 += R) {
+            __syncthreads();
+            stage_profile<R>(lprof, S, s1, La, i0, lane);
+            __syncthreads();
+            const int rcap = La - 1 - i0;  // row of the result cell in this strip (if < R)
+
+            int tr[R], Tr1[R], Tr2[R], Hr[R], TH1[R], TH2[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + r + 1;  // column-0 border: a vertical run of i rows
+                tr[r] = nw_border(o, e, i) - o; Tr1[r] = i << 16; Tr2[r] = (i << 16) | (int)kUpBit | i;
+                Hr[r] = kNeg; TH1[r] = 0; TH2[r] = 0;
+            }
+            int dtb, dTb1, dTb2;  // V[i0][0] for the first row's diagonal
+            if (i0 == 0) { dtb = -o; dTb1 = 0; dTb2 = 0; }
+            else { dtb = nw_border(o, e, i0) - o; dTb1 = i0 << 16; dTb2 = (i0 << 16) | (int)kUpBit | i0; }
+
+            for (int j = 1; j <= ncols; ++j) {
+                const uint32_t w = s2w[(j - 1) >> 2];
+                int c2 = (int)((w >> (8 * ((j - 1) & 3))) & 0xFFu);
+                c2 = j > Lb ? pad : c2;
+                const int4 bu = bnd[(int64_t)j * kWave + lane];
+                const int2 bu2 = bnd2[(int64_t)j * kWave + lane];
+                int tu = bu.x, Gu = bu.y, Tu1 = bu.z, TGu1 = bu.w, Tu2 = bu2.x, TGu2 = bu2.y;
+                int dt = dtb, dT1 = dTb1, dT2 = dTb2;
+                dtb = tu; dTb1 = Tu1; dTb2 = Tu2;
+                const bool hitcol = j == Lb;
+                const int4* pr = lprof + c2 * (R / 2 + 1);
+#pragma unroll
+                for (int r2 = 0; r2 < R / 2; ++r2) {
+                    const int4 pp = pr[r2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int r = 2 * r2 + h;
+                        const int sc = h ? pp.z : pp.x;
+                        const int inc = h ? pp.w : pp.y;
+                        // vertical gap: extend TG or open from T (keeping its run if vertical)
+                        const int g1 = Gu - e;
+                        const bool cg = g1 > tu;
+                        const int G = cg ? g1 : tu;
+                        const int TG1 = (cg ? TGu1 : Tu1) + 0x10000;
+                        const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
+                        const int TG2 = (cg ? TGu2 : baseG) + 1;
+                        // horizontal gap
+                        const int h1 = Hr[r] - e;
+                        const bool ch = h1 > tr[r];
+                        Hr[r] = ch ? h1 : tr[r];
+                        const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
+                        TH1[r] = (ch ? TH1[r] : Tr1[r]) + 0x10000;
+                        TH2[r] = (ch ? TH2[r] : baseH) + 1;
+                        const int f = dt + sc;
+                        const int V = max(max(f, G), Hr[r]);
+                        int x1 = (V == G) ? TG1 : TH1[r];
+                        int x2 = (V == G) ? (TG2 | (int)kUpBit) : TH2[r];
+                        const bool dg = V == f;
+                        x1 = dg ? dT1 + inc : x1;
+                        x2 = dg ? (dT2 & (int)0xFFFF0000u) : x2;
+                        const int tn = V - o;
+                        if (r == rcap) {
+                            capV = hitcol ? V : capV;
+                            cap1 = hitcol ? (uint32_t)x1 : cap1;
+                            cap2 = hitcol ? (uint32_t)x2 : cap2;
+                        }
+                        dt = tr[r]; dT1 = Tr1[r]; dT2 = Tr2[r];
+                        tr[r] = tn; Tr1[r] = x1; Tr2[r] = x2;
+                        tu = tn; Gu = G; Tu1 = x1; TGu1 = TG1; Tu2 = x2; TGu2 = TG2;
+                    }
+                }
+                bnd[(int64_t)j * kWave + lane] = make_int4(tu, Gu, Tu1, TGu1);
+                bnd2[(int64_t)j * kWave + lane] = make_int2(Tu2, TGu2);
+            }
+        }
+
+        if (b >= 0) {
+            const uint32_t cols = cap1 >> 16, cnt = cap1 & 0xFFFFu;
+            const uint32_t lead = cap2 >> 16, trail = cap2 & 0x7FFFu;
+            // region = last - first + 1 over the non-gap indices; with both
+            // sequences empty both indices are -1 and the region counts 1
+            int region = (La == 0 && Lb == 0) ? 1 : (La == 0 || Lb == 0) ? 0 : (int)cols - (int)lead - (int)trail;
+            if (region < 0) region = 0;
+            // nucleic: identity over the aligned region (Program.cs:547-571);
+            // protein: positive-score columns over all columns (:583-604)
+            write_fraction(S, *T.desc, a, b, cnt, S.nw_region ? (uint32_t)region : cols,
+                           make_int4(capV, (int)cols, (int)cnt, region));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Score-only pass, two pairs per lane in packed int16 halves.
 //
 // Same strip structure and query profile as sw_kernel, but only the score
@@ -641,9 +796,33 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
     }
 }
 
-hipError_t launch_nw(const SeqSet&, const TaskSet&, const Scratch&, int, bool, int, hipStream_t)
+hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool persistent, int nslots,
+                     hipStream_t st)
 {
-    return hipErrorNotSupported;
+    switch (strip_rows) {
+    case 8:
+        if (persistent) hipLaunchKernelGGL((nw_kernel<8, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        else hipLaunchKernelGGL((nw_kernel<8, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        break;
+    case 16:
+        if (persistent) hipLaunchKernelGGL((nw_kernel<16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        else hipLaunchKernelGGL((nw_kernel<16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int nw_blocks_per_cu(int strip_rows)
+{
+    int nb = 0;
+    hipError_t err = hipErrorInvalidValue;
+    switch (strip_rows) {
+    case 8: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<8, true>, kWave, 0); break;
+    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<16, true>, kWave, 0); break;
+    default: break;
+    }
+    return err == hipSuccess ? nb : 0;
 }
 
 size_t lpp16_lds_bytes(int strip_rows, int ncode)

@@ -148,3 +148,12 @@ def sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True
     if rc != 0:
         raise IndexError("symbol outside the 127-entry scoring matrix")
     return out, st
+
+
+def nw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True, tdist_double=False, threads=1,
+                want_stats=False):
+    """Global-alignment counterpart of sw_pairs_mt (oracle_nw_literal and the
+    distance of Salsa.NeedlemanWunschTPL/Program.cs:269-300); stats rows are
+    (score, columns, count, region length)."""
+    return sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=nucleic, dist_fn=-1,
+                       tdist_double=tdist_double, threads=threads, want_stats=want_stats)

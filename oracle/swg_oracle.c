@@ -446,7 +446,7 @@ typedef struct {
     const uint8_t* residues; const int64_t* offsets; const int32_t* lengths;
     const int32_t* a; const int32_t* b; int64_t npairs;
     const float* scores; float o, e;
-    int32_t nucleic, dist_fn, tdist_double;
+    int32_t nucleic, dist_fn, tdist_double, global;
     void* out; int32_t* out_stats;
     int64_t next; pthread_mutex_t mu;
     int32_t err;
@@ -465,12 +465,29 @@ static void* mt_worker(void* arg)
         for (int64_t p = lo; p < hi; ++p) {
             int32_t ia = J->a[p], ib = J->b[p];
             oracle_result r;
-            if (oracle_sw_literal(J->residues + J->offsets[ia], J->lengths[ia],
-                                  J->residues + J->offsets[ib], J->lengths[ib],
-                                  J->scores, J->o, J->e, &r, NULL, NULL) != 0) {
-                J->err = -1; continue;
-            }
+            int rc = J->global
+                ? oracle_nw_literal(J->residues + J->offsets[ia], J->lengths[ia], J->residues + J->offsets[ib],
+                                    J->lengths[ib], J->scores, J->o, J->e, &r, NULL, NULL)
+                : oracle_sw_literal(J->residues + J->offsets[ia], J->lengths[ia], J->residues + J->offsets[ib],
+                                    J->lengths[ib], J->scores, J->o, J->e, &r, NULL, NULL);
+            if (rc != 0) { J->err = -1; continue; }
             double d = 0.0; int16_t d16 = 0;
+            if (J->global) {
+                /* Salsa.NeedlemanWunschTPL/Program.cs:269-300 with
+                 * ComputePercentIdentity (:547-571) / ComputePercentSimilarity (:583-604) */
+                int32_t cnt = J->nucleic ? r.nw_region_identity : r.nw_similarity;
+                int32_t den = J->nucleic ? r.nw_region_len : r.columns;
+                d16 = oracle_dist_i16_fraction(cnt, den);
+                d = oracle_dist_f64_fraction(cnt, den);
+                if (J->tdist_double) ((double*)J->out)[p] = d; else ((int16_t*)J->out)[p] = d16;
+                if (J->out_stats) {
+                    J->out_stats[4 * p + 0] = (int32_t)r.score;
+                    J->out_stats[4 * p + 1] = r.columns;
+                    J->out_stats[4 * p + 2] = cnt;
+                    J->out_stats[4 * p + 3] = r.nw_region_len;
+                }
+                continue;
+            }
             if (!J->nucleic) {
                 d16 = oracle_dist_i16_fraction(r.similarity, r.columns);
                 d = oracle_dist_f64_fraction(r.similarity, r.columns);
@@ -507,7 +524,7 @@ int oracle_sw_pairs_mt(const uint8_t* residues, const int64_t* offsets, const in
     mt_job J;
     J.residues = residues; J.offsets = offsets; J.lengths = lengths;
     J.a = a; J.b = b; J.npairs = npairs; J.scores = scores; J.o = o; J.e = e;
-    J.nucleic = nucleic; J.dist_fn = dist_fn; J.tdist_double = tdist_double;
+    J.nucleic = nucleic; J.dist_fn = dist_fn; J.tdist_double = tdist_double; J.global = dist_fn < 0;
     J.out = out; J.out_stats = out_stats; J.next = 0; J.err = 0;
     pthread_mutex_init(&J.mu, NULL);
     if (threads < 1) threads = 1;

@@ -88,7 +88,11 @@ int16_t oracle_f64_to_i16(double x);
  * Salsa.SmithWatermanMPI/Program.cs:259-314 for distance function `dist_fn`
  * (0 PercentIdentity, 1 Kimura2, 2 JukesCantor) and alignment type
  * `nucleic` (0 = protein -> PercentSimilarity).  `out_stats` (optional) gets
- * (score, columns, identity, similarity) as 4 int32 per pair. */
+ * (score, columns, identity, similarity) as 4 int32 per pair.
+ * dist_fn = -1 selects the global aligner oracle_nw_literal and the distance
+ * of Salsa.NeedlemanWunschTPL/Program.cs:269-300 (nucleic: identity over the
+ * aligned region; protein: positive-score columns over all columns);
+ * out_stats then gets (score, columns, count, region length). */
 int oracle_sw_pairs_mt(const uint8_t* residues, const int64_t* offsets, const int32_t* lengths,
                        const int32_t* a, const int32_t* b, int64_t npairs,
                        const float* scores, float gap_open, float gap_ext,
