@@ -54,7 +54,7 @@ struct DevBuf {
 enum Variant : int {
     kVarPlain = 0,     // sw_kernel<16, false>: natural order, static assignment
     kVarI32 = 1,       // sw_kernel<R, true>: one pass, int32, statistics carried
-    kVarI16 = 2,       // score16_kernel (int16x2) + group_kernel + bounded sw_kernel
+    kVarI16 = 2,       // score16_kernel (int16x2) + sort by first-maximum cell + rerun_kernel
     kVarLpp = 3,       // lpp16_kernel: 16 lanes per pair, 4 pairs per wave
     kNumVariants = 4,
 };
@@ -65,21 +65,30 @@ constexpr int kScoreTile = 2 * kWave;  // pairs per score-pass task
 // host can plan block k+1 while block k runs.
 struct BlockBufs {
     // alignment tasks (single pass, or the bounded second pass)
-    DevBuf<int32_t> task_row, task_tile, tile_col, tile_maxlen, task_rowlim, tile_lim;
+    DevBuf<int32_t> task_row, task_tile, tile_col, tile_maxlen;
     // score pass
     DevBuf<int32_t> s_task_row, s_task_tile, s_tile_col, s_tile_maxlen;
     DevBuf<uint32_t> best_cell;
-    DevBuf<GroupChunk> chunks;
-    DevBuf<int32_t> nchunks;
-    DevBuf<int32_t> queue;        // [0]: first pass, [16]: second pass
+    DevBuf<uint32_t> sorted;      // re-run order (slot -> pair position)
+    DevBuf<int32_t> sortbuf;      // bucket count, cursor, base, then meta
+    DevBuf<int32_t> queue;        // [0]: first pass, [16]: flagged re-run, [32]: capture re-run
     DevBuf<BlockDesc> desc;       // [0]: output desc, [1]: score-pass desc
     std::vector<int32_t> h_task_row, h_task_tile, h_tile_col, h_tile_maxlen;
-    std::vector<GroupChunk> h_chunks;
     hipEvent_t done = nullptr;
     bool pending = false;
     // one captured graph per variant, keyed by every baked launch argument
     hipGraphExec_t graph[kNumVariants] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<unsigned char> graph_key[kNumVariants];
+
+    // device memory and the completion event (graphs are destroyed by the owner)
+    void release()
+    {
+        task_row.release(); task_tile.release(); tile_col.release(); tile_maxlen.release();
+        s_task_row.release(); s_task_tile.release(); s_tile_col.release(); s_tile_maxlen.release();
+        best_cell.release(); sorted.release(); sortbuf.release(); queue.release(); desc.release();
+        if (done) (void)hipEventDestroy(done);
+        done = nullptr;
+    }
 };
 
 // Autotuner record of one key (see choose_variant).
@@ -169,7 +178,7 @@ void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
     if (v == kVarLpp) { *s1 = c->strip_rows == 8 ? 8 : 4; *s2 = 0; return; }
     if (v == kVarI32) { *s1 = c->strip_rows ? c->strip_rows : 16; *s2 = 0; return; }
     *s1 = 16;  // score pass
-    *s2 = c->strip_rows ? c->strip_rows : 16;
+    *s2 = c->strip_rows == 32 ? 32 : 16;  // re-run
 }
 
 int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
@@ -180,7 +189,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
           : v == kVarI16 ? score16_blocks_per_cu(s1)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
-    int b = v == kVarI16 ? sw_blocks_per_cu(s2, false) : 0;
+    int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
     if (a <= 0 || (v == kVarI16 && b <= 0)) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
     *n2 = std::max(1, b * c->num_cu / ln.share);
@@ -210,8 +219,6 @@ struct Plan {
     int64_t pairs = 0;
     double cells = 0.0;
     int32_t ntasks = 0;   // tasks of the (first) kernel
-    int32_t ntasks2 = 0;  // bounded second pass
-    int32_t nchunks = 0;
 };
 
 struct BlockGeom {
@@ -288,34 +295,6 @@ void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, st
     plan.ntasks = (int32_t)trow.size();
 }
 
-// Grouping chunks of the two-pass variant: for every row (same order as the
-// score-pass tasks) its partners j (contiguous column indices: the whole
-// column range, or c0..i-1 on a lower-mirror diagonal block) in chunks of
-// kGroupChunk, each emitting ceil(n/64) bounded tasks.
-void plan_chunks(const swg_ctx* c, const BlockGeom& g, std::vector<GroupChunk>& chunks, Plan& plan)
-{
-    chunks.clear();
-    std::vector<int32_t> rows, cols;
-    order_block(c, g, true, rows, cols);
-    const int64_t ld = g.c1 - g.c0 + 1;
-    int32_t base = 0;
-    for (int32_t a : rows) {
-        const int32_t last = g.diag_lower ? a - 1 : g.c1;
-        for (int32_t f = g.c0; f <= last; f += kGroupChunk) {
-            GroupChunk ch;
-            ch.row = a;
-            ch.first_col = f;
-            ch.n = std::min<int32_t>(kGroupChunk, last - f + 1);
-            ch.task_base = base;
-            ch.cell_off = (int64_t)(a - g.r0) * ld + (f - g.c0);
-            base += (ch.n + kWave - 1) / kWave;
-            chunks.push_back(ch);
-        }
-    }
-    plan.ntasks2 = base;
-    plan.nchunks = (int32_t)chunks.size();
-}
-
 template <typename T>
 int upload(hipStream_t st, DevBuf<T>& d, const std::vector<T>& h)
 {
@@ -360,21 +339,25 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     BlockDesc desc[2] = {};
     if (v == kVarI16) {
         plan_tiles(c, g, true, kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
-        plan_chunks(c, g, B.h_chunks, plan);
         if ((rc = upload(st, B.s_task_row, B.h_task_row)) || (rc = upload(st, B.s_task_tile, B.h_task_tile)) ||
-            (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)) ||
-            (rc = upload(st, B.chunks, B.h_chunks)))
+            (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)))
             return rc;
-        const size_t nt2 = std::max<int32_t>(plan.ntasks2, 1);
-        SWG_HIP(B.task_row.reserve(nt2)); SWG_HIP(B.task_tile.reserve(nt2)); SWG_HIP(B.task_rowlim.reserve(nt2));
-        SWG_HIP(B.tile_maxlen.reserve(nt2));
-        SWG_HIP(B.tile_col.reserve(nt2 * kWave)); SWG_HIP(B.tile_lim.reserve(nt2 * kWave));
-        SWG_HIP(B.best_cell.reserve(std::max<size_t>((size_t)(g.r1 - g.r0 + 1) * (size_t)(g.c1 - g.c0 + 1), 1)));
-        SWG_HIP(B.nchunks.reserve(1));
-        SWG_HIP(hipMemcpyAsync(B.nchunks.p, &plan.nchunks, sizeof(int32_t), hipMemcpyHostToDevice, st));
-        desc[0].ntasks = plan.ntasks2;
+        const size_t cells = (size_t)(g.r1 - g.r0 + 1) * (size_t)(g.c1 - g.c0 + 1);
+        if (cells + (size_t)kWave * kNumBuckets >= (size_t(1) << 32))
+            return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^32 pairs)");
+        SWG_HIP(B.best_cell.reserve(std::max<size_t>(cells, 1)));
+        SWG_HIP(B.sorted.reserve(cells + (size_t)kWave * kNumBuckets));
+        SWG_HIP(B.sortbuf.reserve(3 * kNumBuckets + 64));
+        // bucket size: the block's longest sequences over kBucketDim buckets
+        int32_t max_r = 1, max_c = 1;
+        for (int32_t i = g.r0; i <= g.r1; ++i) max_r = std::max(max_r, c->h_len[i]);
+        for (int32_t j = g.c0; j <= g.c1; ++j) max_c = std::max(max_c, c->h_len[j]);
+        desc[0].ntasks = 0;
         desc[1].ntasks = plan.ntasks;
         desc[1].r0 = g.r0; desc[1].c0 = g.c0; desc[1].ld = g.c1 - g.c0 + 1;
+        desc[1].rows = g.r1 - g.r0 + 1;
+        desc[1].wr = (max_r + kBucketDim - 1) / kBucketDim;
+        desc[1].wc = (max_c + kBucketDim - 1) / kBucketDim;
     } else {
         plan_tiles(c, g, v != kVarPlain, v == kVarLpp ? kLppTile : kWave, B.h_task_row, B.h_task_tile, B.h_tile_col,
                    B.h_tile_maxlen, plan);
@@ -397,18 +380,20 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     TaskSet T{};
     T.task_row = B.task_row.p; T.task_tile = B.task_tile.p; T.tile_col = B.tile_col.p;
     T.tile_maxlen = B.tile_maxlen.p; T.desc = B.desc.p; T.queue = B.queue.p;
-    T.task_rowlim = nullptr; T.tile_lim = nullptr;
     ScoreTaskSet ST{};
-    GroupSet G{};
-    const int group_grid = std::max(1, c->num_cu * 4 / ln.share);
+    SortSet P{};
+    RerunSet RT{}, RC{};
+    const int sort_grid = std::max(1, c->num_cu * 4 / ln.share);
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
-        G.chunks = B.chunks.p; G.nchunks = B.nchunks.p; G.best_cell = B.best_cell.p;
-        G.task_row = B.task_row.p; G.task_tile = B.task_tile.p; G.task_rowlim = B.task_rowlim.p;
-        G.tile_col = B.tile_col.p; G.tile_lim = B.tile_lim.p; G.tile_maxlen = B.tile_maxlen.p;
-        T.queue = B.queue.p + 16;
-        T.task_rowlim = B.task_rowlim.p; T.tile_lim = B.tile_lim.p;
+        P.best_cell = B.best_cell.p; P.desc = B.desc.p;
+        P.count = B.sortbuf.p; P.cursor = B.sortbuf.p + kNumBuckets; P.base = B.sortbuf.p + 2 * kNumBuckets;
+        P.meta = B.sortbuf.p + 3 * kNumBuckets; P.sorted = B.sorted.p;
+        RT.sorted = B.sorted.p; RT.best_cell = B.best_cell.p; RT.desc = B.desc.p; RT.meta = P.meta;
+        RC = RT;
+        RT.queue = B.queue.p + 16;
+        RC.queue = B.queue.p + 32;
     }
 
     // capture (or re-capture) before the timing events so that host-side
@@ -417,13 +402,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         // every pointer and scalar baked into the graph is part of its key:
         // re-capture whenever one changed (new sequences, grown buffers)
         std::vector<unsigned char> key;
-        append_key(key, S); append_key(key, T); append_key(key, W); append_key(key, ST); append_key(key, G);
+        append_key(key, S); append_key(key, T); append_key(key, W); append_key(key, ST); append_key(key, P);
+        append_key(key, RT); append_key(key, RC); append_key(key, sort_grid);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         if (!B.graph[v] || key != B.graph_key[v]) {
             if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
             hipGraph_t gr = nullptr;
             SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            hipError_t e = launch_fill_i32(B.queue.p, 0, 32, st);
+            hipError_t e = launch_fill_i32(B.queue.p, 0, 48, st);
             if (e == hipSuccess) {
                 if (c->nw) {
                     e = launch_nw(S, T, W, s1, true, n1, st);
@@ -432,9 +418,11 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
                 } else if (v == kVarLpp) {
                     e = launch_lpp16(S, T, W, s1, true, n1, st);
                 } else {
-                    e = launch_score16(S, ST, W, s1, n1, st);
-                    if (e == hipSuccess) e = launch_group(G, group_grid, st);
-                    if (e == hipSuccess) e = launch_sw(S, T, W, s2, true, n2, st);
+                    e = launch_fill_i32(P.count, 0, kNumBuckets, st);
+                    if (e == hipSuccess) e = launch_score16(S, ST, W, s1, n1, st);
+                    if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
+                    if (e == hipSuccess) e = launch_rerun(S, RT, W, s2, true, n2, st);
+                    if (e == hipSuccess) e = launch_rerun(S, RC, W, s2, false, n2, st);
                 }
             }
             hipError_t e2 = hipStreamEndCapture(st, &gr);
@@ -935,12 +923,7 @@ void swg_destroy(swg_ctx* c)
     for (auto& B : c->main.bufs) {
         for (auto& gexec : B.graph)
             if (gexec) (void)hipGraphExecDestroy(gexec);
-        B.task_row.release(); B.task_tile.release(); B.tile_col.release(); B.tile_maxlen.release();
-        B.task_rowlim.release(); B.tile_lim.release();
-        B.s_task_row.release(); B.s_task_tile.release(); B.s_tile_col.release(); B.s_tile_maxlen.release();
-        B.best_cell.release(); B.chunks.release(); B.nchunks.release();
-        B.queue.release(); B.desc.release();
-        if (B.done) (void)hipEventDestroy(B.done);
+        B.release();
     }
     c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
     c->out_stage.release(); c->stats_stage.release();
@@ -1324,11 +1307,7 @@ void swg_serve_destroy(swg_server* s)
         if (L.lane.stream) (void)hipStreamSynchronize(L.lane.stream);
         for (auto& B : L.lane.bufs) {
             for (auto& g : B.graph) if (g) (void)hipGraphExecDestroy(g);
-            B.task_row.release(); B.task_tile.release(); B.tile_col.release(); B.tile_maxlen.release();
-            B.task_rowlim.release(); B.tile_lim.release();
-            B.s_task_row.release(); B.s_task_tile.release(); B.s_tile_col.release(); B.s_tile_maxlen.release();
-            B.best_cell.release(); B.chunks.release(); B.nchunks.release(); B.queue.release(); B.desc.release();
-            if (B.done) (void)hipEventDestroy(B.done);
+            B.release();
         }
         L.lane.bnd.release();
         L.dout.release(); L.dstats.release();

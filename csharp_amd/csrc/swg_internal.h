@@ -20,7 +20,9 @@ struct BlockDesc {
     int64_t r0, c0, ld;    // output position of pair (a,b): (a-r0)*ld + (b-c0)
     void* out;             // TDistance array (int16 or double)
     int32_t* stats;        // optional swg_pair_stats (4 x int32) at the same position
-    int64_t pad_;
+    int32_t rows;          // score-pass desc: rows of the block
+    int32_t wr, wc;        // score-pass desc: bucket size (first-maximum rows, columns) of the re-run sort
+    int32_t pad_;
 };
 
 // Read-only data of a context (sequence set + scoring).
@@ -45,16 +47,14 @@ struct TaskSet {
     const int32_t* tile_maxlen;   // max (limited) length over the tile's lanes
     const BlockDesc* desc;
     int32_t* queue;               // [0]: next task (persistent schedule)
-    // bounded re-run (stats pass of the two-pass schedule); null = unbounded
-    const int32_t* task_rowlim;   // DP rows 1..rowlim only
-    const int32_t* tile_lim;      // per lane: DP columns 1..lim only
 };
 
 // Score-only pass (int16 x 2 packed): task t aligns s1 = task_row[t] against
 // the 128 sequences of tile task_tile[t] (lane l: tile_col[tile*128 + l] in
 // the low half, tile_col[tile*128 + 64 + l] in the high half) and writes, per
 // pair, the first-maximum cell (bi << 16) | bj into best_cell at the pair's
-// output position, or (La << 16) | Lb when the score may have saturated.
+// position in the block, or kCellTrack | (La << 16) | Lb when the score may
+// have saturated or a sequence is empty.
 struct ScoreTaskSet {
     const int32_t* task_row;
     const int32_t* task_tile;
@@ -65,31 +65,36 @@ struct ScoreTaskSet {
     uint32_t* best_cell;
 };
 
-// Grouping of the score pass's best cells into bounded re-run tasks: chunk c
-// covers partners first_col .. first_col+n-1 of row `row` (contiguous in
-// best_cell at offset cell_off) and emits ceil(n/64) tasks from task_base.
-struct GroupChunk {
-    int32_t row;
-    int32_t first_col;
-    int32_t n;
-    int32_t task_base;
-    int64_t cell_off;
+// best_cell flag: the pair's score may have saturated int16, or a sequence
+// is empty; the re-run then tracks the first maximum over the whole matrix.
+constexpr uint32_t kCellTrack = 0x80000000u;
+
+// Sort of the block's pairs by first-maximum cell for the re-run: buckets of
+// wr x wc cells on a kBucketDim x kBucketDim grid, plus one list (the last
+// bucket) of flagged pairs; each bucket is padded to whole 64-lane tiles.
+constexpr int kBucketDim = 64;
+constexpr int kNumBuckets = kBucketDim * kBucketDim + 1;
+
+struct SortSet {
+    const uint32_t* best_cell;    // [rows * ld] of the score-pass desc
+    const BlockDesc* desc;        // desc[0]: output (mirror), desc[1]: score pass (r0, c0, ld, rows, wr, wc)
+    int32_t* count;               // [kNumBuckets] pairs per bucket (zeroed before the sort)
+    int32_t* cursor;              // [kNumBuckets]
+    int32_t* base;                // [kNumBuckets] first slot of the bucket
+    int32_t* meta;                // [0] tiles of the flagged list (first), [1] all tiles
+    uint32_t* sorted;             // slot -> pair position in best_cell, ~0u = idle lane
 };
 
-struct GroupSet {
-    const GroupChunk* chunks;
-    const int32_t* nchunks;       // device-side count (the launch grid is fixed)
+// Re-run: tile t of `sorted` (64 pairs, one per lane; tiles [meta[0],
+// meta[1]) capture the statistics at the known first-maximum cell, tiles
+// [0, meta[0]) track the first maximum).
+struct RerunSet {
+    const uint32_t* sorted;
     const uint32_t* best_cell;
-    int32_t* task_row;
-    int32_t* task_tile;
-    int32_t* task_rowlim;
-    int32_t* tile_col;
-    int32_t* tile_lim;
-    int32_t* tile_maxlen;
+    const BlockDesc* desc;        // as SortSet
+    const int32_t* meta;
+    int32_t* queue;
 };
-
-constexpr int kGroupChunk = 2048;  // partners sorted together
-constexpr int kGroupSpan = 256;    // partners sharing a row bound
 
 struct Scratch {
     int4* bnd;                    // per-slot row-boundary buffer
@@ -106,7 +111,12 @@ hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 // Score-only int16x2 pass; strip_rows in {16, 32}.
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
                           int nslots, hipStream_t stream);
-hipError_t launch_group(const GroupSet& G, int nchunks, hipStream_t stream);
+// Sort of the score pass's first-maximum cells (count, scan, scatter).
+hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
+// Re-run of the sorted pairs; strip_rows in {16, 32}; track = the flagged list.
+hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, int strip_rows, bool track,
+                        int nslots, hipStream_t stream);
+int rerun_blocks_per_cu(int strip_rows, int ncode);
 // Sixteen-lanes-per-pair kernel: tiles of kLppTile column sequences;
 // strip_rows (rows per lane) in {4, 8}.
 constexpr int kLppTile = 4;

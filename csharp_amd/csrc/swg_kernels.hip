@@ -108,14 +108,8 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
         const int tile = uni(T.task_tile[t]);
         const int maxLb = uni(T.tile_maxlen[tile]);
         const int b = T.tile_col[(int64_t)tile * kWave + lane];
-        const int La0 = uni(S.len[a]);
-        const int Lb0 = b >= 0 ? S.len[b] : 0;
-        // bounded re-run: only the rectangle up to the known first-maximum
-        // cell is recomputed; every cell in it depends only on cells above
-        // and to the left, so its values are exact, and no cell outside the
-        // rectangle precedes the first maximum in row-major order
-        const int La = T.task_rowlim ? min(La0, uni(T.task_rowlim[t])) : La0;
-        const int Lb = T.tile_lim ? min(Lb0, T.tile_lim[(int64_t)tile * kWave + lane]) : Lb0;
+        const int La = uni(S.len[a]);
+        const int Lb = b >= 0 ? S.len[b] : 0;
         const uint8_t* s1 = S.res + S.off[a];
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (b >= 0 ? S.off[b] : S.pad_off));
         const int ncols = (maxLb + 3) & ~3;
@@ -203,7 +197,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
 
         if (b >= 0) {
             int score = best, Tb = bestT;
-            if (La0 == 0 || Lb0 == 0) { score = INT_MIN; Tb = 0; }
+            if (La == 0 || Lb == 0) { score = INT_MIN; Tb = 0; }
             write_distance(S, *T.desc, a, b, score, Tb);
         }
     }
@@ -679,35 +673,12 @@ __global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, S
             }
         }
         if (bA >= 0) {
-            const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
+            const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
             T.best_cell[(int64_t)(a - D->r0) * D->ld + (bA - D->c0)] = c;
         }
         if (bB >= 0) {
-            const uint32_t c = (La == 0 || LbB == 0 || bestB >= kSat16) ? (((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
+            const uint32_t c = (La == 0 || LbB == 0 || bestB >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
             T.best_cell[(int64_t)(a - D->r0) * D->ld + (bB - D->c0)] = c;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Grouping: for one row sequence, sort its partners by first-maximum row and,
-// inside spans of kGroupSpan, by first-maximum column, then emit 64-lane tiles
-// (longest first) with per-lane column limits and a per-task row limit.  The
-// bounded re-run's cost per task is rowlim x max(lim), so grouping pairs with
-// similar best cells keeps it close to the area the pairs actually need.
-__device__ __forceinline__ void bitonic_u64(uint64_t* key, int n, int seg, int tid, int nthr)
-{
-    for (int k = 2; k <= seg; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < n; i += nthr) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (k == seg) || ((i & k) == 0);
-                    const uint64_t x = key[i], y = key[ixj];
-                    if ((x > y) == up) { key[i] = y; key[ixj] = x; }
-                }
-            }
-            __syncthreads();
         }
     }
 }
@@ -719,54 +690,237 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void group_kernel(GroupSet G)
+// ---------------------------------------------------------------------------
+// Re-run of the two-pass schedule, pairs sorted by first-maximum cell.
+//
+// sort_count / sort_scan / sort_scatter put the block's pairs into buckets of
+// similar first-maximum cell (bi, bj), each padded to whole 64-lane tiles,
+// flagged pairs (kCellTrack) first and then the buckets largest first, so the
+// re-run's queue drains the long work first.
+__device__ __forceinline__ int bucket_of(uint32_t cell, int wr, int wc)
 {
-    __shared__ uint64_t key[kGroupChunk];
-    const int nchunks = *G.nchunks;
-    const int tid = threadIdx.x;
-    for (int ci = blockIdx.x; ci < nchunks; ci += (int)gridDim.x) {
-    __syncthreads();
-    const GroupChunk C = G.chunks[ci];
-    int N = kWave;
-    while (N < C.n) N <<= 1;
-    for (int i = tid; i < N; i += 256) {
-        uint64_t k = ~0ull;
-        if (i < C.n) k = ((uint64_t)G.best_cell[C.cell_off + i] << 32) | (uint32_t)i;  // (bi, bj, partner)
-        key[i] = k;
-    }
-    __syncthreads();
-    bitonic_u64(key, N, N, tid, 256);
-    for (int i = tid; i < N; i += 256) {
-        const uint64_t k = key[i];
-        if (k != ~0ull) {
-            const uint64_t cell = k >> 32;
-            key[i] = ((cell & 0xFFFFull) << 48) | ((cell >> 16) << 32) | (k & 0xFFFFFFFFull);  // (bj, bi, partner)
-        }
-    }
-    __syncthreads();
-    bitonic_u64(key, N, N < kGroupSpan ? N : kGroupSpan, tid, 256);
+    if (cell & kCellTrack) return kNumBuckets - 1;
+    const int bi = (int)((cell >> 16) & 0x7FFFu), bj = (int)(cell & 0xFFFFu);
+    const int qi = min(max(bi - 1, 0) / wr, kBucketDim - 1);
+    const int qj = min(max(bj - 1, 0) / wc, kBucketDim - 1);
+    return qi * kBucketDim + qj;
+}
 
-    const int ntiles = (C.n + kWave - 1) / kWave;
-    const int lane = tid & (kWave - 1);
-    for (int k = tid / kWave; k < ntiles; k += 256 / kWave) {
-        const uint64_t kk = key[(ntiles - 1 - k) * kWave + lane];
-        int col = -1, lim = 0, bi = 0;
-        if (kk != ~0ull) {
-            col = C.first_col + (int)(kk & 0xFFFFFFFFull);
-            lim = (int)((kk >> 48) & 0xFFFFull);
-            bi = (int)((kk >> 32) & 0xFFFFull);
-        }
-        const int task = C.task_base + k;
-        G.tile_col[(int64_t)task * kWave + lane] = col;
-        G.tile_lim[(int64_t)task * kWave + lane] = lim;
-        const int mlim = wave_max(lim), mbi = wave_max(bi);
-        if (lane == 0) {
-            G.task_row[task] = C.row;
-            G.task_tile[task] = task;
-            G.task_rowlim[task] = mbi;
-            G.tile_maxlen[task] = mlim;
+// Slot order: flagged list, then bucket kNumBuckets-2 down to 0.
+__device__ __forceinline__ int bucket_rank(int k) { return k == kNumBuckets - 1 ? 0 : kNumBuckets - 1 - k; }
+__device__ __forceinline__ int bucket_at_rank(int q) { return q == 0 ? kNumBuckets - 1 : kNumBuckets - 1 - q; }
+
+// Pairs of the block: row `row` of the score-pass desc holds columns
+// 0 .. ncols(row)-1 (a lower-mirror diagonal block only j < i).
+__device__ __forceinline__ int64_t row_pairs(const BlockDesc& D1, bool lower, int row)
+{
+    if (!lower) return D1.ld;
+    const int64_t n = D1.r0 + row - D1.c0;
+    return n < 0 ? 0 : (n < D1.ld ? n : D1.ld);
+}
+
+__global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
+{
+    const BlockDesc D1 = P.desc[1];
+    const bool lower = P.desc[0].mirror != 0;
+    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
+        const int64_t n = row_pairs(D1, lower, row);
+        const uint32_t* cell = P.best_cell + (int64_t)row * D1.ld;
+        for (int64_t c = threadIdx.x; c < n; c += 256) atomicAdd(&P.count[bucket_of(cell[c], D1.wr, D1.wc)], 1);
+    }
+}
+
+// One wave: exclusive scan of the padded bucket sizes in slot order.
+__global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
+{
+    const int lane = threadIdx.x;
+    constexpr int per = (kNumBuckets + kWave - 1) / kWave;
+    const int q0 = lane * per;
+    int sum = 0;
+    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) sum += (P.count[bucket_at_rank(q)] + kWave - 1) / kWave;
+    // inclusive prefix over lanes
+    int incl = sum;
+#pragma unroll
+    for (int s = 1; s < kWave; s <<= 1) {
+        const int y = __shfl_up(incl, s, kWave);
+        if (lane >= s) incl += y;
+    }
+    int tiles = incl - sum;
+    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) {
+        const int k = bucket_at_rank(q);
+        P.base[k] = tiles * kWave;
+        P.cursor[k] = 0;
+        tiles += (P.count[k] + kWave - 1) / kWave;
+    }
+    if (lane == 0) P.meta[0] = (P.count[kNumBuckets - 1] + kWave - 1) / kWave;
+    if (lane == kWave - 1) P.meta[1] = incl;
+}
+
+__global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
+{
+    const BlockDesc D1 = P.desc[1];
+    const bool lower = P.desc[0].mirror != 0;
+    // idle lanes of each bucket's last tile
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < kNumBuckets; k += (int)gridDim.x * 256) {
+        const int n = P.count[k];
+        const int end = (n + kWave - 1) / kWave * kWave;
+        for (int q = n; q < end; ++q) P.sorted[P.base[k] + q] = ~0u;
+    }
+    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
+        const int64_t n = row_pairs(D1, lower, row);
+        const int64_t first = (int64_t)row * D1.ld;
+        for (int64_t c = threadIdx.x; c < n; c += 256) {
+            const int k = bucket_of(P.best_cell[first + c], D1.wr, D1.wc);
+            const int slot = P.base[k] + atomicAdd(&P.cursor[k], 1);
+            P.sorted[slot] = (uint32_t)(first + c);
         }
     }
+}
+
+// Each lane aligns its own pair: s1 differs per lane, so the substitution
+// scores come from the [c1][c2] profile table staged in LDS.  A lane's rows
+// are bottom-aligned to the wave's last row Rmax = max bi: its first Rmax - bi
+// rows are pad rows, whose cells equal the DP's row 0 (V = 0, T = 0, and
+// G = -gap_open below them; this needs gap_open >= 0 and gap_ext >= 0), so
+// every lane reaches its row bi at wave row Rmax and the capture row is
+// wave-uniform; the capture column is the lane's bj.  Every value in the
+// rectangle [1..bi] x [1..bj] depends only on cells above and to the left, so
+// it equals the full DP's, and the T word at (bi, bj) is that of the
+// reference's traceback from its first maximum (SmithWatermanGotoh.cs:211-229).
+// TRACK (flagged pairs): bi = La, bj = Lb and the first maximum is tracked as
+// in sw_kernel; pad rows hold V = 0, which can only be a first maximum when
+// every cell is 0, and then T = 0 as at (1, 1).
+template <int R, bool TRACK>
+__global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch W)
+{
+    extern __shared__ int2 tab[];  // [ncode][ncode]
+    const int lane = threadIdx.x;
+    const int ncode = S.ncode, pad = ncode - 1;
+    for (int k = lane; k < ncode * ncode; k += kWave) tab[k] = S.prof[k];
+    __syncthreads();
+
+    int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
+    const BlockDesc* D = T.desc;
+    const int64_t r0 = D[1].r0, c0 = D[1].c0;
+    const uint32_t ld1 = (uint32_t)D[1].ld;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int t_lo = TRACK ? 0 : T.meta[0];
+    const int t_hi = TRACK ? T.meta[0] : T.meta[1];
+
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(T.queue, 1);
+        const int t = uni(v) + t_lo;
+        if (t >= t_hi) break;
+
+        const uint32_t p = T.sorted[(int64_t)t * kWave + lane];
+        const bool live = p != ~0u;
+        int a = 0, b = 0, La = 0, Lb = 0, bi = 0, bj = 0;
+        if (live) {
+            a = (int)(r0 + p / ld1);
+            b = (int)(c0 + p % ld1);
+            La = S.len[a];
+            Lb = S.len[b];
+            if (TRACK) {
+                bi = La; bj = Lb;
+            } else {
+                const uint32_t cell = T.best_cell[p];
+                bi = (int)((cell >> 16) & 0x7FFFu);
+                bj = (int)(cell & 0xFFFFu);
+            }
+        }
+        const int Rmax = uni(wave_max(bi)), Cmax = uni(wave_max(bj));
+        const int ro = Rmax - bi;  // pad rows before the lane's row 1
+        const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (live ? S.off[b] : S.pad_off));
+        const int ncols = (Cmax + 3) & ~3;
+
+        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int4(-o, kNeg, 0, 0);
+
+        int best = INT_MIN, bestT = 0, capV = 0, capT = 0;
+        for (int i0 = 0; i0 < Rmax; i0 += R) {
+            int rowb[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int si = i0 + r - ro;
+                rowb[r] = ((si >= 0 && si < La) ? (int)s1[si] : pad) * ncode;
+            }
+            const int rcap = Rmax - 1 - i0;  // wave-uniform; < R only in the last strip
+
+            int tr[R], Tr[R], Hr[R], THr[R];
+            int rb[TRACK ? R : 1], rT[TRACK ? R : 1];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0;
+                if (TRACK) { rb[r] = INT_MIN; rT[r] = 0; }
+            }
+            int dtb = -o, dTb = 0;
+
+            uint32_t nw = s2w[0];
+            int4 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
+            int4 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
+            for (int j = 1; j <= ncols; j += 4) {
+                const uint32_t w = nw;
+                int4 cb[4] = {nb0, nb1, nb2, nb3};
+                nw = s2w[(j + 3) >> 2];
+                const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
+                nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int jj = j + q;
+                    int c2 = (int)((w >> (8 * q)) & 0xFFu);
+                    c2 = jj > Lb ? pad : c2;
+                    int tu = cb[q].x, Gu = cb[q].y, Tu = cb[q].z, TGu = cb[q].w;
+                    int dt = dtb, dT = dTb;
+                    dtb = tu; dTb = Tu;
+                    const bool hit = jj == bj;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int2 pp = tab[rowb[r] + c2];
+                        const int sc = pp.x, inc = pp.y;
+                        const int g1 = Gu - e;
+                        const bool cg = g1 > tu;
+                        const int G = cg ? g1 : tu;
+                        const int TG = (cg ? TGu : Tu) + 0x10000;
+                        const int h1 = Hr[r] - e;
+                        const bool ch = h1 > tr[r];
+                        Hr[r] = ch ? h1 : tr[r];
+                        THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                        const int f = dt + sc;
+                        int V = max(max(f, G), Hr[r]);
+                        V = max(V, 0);
+                        const int Td = dT + inc;
+                        int x = (V == G) ? TG : THr[r];
+                        x = (V == f) ? Td : x;
+                        const int Tn = (V == 0) ? 0 : x;
+                        if (TRACK) {
+                            const bool cb2 = V > rb[r];
+                            rb[r] = cb2 ? V : rb[r];
+                            rT[r] = cb2 ? Tn : rT[r];
+                        } else if (r == rcap) {
+                            capV = hit ? V : capV;
+                            capT = hit ? Tn : capT;
+                        }
+                        const int tn = V - o;
+                        dt = tr[r]; dT = Tr[r];
+                        tr[r] = tn; Tr[r] = Tn;
+                        tu = tn; Gu = G; Tu = Tn; TGu = TG;
+                    }
+                    bnd[(int64_t)jj * kWave + lane] = make_int4(tu, Gu, Tu, TGu);
+                }
+            }
+            if (TRACK) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (i0 + r < Rmax && rb[r] > best) { best = rb[r]; bestT = rT[r]; }
+            }
+        }
+
+        if (live) {
+            int score = TRACK ? best : capV, Tb = TRACK ? bestT : capT;
+            if (La == 0 || Lb == 0) { score = INT_MIN; Tb = 0; }
+            write_distance(S, D[0], a, b, score, Tb);
+        }
     }
 }
 
@@ -873,10 +1027,46 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
     return hipGetLastError();
 }
 
-hipError_t launch_group(const GroupSet& G, int grid, hipStream_t st)
+hipError_t launch_sort(const SortSet& P, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(group_kernel, dim3(grid), dim3(256), 0, st, G);
+    hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(256), 0, st, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kWave), 0, st, P);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(256), 0, st, P);
     return hipGetLastError();
+}
+
+hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, int strip_rows, bool track,
+                        int nslots, hipStream_t st)
+{
+    const size_t lds = (size_t)S.ncode * S.ncode * sizeof(int2);
+    switch (strip_rows) {
+    case 16:
+        if (track) hipLaunchKernelGGL((rerun_kernel<16, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((rerun_kernel<16, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    case 32:
+        if (track) hipLaunchKernelGGL((rerun_kernel<32, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((rerun_kernel<32, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int rerun_blocks_per_cu(int strip_rows, int ncode)
+{
+    const size_t lds = (size_t)ncode * ncode * sizeof(int2);
+    int nb = 0;
+    hipError_t err = hipErrorInvalidValue;
+    switch (strip_rows) {
+    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rerun_kernel<16, false>, kWave, lds); break;
+    case 32: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rerun_kernel<32, false>, kWave, lds); break;
+    default: break;
+    }
+    return err == hipSuccess ? nb : 0;
 }
 
 int score16_blocks_per_cu(int strip_rows)
