@@ -21,8 +21,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-W_OPS_PER_CELL = 24            # integer VALU lane-instructions per cell of the bit-exact recurrence
-PEAK_INT32_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 3.93e13 lane-instructions/s
+PEAK_VALU = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 3.93e13 VALU lane-instructions/s
+# VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
+# unrolled), counted from this build's gfx950 assembly by
+# `python -m csharp_amd.isa_valu_count`; packed int16 instructions count once
+# for the two cells they evaluate.
+VALU_PER_CELL = {"score16": 5.75, "rerun_capture": 22.766, "rerun_track": 24.969, "one_pass": 24.969,
+                 "global": 38.875}
 
 
 def log(msg):
@@ -41,6 +46,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "plain"],
+                    help="force one kernel variant for the timed steps (auto = autotuned)")
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"],
                     help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity check")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
@@ -95,11 +102,13 @@ def main():
     al = ca.Aligner(M, o, e, alignment_type=ca.ALIGN_NUCLEIC, tdistance=ca.TDIST_INT16,
                     schedule=ca.SCHED_PERSISTENT)
     al.set_sequences(seqs)
-    if args.strip:
-        al.set_schedule(ca.SCHED_PERSISTENT, args.strip)
+    SCHED = {"auto": ca.SCHED_PERSISTENT, "i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
+             "lpp16": ca.SCHED_PERSISTENT_LPP16, "plain": ca.SCHED_PLAIN}
+    al.set_schedule(SCHED[args.schedule], args.strip)
 
     VNAME = {ca.SCHED_PLAIN: "plain", ca.SCHED_PERSISTENT_I32: "int32 one-pass",
-             ca.SCHED_PERSISTENT_I16: "int16 score pass + bounded int32 re-run"}
+             ca.SCHED_PERSISTENT_I16: "int16 score pass + sorted int32 re-run",
+             ca.SCHED_PERSISTENT_LPP16: "16 lanes per pair"}
 
     def run_block(k):
         (r0, r1), (c0, c1) = blocks[k % len(blocks)]
@@ -127,6 +136,15 @@ def main():
     last_block = blocks[(args.warmup + args.steps - 1) % len(blocks)]
     result_dev = dev_out[: (last_block[0][1] - last_block[0][0] + 1) * (last_block[1][1] - last_block[1][0] + 1)]
     result = result_dev.cpu().numpy().reshape(last_block[0][1] - last_block[0][0] + 1, -1)
+
+    # per-kernel times of one more block (direct launches, an event after each
+    # kernel) for the roofline of the dominant kernel
+    al.set_kernel_timing(True)
+    run_block(args.warmup + args.steps - 1)
+    ktimes = al.kernel_times()
+    al.set_kernel_timing(False)
+    for name, kms, kc, krun in ktimes:
+        log("kernel %s: %.2f ms, %.3e cells (%.3e evaluated)" % (name, kms, kc, krun))
 
     # PCIe-inclusive rate of the host-memory entry point (one block)
     (r0, r1), (c0, c1) = blocks[0]
@@ -195,13 +213,18 @@ def main():
     al.close()
     serving = None if args.no_serve else serve_bench(args)
 
-    achieved = kernel_gcups * 1e9 * W_OPS_PER_CELL / 1e12
-    peak = PEAK_INT32_OPS / 1e12
-    # HBM bytes per launch of this kernel variant from the committed rocprofv3
+    # roofline of the dominant kernel: its algorithmic cells x VALU
+    # instructions per cell / its event time, against the VALU issue peak
+    dom = max(ktimes, key=lambda k: k[1])
+    dname, dms, dcells, drun = dom
+    w = VALU_PER_CELL.get(dname)
+    achieved = dcells * w / (dms / 1e3) / 1e12 if w else None
+    peak = PEAK_VALU / 1e12
+    # HBM bytes per launch of that kernel from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), if one exists
     traffic = None
     try:
-        key = "%s|c3_block_p%d" % (VNAME[var], args.world)
+        key = "%s|c3_block_p%d" % (dname, args.world)
         traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(key, {}).get("traffic_bytes")
     except (OSError, ValueError):
         traffic = None
@@ -235,13 +258,18 @@ def main():
         "gcups_host_memory_entry": round(host_cells / t_host / 1e9, 2),
         "roofline": {
             "bound": "valu",
-            "achieved": round(achieved, 3),
+            "achieved": round(achieved, 3) if achieved else None,
             "peak": round(peak, 3),
-            "unit": "Tops/s (int32 VALU lane-instructions, W=%d per cell)" % W_OPS_PER_CELL,
-            "frac": round(achieved / peak, 4),
+            "unit": "T VALU lane-instructions/s (%.3f per algorithmic cell)" % w if w else None,
+            "frac": round(achieved / peak, 4) if achieved else None,
             "traffic": traffic,
-            "kernel": "%s (one hipGraph per block; strip %d)" % (VNAME[var], strip),
+            "kernel": dname,
+            "kernel_ms": round(dms, 2),
+            "cells": dcells,
+            "cells_evaluated": drun,
         },
+        "kernels": [{"kernel": k[0], "ms": round(k[1], 2), "cells": k[2], "cells_evaluated": k[3]}
+                    for k in ktimes],
         "cpu_baseline": cpu,
         "parity": parity,
         "serving": serving,

@@ -34,13 +34,16 @@ MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
 TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
+OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING = 1, 2
+KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
+                7: "global"}
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (
     "swg_create", "swg_set_sequences", "swg_compute_block", "swg_compute_block_device",
     "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
     "swg_serve_create", "swg_serve_submit", "swg_serve_wait", "swg_serve_times", "swg_serve_release",
-    "swg_serve_stats", "swg_serve_destroy", "swg_set_option",
+    "swg_serve_stats", "swg_serve_destroy", "swg_set_option", "swg_last_kernel_times",
 )
 HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
@@ -60,6 +63,11 @@ class SwgError(RuntimeError):
 class SmithWatermanGotohException(SwgError):
     """Raised where the reference raises SmithWatermanGotohException
     (Salsa.CoreTPL/Bio/Algorithms/SmithWaterman/SmithWatermanGotohException.cs:30-47)."""
+
+
+class swg_kernel_time(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32), ("ms", ctypes.c_double),
+                ("cells", ctypes.c_double), ("cells_run", ctypes.c_double)]
 
 
 class swg_params(ctypes.Structure):
@@ -96,6 +104,7 @@ def lib():
         L.swg_set_schedule.argtypes = [vp, i32, i32]
         L.swg_last_kernel_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.swg_last_kernel_times.argtypes = [vp, ctypes.POINTER(swg_kernel_time), i32, ctypes.POINTER(i32)]
         L.swg_last_error.restype = ctypes.c_char_p
         L.swg_destroy.argtypes = [vp]
         L.swg_destroy.restype = None
@@ -298,7 +307,20 @@ class Aligner:
         _check(lib().swg_set_schedule(self._h, schedule, strip_rows))
 
     def set_stream_chunk_bytes(self, nbytes):
-        _check(lib().swg_set_option(self._h, 1, int(nbytes)))
+        _check(lib().swg_set_option(self._h, OPT_STREAM_CHUNK_BYTES, int(nbytes)))
+
+    def set_kernel_timing(self, on):
+        """Direct launches with an event after each kernel (see kernel_times)."""
+        _check(lib().swg_set_option(self._h, OPT_KERNEL_TIMING, 1 if on else 0))
+
+    def kernel_times(self):
+        """Per-kernel [(name, ms, cells, cells_run)] of the last block run with
+        kernel timing on."""
+        buf = (swg_kernel_time * 8)()
+        n = ctypes.c_int32()
+        _check(lib().swg_last_kernel_times(self._h, buf, 8, ctypes.byref(n)))
+        return [(KERNEL_NAMES.get(buf[k].kernel, str(buf[k].kernel)), buf[k].ms, buf[k].cells, buf[k].cells_run)
+                for k in range(n.value)]
 
     def compute_block(self, row_start, row_end, col_start, col_end, is_diagonal, stats=False, out=None):
         """ComputeBlock: returns (distances[rows, cols], stats or None, pairs)."""

@@ -91,6 +91,13 @@ struct BlockBufs {
     }
 };
 
+struct Plan {
+    int64_t pairs = 0;
+    double cells = 0.0;      // DP cells of the block's pairs
+    double cells_run = 0.0;  // DP cells the (first) kernel evaluates, pad lanes and columns included
+    int32_t ntasks = 0;      // tasks of the (first) kernel
+};
+
 // Autotuner record of one key (see choose_variant).
 struct TuneEntry {
     int64_t key = 0;
@@ -162,6 +169,15 @@ struct swg_ctx {
     double last_kernel_ms = 0.0, last_cells = 0.0;
     int32_t last_strip = 0;
     int32_t last_variant = 0;
+
+    // kernel-timing mode: direct launches with an event after each kernel
+    bool kernel_timing = false;
+    static constexpr int kMaxKernels = 8;
+    hipEvent_t kt_ev[kMaxKernels] = {};
+    int32_t kt_ids[kMaxKernels] = {};
+    int kt_n = 0;
+    Plan kt_plan;
+    const int32_t* kt_meta = nullptr;  // device meta words of the two-pass sort
 };
 
 namespace {
@@ -186,7 +202,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
     int a = c->nw        ? nw_blocks_per_cu(s1)
-          : v == kVarI16 ? score16_blocks_per_cu(s1)
+          : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
@@ -214,12 +230,6 @@ int ensure_scratch(swg_ctx* c, Lane& ln, int* slots)
 }
 
 // ---- task planning ------------------------------------------------------
-
-struct Plan {
-    int64_t pairs = 0;
-    double cells = 0.0;
-    int32_t ntasks = 0;   // tasks of the (first) kernel
-};
 
 struct BlockGeom {
     int32_t r0, r1, c0, c1;
@@ -267,6 +277,7 @@ void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, st
     };
     plan.pairs = 0;
     plan.cells = 0.0;
+    plan.cells_run = 0.0;
     if (!g.diag_lower) {
         std::vector<int32_t> tiles;
         double sum_cols = 0.0;
@@ -293,6 +304,8 @@ void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, st
         }
     }
     plan.ntasks = (int32_t)trow.size();
+    for (size_t k = 0; k < trow.size(); ++k)
+        plan.cells_run += (double)L[trow[k]] * (double)((tmax[ttile[k]] + 3) & ~3) * width;
 }
 
 template <typename T>
@@ -347,7 +360,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^32 pairs)");
         SWG_HIP(B.best_cell.reserve(std::max<size_t>(cells, 1)));
         SWG_HIP(B.sorted.reserve(cells + (size_t)kWave * kNumBuckets));
-        SWG_HIP(B.sortbuf.reserve(3 * kNumBuckets + 64));
+        SWG_HIP(B.sortbuf.reserve(kMetaWords + 3 * kNumBuckets));
         // bucket size: the block's longest sequences over kBucketDim buckets
         int32_t max_r = 1, max_c = 1;
         for (int32_t i = g.r0; i <= g.r1; ++i) max_r = std::max(max_r, c->h_len[i]);
@@ -387,18 +400,55 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
-        P.best_cell = B.best_cell.p; P.desc = B.desc.p;
-        P.count = B.sortbuf.p; P.cursor = B.sortbuf.p + kNumBuckets; P.base = B.sortbuf.p + 2 * kNumBuckets;
-        P.meta = B.sortbuf.p + 3 * kNumBuckets; P.sorted = B.sorted.p;
+        P.best_cell = B.best_cell.p; P.desc = B.desc.p; P.len = c->len.p;
+        P.meta = B.sortbuf.p; P.count = B.sortbuf.p + kMetaWords;
+        P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
         RT.sorted = B.sorted.p; RT.best_cell = B.best_cell.p; RT.desc = B.desc.p; RT.meta = P.meta;
         RC = RT;
         RT.queue = B.queue.p + 16;
         RC.queue = B.queue.p + 32;
     }
 
+    // the block's kernels in stream order; with `marks`, an event is recorded
+    // after each kernel (kernel-timing mode, direct launches)
+    const bool timing = c->kernel_timing && &ln == &c->main;
+    auto launch_all = [&](bool mark) -> hipError_t {
+        int nk = 0;
+        auto after = [&](int kernel_id) -> hipError_t {
+            if (!mark) return hipSuccess;
+            c->kt_ids[nk] = kernel_id;
+            return hipEventRecord(c->kt_ev[nk++], st);
+        };
+        hipError_t e = launch_fill_i32(B.queue.p, 0, 48, st);
+        if (e != hipSuccess) return e;
+        const bool persist = v != kVarPlain;
+        if (c->nw) {
+            e = launch_nw(S, T, W, s1, persist, n1, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_GLOBAL);
+        } else if (v == kVarI32 || v == kVarPlain) {
+            e = launch_sw(S, T, W, s1, persist, n1, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_ONE_PASS);
+        } else if (v == kVarLpp) {
+            e = launch_lpp16(S, T, W, s1, true, n1, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_LPP16);
+        } else {
+            e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
+            if (e == hipSuccess) e = launch_score16(S, ST, W, s1, n1, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_SCORE16);
+            if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
+            if (e == hipSuccess) e = launch_rerun(S, RT, W, s2, true, n2, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_TRACK);
+            if (e == hipSuccess) e = launch_rerun(S, RC, W, s2, false, n2, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_CAPTURE);
+        }
+        if (mark) c->kt_n = nk;
+        return e;
+    };
+
     // capture (or re-capture) before the timing events so that host-side
     // capture time is not counted as device time
-    if (plan.ntasks > 0 && v != kVarPlain) {
+    if (plan.ntasks > 0 && v != kVarPlain && !timing) {
         // every pointer and scalar baked into the graph is part of its key:
         // re-capture whenever one changed (new sequences, grown buffers)
         std::vector<unsigned char> key;
@@ -409,22 +459,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
             hipGraph_t gr = nullptr;
             SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            hipError_t e = launch_fill_i32(B.queue.p, 0, 48, st);
-            if (e == hipSuccess) {
-                if (c->nw) {
-                    e = launch_nw(S, T, W, s1, true, n1, st);
-                } else if (v == kVarI32) {
-                    e = launch_sw(S, T, W, s1, true, n1, st);
-                } else if (v == kVarLpp) {
-                    e = launch_lpp16(S, T, W, s1, true, n1, st);
-                } else {
-                    e = launch_fill_i32(P.count, 0, kNumBuckets, st);
-                    if (e == hipSuccess) e = launch_score16(S, ST, W, s1, n1, st);
-                    if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
-                    if (e == hipSuccess) e = launch_rerun(S, RT, W, s2, true, n2, st);
-                    if (e == hipSuccess) e = launch_rerun(S, RC, W, s2, false, n2, st);
-                }
-            }
+            const hipError_t e = launch_all(false);
             hipError_t e2 = hipStreamEndCapture(st, &gr);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (gr) (void)hipGraphDestroy(gr);
@@ -437,9 +472,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             B.graph_key[v] = key;
         }
     }
+    if (timing) {
+        c->kt_n = 0;
+        c->kt_plan = plan;
+        c->kt_meta = v == kVarI16 ? P.meta : nullptr;
+    }
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
-        if (v == kVarPlain) SWG_HIP(c->nw ? launch_nw(S, T, W, s1, false, n1, st) : launch_sw(S, T, W, s1, false, n1, st));
+        if (v == kVarPlain || timing) SWG_HIP(launch_all(timing));
         else SWG_HIP(hipGraphLaunch(B.graph[v], st));
     }
     SWG_HIP(hipEventRecord(ln.ev1, st));
@@ -747,6 +787,8 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (e != hipSuccess) return bail(e, "hipStreamCreate");
     if ((e = hipEventCreate(&c->main.ev0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&c->main.ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    for (auto& ev : c->kt_ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "hipEventCreate");
     for (auto& B : c->main.bufs)
         if ((e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     *out = c;
@@ -898,7 +940,50 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         c->stream_chunk_bytes = (size_t)value;
         return SWG_OK;
     }
+    if (option == SWG_OPT_KERNEL_TIMING) {
+        if (value != 0 && value != 1) return fail(SWG_E_ARGUMENT, "kernel timing is 0 or 1");
+        c->kernel_timing = value == 1;
+        return SWG_OK;
+    }
     return fail(SWG_E_ARGUMENT, "unknown option");
+}
+
+int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t* n)
+{
+    if (!c || !n || (max > 0 && !out)) return fail(SWG_E_ARGUMENT, "null argument");
+    *n = 0;
+    SWG_HIP(hipSetDevice(c->device));
+    SWG_HIP(hipStreamSynchronize(c->main.stream));
+    uint64_t meta64[kMetaWords / 2] = {};
+    if (c->kt_meta) SWG_HIP(hipMemcpy(meta64, c->kt_meta, sizeof(meta64), hipMemcpyDeviceToHost));
+    const int cnt = std::min<int>(c->kt_n, max);
+    for (int k = 0; k < cnt; ++k) {
+        float ms = 0.0f;
+        SWG_HIP(hipEventElapsedTime(&ms, k == 0 ? c->main.ev0 : c->kt_ev[k - 1], c->kt_ev[k]));
+        swg_kernel_time& t = out[k];
+        t.kernel = c->kt_ids[k];
+        t.reserved = 0;
+        t.ms = ms;
+        t.cells = 0.0;
+        t.cells_run = 0.0;
+        switch (t.kernel) {
+        case SWG_KERNEL_ONE_PASS: case SWG_KERNEL_GLOBAL: case SWG_KERNEL_LPP16: case SWG_KERNEL_SCORE16:
+            t.cells = c->kt_plan.cells;
+            t.cells_run = c->kt_plan.cells_run;
+            break;
+        case SWG_KERNEL_RERUN_TRACK:
+            t.cells = (double)meta64[kMetaFlaggedCells];
+            t.cells_run = (double)meta64[kMetaFlaggedRun];
+            break;
+        case SWG_KERNEL_RERUN_CAPTURE:
+            t.cells = (double)meta64[kMetaRectCells];
+            t.cells_run = (double)meta64[kMetaCaptureRun];
+            break;
+        default: break;
+        }
+    }
+    *n = cnt;
+    return SWG_OK;
 }
 
 int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t* strip_rows, int32_t* variant)
@@ -937,6 +1022,8 @@ void swg_destroy(swg_ctx* c)
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->main.ev0) (void)hipEventDestroy(c->main.ev0);
     if (c->main.ev1) (void)hipEventDestroy(c->main.ev1);
+    for (auto& ev : c->kt_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (c->main.stream) (void)hipStreamDestroy(c->main.stream);
     delete c;
 }

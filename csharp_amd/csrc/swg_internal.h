@@ -52,7 +52,8 @@ struct TaskSet {
 // Score-only pass (int16 x 2 packed): task t aligns s1 = task_row[t] against
 // the 128 sequences of tile task_tile[t] (lane l: tile_col[tile*128 + l] in
 // the low half, tile_col[tile*128 + 64 + l] in the high half) and writes, per
-// pair, the first-maximum cell (bi << 16) | bj into best_cell at the pair's
+// pair, the first-maximum row bi and the first column jb of the 4-column block
+// holding the first maximum, (bi << 16) | jb, into best_cell at the pair's
 // position in the block, or kCellTrack | (La << 16) | Lb when the score may
 // have saturated or a sequence is empty.
 struct ScoreTaskSet {
@@ -75,13 +76,23 @@ constexpr uint32_t kCellTrack = 0x80000000u;
 constexpr int kBucketDim = 64;
 constexpr int kNumBuckets = kBucketDim * kBucketDim + 1;
 
+// meta words (int32 index; the 64-bit counters at meta64[k] = meta + 2k)
+constexpr int kMetaFlaggedTiles = 0;   // tiles of the flagged list (slots first)
+constexpr int kMetaTiles = 1;          // all tiles
+constexpr int kMetaRectCells = 1;      // meta64: cells of the capture rectangles
+constexpr int kMetaFlaggedCells = 2;   // meta64: cells of the flagged pairs
+constexpr int kMetaCaptureRun = 3;     // meta64: cells evaluated by the capture re-run
+constexpr int kMetaFlaggedRun = 4;     // meta64: cells evaluated by the flagged re-run
+constexpr int kMetaWords = 64;         // zeroed with the bucket counts before every sort
+
 struct SortSet {
     const uint32_t* best_cell;    // [rows * ld] of the score-pass desc
     const BlockDesc* desc;        // desc[0]: output (mirror), desc[1]: score pass (r0, c0, ld, rows, wr, wc)
+    const int32_t* len;           // sequence lengths
+    int32_t* meta;                // kMetaWords words, then the bucket counts
     int32_t* count;               // [kNumBuckets] pairs per bucket (zeroed before the sort)
     int32_t* cursor;              // [kNumBuckets]
     int32_t* base;                // [kNumBuckets] first slot of the bucket
-    int32_t* meta;                // [0] tiles of the flagged list (first), [1] all tiles
     uint32_t* sorted;             // slot -> pair position in best_cell, ~0u = idle lane
 };
 
@@ -92,7 +103,7 @@ struct RerunSet {
     const uint32_t* sorted;
     const uint32_t* best_cell;
     const BlockDesc* desc;        // as SortSet
-    const int32_t* meta;
+    int32_t* meta;
     int32_t* queue;
 };
 
@@ -125,7 +136,7 @@ hipError_t launch_lpp16(const SeqSet& S, const TaskSet& T, const Scratch& W, int
 int lpp16_blocks_per_cu(int strip_rows, int ncode);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
-int score16_blocks_per_cu(int strip_rows);
+int score16_blocks_per_cu(int strip_rows, int ncode);
 int nw_blocks_per_cu(int strip_rows);
 // Fill kernels (graph-capturable replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);

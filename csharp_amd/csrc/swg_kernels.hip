@@ -555,19 +555,51 @@ __device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, co
     }
 }
 
+// Joint profile for small alphabets: entry (ca, cc) of row r holds the
+// packed pair (score of code ca | score of code cc << 16), i.e. both halves'
+// scores of one lane in one word.  Layout [r][pair] with a row stride of 64
+// words: a lane reads two rows with one ds_read2st64_b32, and lanes reading
+// different pairs of one row hit different banks (up to 32 pairs).
+constexpr int kJointMaxCodes = 8;
+constexpr int kJointRow = 64;  // words per row (>= kJointMaxCodes^2)
+
 template <int R>
-__global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
+__device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, const uint8_t* s1, int La, int i0,
+                                              int lane)
+{
+    const int ncode = S.ncode, pad = ncode - 1, npr = ncode * ncode;
+    for (int idx = lane; idx < npr * R; idx += kWave) {
+        const int r = idx / npr, pr = idx - r * npr;
+        const int ca = pr / ncode, cc = pr - ca * ncode;
+        const int row = i0 + r;
+        const int c1 = row < La ? (int)s1[row] : pad;
+        int va = S.prof[c1 * ncode + ca].x, vc = S.prof[c1 * ncode + cc].x;
+        va = va < kPad16 ? kPad16 : va;
+        vc = vc < kPad16 ? kPad16 : vc;
+        jp[r * kJointRow + pr] = ((uint32_t)va & 0xFFFFu) | ((uint32_t)vc << 16);
+    }
+}
+
+// Per row the running maximum and the first 4-column block in which it was
+// reached are tracked (the block maximum is folded in once per 4 columns);
+// the re-run finds the exact column inside that block.  best_cell holds
+// (bi << 16) | jb, jb = the block's first column.
+// Two waves per SIMD: the register budget lets the scheduler keep the
+// look-ahead profile loads in flight (measured faster than three waves).
+template <int R, bool JOINT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 {
     static_assert(R % 8 == 0, "strip height must be a multiple of 8");
     constexpr int PS = R / 8 + 1;  // int4 per residue code (R int16 + 8 int16 of padding)
-    __shared__ int4 lprof[kMaxCodes * PS];
+    constexpr int LDS_INT4 = JOINT ? R * kJointRow / 4 : kMaxCodes * PS;
+    __shared__ int4 lprof[LDS_INT4];
 
     const int lane = threadIdx.x;
     int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
     const BlockDesc* D = T.desc;
     const int ntasks = D->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
-    const int pad = S.ncode - 1;
+    const int ncode = S.ncode, pad = ncode - 1;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e)), Z2 = v2(0u);
     const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
 
@@ -596,10 +628,11 @@ __global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, S
         uint32_t cellA = 0, cellB = 0;
         for (int i0 = 0; i0 < La; i0 += R) {
             __syncthreads();
-            stage_profile16<R>(lprof, S, s1, La, i0, lane);
+            if (JOINT) stage_joint16<R>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
+            else stage_profile16<R>(lprof, S, s1, La, i0, lane);
             __syncthreads();
 
-            uint32_t tr[R], Hr[R], rb[R], rj[R];
+            uint32_t tr[R], Hr[R], rb[R], rj[R], bm[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = NEG2; rb[r] = M1; rj[r] = 0u; }
             uint32_t dtb = T0;
@@ -607,51 +640,78 @@ __global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, S
             uint32_t nwa = wA[0], nwb = wB[0];
             int2 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
             int2 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
+            // a column's profile words (R/4 int4) are loaded one column ahead:
+            // joint entries (ca, cc) of the R rows, or the entries of ca and
+            // cc interleaved
+            auto load_col = [&](int ca, int cc, int4* out) {
+                if (JOINT) {
+                    const uint32_t* pj = reinterpret_cast<const uint32_t*>(lprof) + (ca * ncode + cc);
+#pragma unroll
+                    for (int r4 = 0; r4 < R / 4; ++r4)
+                        out[r4] = make_int4((int)pj[(4 * r4) * kJointRow], (int)pj[(4 * r4 + 1) * kJointRow],
+                                            (int)pj[(4 * r4 + 2) * kJointRow], (int)pj[(4 * r4 + 3) * kJointRow]);
+                } else {
+                    const int4* pa = lprof + ca * PS;
+                    const int4* pc = lprof + cc * PS;
+#pragma unroll
+                    for (int r8 = 0; r8 < R / 8; ++r8) { out[2 * r8] = pa[r8]; out[2 * r8 + 1] = pc[r8]; }
+                }
+            };
+            int4 cur[R / 4];
+            load_col(LbA >= 1 ? (int)(nwa & 0xFFu) : pad, LbB >= 1 ? (int)(nwb & 0xFFu) : pad, cur);
+            // residue words: current block, next block (needed by the
+            // look-ahead load of the last column) and the one after
+            uint32_t n2a = wA[1], n2b = wB[1];
             for (int j = 1; j <= ncols; j += 4) {
                 const uint32_t cwa = nwa, cwb = nwb;
                 int2 cb[4] = {nb0, nb1, nb2, nb3};
-                nwa = wA[(j + 3) >> 2];
-                nwb = wB[(j + 3) >> 2];
+                nwa = n2a;
+                nwb = n2b;
+                n2a = wA[(j + 7) >> 2];
+                n2b = wB[(j + 7) >> 2];
                 const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
                 nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
-                    int ca = (int)((cwa >> (8 * q)) & 0xFFu);
-                    int cc = (int)((cwb >> (8 * q)) & 0xFFu);
-                    ca = jj > LbA ? pad : ca;
-                    cc = jj > LbB ? pad : cc;
+                    const int sh = 8 * ((q + 1) & 3);
+                    int ca = (int)(((q < 3 ? cwa : nwa) >> sh) & 0xFFu);
+                    int cc = (int)(((q < 3 ? cwb : nwb) >> sh) & 0xFFu);
+                    ca = jj + 1 > LbA ? pad : ca;
+                    cc = jj + 1 > LbB ? pad : cc;
+                    int4 nx[R / 4];
+                    load_col(ca, cc, nx);
                     uint32_t tu = (uint32_t)cb[q].x, Gu = (uint32_t)cb[q].y;
                     uint32_t dt = dtb;
                     dtb = tu;
-                    const uint32_t JJ = splat16(jj);
-                    const int4* pa = lprof + ca * PS;
-                    const int4* pc = lprof + cc * PS;
 #pragma unroll
-                    for (int r8 = 0; r8 < R / 8; ++r8) {
-                        const int4 xa = pa[r8], xc = pc[r8];
-                        const uint32_t wa4[4] = {(uint32_t)xa.x, (uint32_t)xa.y, (uint32_t)xa.z, (uint32_t)xa.w};
-                        const uint32_t wc4[4] = {(uint32_t)xc.x, (uint32_t)xc.y, (uint32_t)xc.z, (uint32_t)xc.w};
+                    for (int r4 = 0; r4 < R / 4; ++r4) {
+                        uint32_t sp4[4];
+                        if (JOINT) {
+                            const int4 x = cur[r4];
+                            sp4[0] = (uint32_t)x.x; sp4[1] = (uint32_t)x.y; sp4[2] = (uint32_t)x.z; sp4[3] = (uint32_t)x.w;
+                        } else {
+                            // rows 4*r4 .. 4*r4+3 of pair A (low) and pair B (high)
+                            const int4 xa = cur[2 * (r4 >> 1)], xc = cur[2 * (r4 >> 1) + 1];
+                            const uint32_t wa0 = (r4 & 1) ? (uint32_t)xa.z : (uint32_t)xa.x;
+                            const uint32_t wa1 = (r4 & 1) ? (uint32_t)xa.w : (uint32_t)xa.y;
+                            const uint32_t wc0 = (r4 & 1) ? (uint32_t)xc.z : (uint32_t)xc.x;
+                            const uint32_t wc1 = (r4 & 1) ? (uint32_t)xc.w : (uint32_t)xc.y;
+                            sp4[0] = __builtin_amdgcn_perm(wc0, wa0, 0x05040100u);
+                            sp4[1] = __builtin_amdgcn_perm(wc0, wa0, 0x07060302u);
+                            sp4[2] = __builtin_amdgcn_perm(wc1, wa1, 0x05040100u);
+                            sp4[3] = __builtin_amdgcn_perm(wc1, wa1, 0x07060302u);
+                        }
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const int r = 8 * r8 + k;
-                            // rows r of pair A (low half) and pair B (high half)
-                            const uint32_t sp = __builtin_amdgcn_perm(wc4[k >> 1], wa4[k >> 1],
-                                                                      (k & 1) ? 0x07060302u : 0x05040100u);
+                        for (int k = 0; k < 4; ++k) {
+                            const int r = 4 * r4 + k;
                             const s16x2 G = __builtin_elementwise_max(v2(Gu) - E2, v2(tu));
                             const s16x2 H = __builtin_elementwise_max(v2(Hr[r]) - E2, v2(tr[r]));
-                            const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                            const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp4[k]));
                             const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G),
                                                                       __builtin_elementwise_max(H, Z2));
                             const s16x2 tn = V - O2;
-                            // V > row best (no wrap: row best >= -1, V >= 0): the sign of
-                            // (row best - V) per half, spread to a 16-bit mask, selects
-                            // the column with one bitfield insert
-                            const uint32_t d = u2(v2(rb[r]) - V);
-                            uint32_t mk;
-                            asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
-                            rj[r] = (mk & JJ) | (~mk & rj[r]);
-                            rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), V));
+                            bm[r] = q == 0 ? u2(V) : u2(__builtin_elementwise_max(v2(bm[r]), V));
                             dt = tr[r];
                             tr[r] = u2(tn);
                             Hr[r] = u2(H);
@@ -660,6 +720,21 @@ __global__ __launch_bounds__(64) void score16_kernel(SeqSet S, ScoreTaskSet T, S
                         }
                     }
                     bnd[(int64_t)jj * kWave + lane] = make_int2((int)tu, (int)Gu);
+#pragma unroll
+                    for (int r4 = 0; r4 < R / 4; ++r4) cur[r4] = nx[r4];
+                }
+                // block maximum > row maximum (no wrap: row max >= -1, block
+                // max >= 0): the sign of (row max - block max) per half,
+                // spread to a 16-bit mask, selects the block with one
+                // bitfield insert
+                const uint32_t JB = splat16(j);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t d = u2(v2(rb[r]) - v2(bm[r]));
+                    uint32_t mk;
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
+                    rj[r] = (mk & JB) | (~mk & rj[r]);
+                    rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), v2(bm[r])));
                 }
             }
 #pragma unroll
@@ -719,14 +794,45 @@ __device__ __forceinline__ int64_t row_pairs(const BlockDesc& D1, bool lower, in
     return n < 0 ? 0 : (n < D1.ld ? n : D1.ld);
 }
 
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, s, kWave), hi = __shfl_xor((uint32_t)(v >> 32), s, kWave);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Also accumulates, for the re-run's accounting, the cells of the rectangles
+// the re-run must evaluate: bi x min(jb + 3, Lb) per captured pair and
+// La x Lb per flagged pair.
 __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
 {
     const BlockDesc D1 = P.desc[1];
     const bool lower = P.desc[0].mirror != 0;
+    uint64_t rect = 0, flagged = 0;
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
         const uint32_t* cell = P.best_cell + (int64_t)row * D1.ld;
-        for (int64_t c = threadIdx.x; c < n; c += 256) atomicAdd(&P.count[bucket_of(cell[c], D1.wr, D1.wc)], 1);
+        for (int64_t c = threadIdx.x; c < n; c += 256) {
+            const uint32_t x = cell[c];
+            atomicAdd(&P.count[bucket_of(x, D1.wr, D1.wc)], 1);
+            const int Lb = P.len[D1.c0 + c];
+            if (x & kCellTrack) {
+                flagged += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)Lb;
+            } else {
+                const int jb = (int)(x & 0xFFFFu);
+                rect += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)min(jb + 3, Lb);
+            }
+        }
+    }
+    rect = wave_sum_u64(rect);
+    flagged = wave_sum_u64(flagged);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unsigned long long* m = reinterpret_cast<unsigned long long*>(P.meta);
+        atomicAdd(&m[kMetaRectCells], (unsigned long long)rect);
+        atomicAdd(&m[kMetaFlaggedCells], (unsigned long long)flagged);
     }
 }
 
@@ -752,8 +858,8 @@ __global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
         P.cursor[k] = 0;
         tiles += (P.count[k] + kWave - 1) / kWave;
     }
-    if (lane == 0) P.meta[0] = (P.count[kNumBuckets - 1] + kWave - 1) / kWave;
-    if (lane == kWave - 1) P.meta[1] = incl;
+    if (lane == 0) P.meta[kMetaFlaggedTiles] = (P.count[kNumBuckets - 1] + kWave - 1) / kWave;
+    if (lane == kWave - 1) P.meta[kMetaTiles] = incl;
 }
 
 __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
@@ -783,10 +889,14 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
 // rows are pad rows, whose cells equal the DP's row 0 (V = 0, T = 0, and
 // G = -gap_open below them; this needs gap_open >= 0 and gap_ext >= 0), so
 // every lane reaches its row bi at wave row Rmax and the capture row is
-// wave-uniform; the capture column is the lane's bj.  Every value in the
-// rectangle [1..bi] x [1..bj] depends only on cells above and to the left, so
-// it equals the full DP's, and the T word at (bi, bj) is that of the
-// reference's traceback from its first maximum (SmithWatermanGotoh.cs:211-229).
+// wave-uniform.  The score pass gives the first column jb of the 4-column
+// block holding the first maximum of row bi, which is the maximum of that
+// row: its first strict maximum from column jb on is the first maximum
+// (columns past the sequence end hold pad cells, never a new maximum).  Every
+// value in the rectangle [1..bi] x [1..jb+3] depends only on cells above and
+// to the left, so it equals the full DP's, and the T word captured there is
+// that of the reference's traceback from its first maximum
+// (SmithWatermanGotoh.cs:211-229).
 // TRACK (flagged pairs): bi = La, bj = Lb and the first maximum is tracked as
 // in sw_kernel; pad rows hold V = 0, which can only be a first maximum when
 // every cell is 0, and then T = 0 as at (1, 1).
@@ -804,8 +914,12 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     const int64_t r0 = D[1].r0, c0 = D[1].c0;
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
-    const int t_lo = TRACK ? 0 : T.meta[0];
-    const int t_hi = TRACK ? T.meta[0] : T.meta[1];
+    const int t_lo = TRACK ? 0 : T.meta[kMetaFlaggedTiles];
+    const int t_hi = TRACK ? T.meta[kMetaFlaggedTiles] : T.meta[kMetaTiles];
+    // cells evaluated, added to meta once after the task loop: a second
+    // lane-0 branch inside the loop lets the compiler merge it with the
+    // queue's and leave lanes 1..63 in the loop without lane 0
+    uint64_t run = 0;
 
     for (;;) {
         int v = 0;
@@ -829,7 +943,10 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                 bj = (int)(cell & 0xFFFFu);
             }
         }
-        const int Rmax = uni(wave_max(bi)), Cmax = uni(wave_max(bj));
+        // capture: the rows end at a strip boundary, so row bi is the last
+        // strip's bottom row, which the boundary buffer holds afterwards
+        const int Rmax0 = uni(wave_max(bi)), Cmax = uni(wave_max(bj));
+        const int Rmax = TRACK ? Rmax0 : (Rmax0 + R - 1) / R * R;
         const int ro = Rmax - bi;  // pad rows before the lane's row 1
         const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (live ? S.off[b] : S.pad_off));
@@ -837,7 +954,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 
         for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int4(-o, kNeg, 0, 0);
 
-        int best = INT_MIN, bestT = 0, capV = 0, capT = 0;
+        int best = INT_MIN, bestT = 0;
         for (int i0 = 0; i0 < Rmax; i0 += R) {
             int rowb[R];
 #pragma unroll
@@ -845,8 +962,6 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                 const int si = i0 + r - ro;
                 rowb[r] = ((si >= 0 && si < La) ? (int)s1[si] : pad) * ncode;
             }
-            const int rcap = Rmax - 1 - i0;  // wave-uniform; < R only in the last strip
-
             int tr[R], Tr[R], Hr[R], THr[R];
             int rb[TRACK ? R : 1], rT[TRACK ? R : 1];
 #pragma unroll
@@ -859,25 +974,37 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
             uint32_t nw = s2w[0];
             int4 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
             int4 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
+            // the table entries of a column are loaded one column ahead
+            int2 cur[R];
+            {
+                const int c2 = Lb >= 1 ? (int)(nw & 0xFFu) : pad;
+#pragma unroll
+                for (int r = 0; r < R; ++r) cur[r] = tab[rowb[r] + c2];
+            }
+            uint32_t n2 = s2w[1];  // residue word of the block after next
             for (int j = 1; j <= ncols; j += 4) {
                 const uint32_t w = nw;
                 int4 cb[4] = {nb0, nb1, nb2, nb3};
-                nw = s2w[(j + 3) >> 2];
+                nw = n2;
+                n2 = s2w[(j + 7) >> 2];
                 const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
                 nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
-                    int c2 = (int)((w >> (8 * q)) & 0xFFu);
-                    c2 = jj > Lb ? pad : c2;
+                    int c2n = (int)(((q < 3 ? w : nw) >> (8 * ((q + 1) & 3))) & 0xFFu);
+                    c2n = jj + 1 > Lb ? pad : c2n;
+                    int2 nxtp[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) nxtp[r] = tab[rowb[r] + c2n];
+                    // keep the look-ahead loads ahead of this column's cells
+                    __builtin_amdgcn_sched_barrier(0);
                     int tu = cb[q].x, Gu = cb[q].y, Tu = cb[q].z, TGu = cb[q].w;
                     int dt = dtb, dT = dTb;
                     dtb = tu; dTb = Tu;
-                    const bool hit = jj == bj;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const int2 pp = tab[rowb[r] + c2];
-                        const int sc = pp.x, inc = pp.y;
+                        const int sc = cur[r].x, inc = cur[r].y;
                         const int g1 = Gu - e;
                         const bool cg = g1 > tu;
                         const int G = cg ? g1 : tu;
@@ -897,9 +1024,6 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                             const bool cb2 = V > rb[r];
                             rb[r] = cb2 ? V : rb[r];
                             rT[r] = cb2 ? Tn : rT[r];
-                        } else if (r == rcap) {
-                            capV = hit ? V : capV;
-                            capT = hit ? Tn : capT;
                         }
                         const int tn = V - o;
                         dt = tr[r]; dT = Tr[r];
@@ -907,6 +1031,8 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                         tu = tn; Gu = G; Tu = Tn; TGu = TG;
                     }
                     bnd[(int64_t)jj * kWave + lane] = make_int4(tu, Gu, Tu, TGu);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) cur[r] = nxtp[r];
                 }
             }
             if (TRACK) {
@@ -916,12 +1042,25 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
             }
         }
 
+        if (!TRACK) {
+            // first strict maximum of row bi in columns jb .. jb+3
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int4 x = bnd[(int64_t)(bj + q) * kWave + lane];
+                const int V = x.x + o;
+                if (V > best) { best = V; bestT = x.z; }
+            }
+        }
         if (live) {
-            int score = TRACK ? best : capV, Tb = TRACK ? bestT : capT;
+            int score = best, Tb = bestT;
             if (La == 0 || Lb == 0) { score = INT_MIN; Tb = 0; }
             write_distance(S, D[0], a, b, score, Tb);
         }
+        run += (uint64_t)Rmax * (uint64_t)ncols * kWave;
     }
+    if (lane == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + (TRACK ? kMetaFlaggedRun : kMetaCaptureRun),
+                  (unsigned long long)run);
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n)
@@ -1019,9 +1158,16 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode)
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
                           hipStream_t st)
 {
+    const bool joint = S.ncode <= kJointMaxCodes;
     switch (strip_rows) {
-    case 16: hipLaunchKernelGGL(score16_kernel<16>, dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
-    case 32: hipLaunchKernelGGL(score16_kernel<32>, dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
+    case 16:
+        if (joint) hipLaunchKernelGGL((score16_kernel<16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        else hipLaunchKernelGGL((score16_kernel<16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        break;
+    case 32:
+        if (joint) hipLaunchKernelGGL((score16_kernel<32, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        else hipLaunchKernelGGL((score16_kernel<32, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1042,15 +1188,14 @@ hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, in
                         int nslots, hipStream_t st)
 {
     const size_t lds = (size_t)S.ncode * S.ncode * sizeof(int2);
+    if (track) {
+        // the flagged list (rare) always runs 16-row strips
+        hipLaunchKernelGGL((rerun_kernel<16, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        return hipGetLastError();
+    }
     switch (strip_rows) {
-    case 16:
-        if (track) hipLaunchKernelGGL((rerun_kernel<16, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        else hipLaunchKernelGGL((rerun_kernel<16, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        break;
-    case 32:
-        if (track) hipLaunchKernelGGL((rerun_kernel<32, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        else hipLaunchKernelGGL((rerun_kernel<32, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        break;
+    case 16: hipLaunchKernelGGL((rerun_kernel<16, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W); break;
+    case 32: hipLaunchKernelGGL((rerun_kernel<32, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1069,13 +1214,20 @@ int rerun_blocks_per_cu(int strip_rows, int ncode)
     return err == hipSuccess ? nb : 0;
 }
 
-int score16_blocks_per_cu(int strip_rows)
+int score16_blocks_per_cu(int strip_rows, int ncode)
 {
+    const bool joint = ncode <= kJointMaxCodes;
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
     switch (strip_rows) {
-    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16>, kWave, 0); break;
-    case 32: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32>, kWave, 0); break;
+    case 16:
+        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, true>, kWave, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, false>, kWave, 0);
+        break;
+    case 32:
+        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32, true>, kWave, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32, false>, kWave, 0);
+        break;
     default: break;
     }
     return err == hipSuccess ? nb : 0;

@@ -1,0 +1,57 @@
+"""VALU instructions per DP cell in the inner loop of each kernel.
+
+Compiles csrc/swg_kernels.hip for gfx950 to assembly (this build's own
+code), takes each kernel's largest basic block (the unrolled 4-column x
+R-row loop body) and divides its VALU instruction count by the cells that
+body evaluates.  bench.py's roofline uses these figures as the kernels'
+operations per cell.
+
+usage: python -m csharp_amd.isa_valu_count
+"""
+import os
+import re
+import subprocess
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# kernel symbol fragment -> DP cells evaluated by one pass of the loop body
+KERNELS = {
+    "score16_kernelILi16ELb1E": ("score16", 4 * 16 * 2),       # 4 columns x 16 rows x 2 pairs per lane
+    "score16_kernelILi16ELb0E": ("score16 (large alphabet)", 4 * 16 * 2),
+    "rerun_kernelILi16ELb0E": ("rerun_capture", 4 * 16),
+    "sw_kernelILi16ELb1E": ("one_pass", 4 * 16),
+    "nw_kernelILi16ELb1E": ("global", 16),
+}
+
+
+def count(asm_text):
+    out = {}
+    for frag, (name, cells) in KERNELS.items():
+        m = re.search(r"\n(_Z\S*%s\S*):[^\n]*\n" % re.escape(frag), asm_text)
+        if not m:
+            continue
+        start = m.end()
+        end = asm_text.index(".Lfunc_end", start)
+        body = asm_text[start:end]
+        blocks = re.split(r"\n\.LBB[0-9_]+:", body)
+        best = 0
+        for b in blocks:
+            n = sum(1 for line in b.split("\n") if line.strip().startswith("v_"))
+            best = max(best, n)
+        out[name] = round(best / cells, 3)
+    return out
+
+
+def main():
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
+             "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-S", "-I" + os.path.join(HERE, "csrc")]
+    with tempfile.TemporaryDirectory() as d:
+        s = os.path.join(d, "k.s")
+        subprocess.run(["/opt/rocm/bin/hipcc"] + flags + [os.path.join(HERE, "csrc", "swg_kernels.hip"), "-o", s],
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        print(count(open(s).read()))
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,10 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * by chunk while the previous chunk is copied to pinned host memory;
  * default 64 MiB). */
 #define SWG_OPT_STREAM_CHUNK_BYTES 1
+/* SWG_OPT_KERNEL_TIMING = 1: the following blocks launch their kernels
+ * directly (no captured graph) with an event after each kernel, for
+ * swg_last_kernel_times; 0 (default) restores graph launches. */
+#define SWG_OPT_KERNEL_TIMING 2
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the
@@ -146,6 +150,28 @@ int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
  * SWG_SCHED_* value). */
 int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows,
                            int32_t* variant);
+
+/* Kernels of a block (no counterpart in the reference: per-kernel
+ * accounting for roofline measurements). */
+#define SWG_KERNEL_ONE_PASS      1  /* int32 one-pass alignment (persistent or plain) */
+#define SWG_KERNEL_SCORE16       2  /* int16x2 score pass */
+#define SWG_KERNEL_SORT          3  /* sort of the pairs by first-maximum cell (3 small kernels) */
+#define SWG_KERNEL_RERUN_TRACK   4  /* re-run of flagged pairs (saturation, empty sequence) */
+#define SWG_KERNEL_RERUN_CAPTURE 5  /* re-run up to the known first-maximum cell */
+#define SWG_KERNEL_LPP16         6  /* 16 lanes per pair */
+#define SWG_KERNEL_GLOBAL        7  /* global (NW) alignment */
+typedef struct swg_kernel_time {
+    int32_t kernel;     /* SWG_KERNEL_* */
+    int32_t reserved;
+    double  ms;         /* event time from the previous kernel's end (block start for the first) */
+    double  cells;      /* DP cells the kernel is charged with: the block's cells (one-pass, score
+                           pass); the rectangles up to each pair's first-maximum block (capture
+                           re-run); the flagged pairs' cells (tracking re-run); 0 for the sort */
+    double  cells_run;  /* DP cells the kernel evaluated, idle lanes, pad rows and columns included */
+} swg_kernel_time;
+/* After a block run with SWG_OPT_KERNEL_TIMING (synchronizes the context
+ * stream): up to `max` entries for the last block's kernels; *n = count. */
+int  swg_last_kernel_times(swg_ctx* ctx, swg_kernel_time* out, int32_t max, int32_t* n);
 
 const char* swg_last_error(void);
 void swg_destroy(swg_ctx* ctx);

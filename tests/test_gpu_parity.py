@@ -362,3 +362,32 @@ def test_swms_restart_reproduces_block_files(tmp_path, ednafull):
     want, _ = oracle_block(s, r0, r1, c0, c1, False, ednafull, 16, 4)
     assert np.array_equal(got, want)
     al.close()
+
+
+@pytest.mark.parametrize("schedule,names", [
+    (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "rerun_track", "rerun_capture"]),
+    (ca.SCHED_PERSISTENT_I32, ["one_pass"]), (ca.SCHED_PLAIN, ["one_pass"])])
+def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
+    say("kernel timing schedule=%d" % schedule)
+    al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
+    al.set_schedule(schedule, 0)
+    al.set_sequences(dna_set)
+    n = len(dna_set)
+    ref, rst, _ = al.compute_block(0, 89, 90, n - 1, False, stats=True)
+    al.set_kernel_timing(True)
+    got, gst, _ = al.compute_block(0, 89, 90, n - 1, False, stats=True)
+    kt = al.kernel_times()
+    ms, cells, _, _ = al.last_kernel_stats()
+    al.set_kernel_timing(False)
+    assert np.array_equal(got, ref) and np.array_equal(gst, rst)
+    assert [k[0] for k in kt] == names
+    assert abs(sum(k[1] for k in kt) - ms) <= 0.05 * ms + 0.1
+    L = dna_set.lengths.astype(np.float64)
+    block = float(L[0:90].sum() * L[90:n].sum())
+    for name, _, c, run in kt:
+        if name in ("score16", "one_pass"):
+            assert c == block
+        if name == "rerun_capture":
+            assert 0 < c < block
+        assert run >= c
+    al.close()
