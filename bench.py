@@ -21,11 +21,19 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_VALU = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 3.93e13 VALU lane-instructions/s
+# VALU issue peak: CUs x SIMDs x lanes x clock = 3.93e13 int32 lane-operations/s;
+# packed int16 (v_pk_*_i16) does two 16-bit operations per lane-instruction.
+PEAK_OPS = {"int32": 256 * 4 * 16 * 2.4e9, "int16x2": 2 * 256 * 4 * 16 * 2.4e9}
+# Algorithmic operations per DP cell: 24 for the recurrence with the carried
+# traceback statistics, 9 for its score-only subset with the first-maximum
+# update (the score pass); the global variant carries one more word (lead
+# and trailing gap run), 30.  Mode = the arithmetic the kernel runs in.
+ALG_OPS = {"score16": (9, "int16x2"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
+           "one_pass": (24, "int32"), "lpp16": (24, "int32"), "global": (30, "int32")}
 # VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
 # unrolled), counted from this build's gfx950 assembly by
-# `python -m csharp_amd.isa_valu_count`; packed int16 instructions count once
-# for the two cells they evaluate.
+# `python -m csharp_amd.isa_valu_count`; a packed instruction counts once for
+# the two cells it evaluates.  Reported for information.
 VALU_PER_CELL = {"score16": 5.75, "rerun_capture": 22.766, "rerun_track": 24.969, "one_pass": 24.969,
                  "global": 38.875}
 
@@ -48,7 +56,7 @@ def main():
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
     ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "plain"],
                     help="force one kernel variant for the timed steps (auto = autotuned)")
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2nw", "c3", "c4"],
                     help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity check")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
     ap.add_argument("--serve-only", action="store_true")
@@ -213,13 +221,13 @@ def main():
     al.close()
     serving = None if args.no_serve else serve_bench(args)
 
-    # roofline of the dominant kernel: its algorithmic cells x VALU
-    # instructions per cell / its event time, against the VALU issue peak
+    # roofline of the dominant kernel: its algorithmic cells x algorithmic
+    # operations per cell / its event time, against the VALU peak of its mode
     dom = max(ktimes, key=lambda k: k[1])
     dname, dms, dcells, drun = dom
-    w = VALU_PER_CELL.get(dname)
+    w, mode = ALG_OPS.get(dname, (None, "int32"))
     achieved = dcells * w / (dms / 1e3) / 1e12 if w else None
-    peak = PEAK_VALU / 1e12
+    peak = PEAK_OPS[mode] / 1e12
     # HBM bytes per launch of that kernel from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), if one exists
     traffic = None
@@ -260,15 +268,18 @@ def main():
             "bound": "valu",
             "achieved": round(achieved, 3) if achieved else None,
             "peak": round(peak, 3),
-            "unit": "T VALU lane-instructions/s (%.3f per algorithmic cell)" % w if w else None,
+            "unit": "T %s VALU ops/s (%d per DP cell)" % (mode, w) if w else None,
             "frac": round(achieved / peak, 4) if achieved else None,
             "traffic": traffic,
             "kernel": dname,
             "kernel_ms": round(dms, 2),
             "cells": dcells,
             "cells_evaluated": drun,
+            "valu_instr_per_cell": VALU_PER_CELL.get(dname),
         },
-        "kernels": [{"kernel": k[0], "ms": round(k[1], 2), "cells": k[2], "cells_evaluated": k[3]}
+        "kernels": [{"kernel": k[0], "ms": round(k[1], 2), "cells": k[2], "cells_evaluated": k[3],
+                     "frac": (round(k[2] * ALG_OPS[k[0]][0] / (k[1] / 1e3) / PEAK_OPS[ALG_OPS[k[0]][1]], 4)
+                              if k[0] in ALG_OPS and k[1] > 0 else None)}
                     for k in ktimes],
         "cpu_baseline": cpu,
         "parity": parity,
@@ -277,19 +288,20 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble):
+def oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble, global_=False):
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     t = time.perf_counter()
-    d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, nucleic=nucleic, tdist_double=tdouble,
-                         threads=threads)
+    fn = O.nw_pairs_mt if global_ else O.sw_pairs_mt
+    d, _ = fn(seqs.residues, seqs.offsets, a, b, M, o, e, nucleic=nucleic, tdist_double=tdouble, threads=threads)
     return d, time.perf_counter() - t, threads
 
 
 def workload_run(args):
     """C1: 1,000 DNA all pairs -> full Double matrix (SWMPI layout), every pair
     checked against the oracle.  C2: 10,000 proteins, the full Int16 matrix
-    (PercentSimilarity), sampled pairs checked.  C4: rank 0 of 100,000 DNA at
+    (PercentSimilarity), sampled pairs checked; c2nw: the same with the global
+    (Needleman-Wunsch) variant.  C4: rank 0 of 100,000 DNA at
     P = 4: its 25,000-row stripe, diagonal block (lower-mirror) and the three
     blocks right of it, streamed to host and written as SWMS block files."""
     import hashlib
@@ -298,18 +310,21 @@ def workload_run(args):
     from csharp_amd import workloads as WL
 
     name = args.workload
-    cfg = WL.CONFIGS[name]
-    seqs = WL.sequences(name)
-    M, o, e = WL.scoring(name)
+    glob = name == "c2nw"
+    base = "c2" if glob else name
+    cfg = WL.CONFIGS[base]
+    seqs = WL.sequences(base)
+    M, o, e = WL.scoring(base)
     n = len(seqs)
     L = seqs.lengths.astype(np.float64)
     nucleic = cfg["alignment_type"] == ca.ALIGN_NUCLEIC
     tdouble = cfg["tdistance"] == ca.TDIST_DOUBLE
-    al = ca.Aligner(M, o, e, alignment_type=cfg["alignment_type"], tdistance=cfg["tdistance"])
+    al = ca.Aligner(M, o, e, alignment_type=cfg["alignment_type"], tdistance=cfg["tdistance"],
+                    mode=ca.MODE_NW_GLOBAL if glob else ca.MODE_SW_LOCAL)
     al.set_sequences(seqs)
     rng = np.random.default_rng(7)
     out = {"workload": name, "sequences": n}
-    if name in ("c1", "c2"):
+    if name in ("c1", "c2", "c2nw"):
         t = time.perf_counter()
         full, pairs = al.compute_full_matrix(world=1)
         wall = time.perf_counter() - t
@@ -321,7 +336,7 @@ def workload_run(args):
         else:
             a = rng.integers(1, n, size=20000)
             b = (rng.random(20000) * a).astype(np.int64)
-        d, dt, th = oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble)
+        d, dt, th = oracle_sample(seqs, a, b, M, o, e, nucleic, tdouble, global_=glob)
         got = full[a, b]
         same = (got == d) | (np.isnan(got) & np.isnan(d)) if tdouble else (got == d)
         sym = bool(np.array_equal(full[a, b], full[b, a], equal_nan=True)) if tdouble else bool(
