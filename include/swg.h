@@ -120,9 +120,9 @@ int  swg_compute_block(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                        void* out, int64_t out_ld, swg_pair_stats* stats, int64_t* pairs_done);
 
 /* Same as swg_compute_block, but `dev_out`/`dev_stats` are device pointers
- * and the call only enqueues work on the context's own stream (the first block
- * of a new length class also runs the autotuner's sample synchronously).
- * Used for device-resident pipelines; completes at swg_synchronize(). */
+ * and the call only enqueues work on the context's own stream (except the
+ * autotuner's first blocks of a new key, which run each candidate variant
+ * synchronously on the real work).  Completes at swg_synchronize(). */
 int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                               int32_t col_start, int32_t col_end, int32_t is_diagonal,
                               void* dev_out, int64_t out_ld, swg_pair_stats* dev_stats,
