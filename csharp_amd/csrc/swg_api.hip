@@ -356,12 +356,13 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)))
             return rc;
         const size_t cells = (size_t)(g.r1 - g.r0 + 1) * (size_t)(g.c1 - g.c0 + 1);
-        if (cells + (size_t)kWave * kNumBuckets >= (size_t(1) << 32))
-            return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^32 pairs)");
+        if (cells + 2 * (size_t)kWave >= (size_t(1) << 31))
+            return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^31 pairs)");
         SWG_HIP(B.best_cell.reserve(std::max<size_t>(cells, 1)));
-        SWG_HIP(B.sorted.reserve(cells + (size_t)kWave * kNumBuckets));
+        SWG_HIP(B.sorted.reserve(cells + 2 * (size_t)kWave));
         SWG_HIP(B.sortbuf.reserve(kMetaWords + 3 * kNumBuckets));
         // bucket size: the block's longest sequences over kBucketDim buckets
+        // (slots: every pair, the flagged list's padding and the last tile's)
         int32_t max_r = 1, max_c = 1;
         for (int32_t i = g.r0; i <= g.r1; ++i) max_r = std::max(max_r, c->h_len[i]);
         for (int32_t j = g.c0; j <= g.c1; ++j) max_c = std::max(max_c, c->h_len[j]);

@@ -52,7 +52,7 @@ struct TaskSet {
 // Score-only pass (int16 x 2 packed): task t aligns s1 = task_row[t] against
 // the 128 sequences of tile task_tile[t] (lane l: tile_col[tile*128 + l] in
 // the low half, tile_col[tile*128 + 64 + l] in the high half) and writes, per
-// pair, the first-maximum row bi and the first column jb of the 4-column block
+// pair, the first-maximum row bi and the first column jb of the kFirstMaxBlock-column block
 // holding the first maximum, (bi << 16) | jb, into best_cell at the pair's
 // position in the block, or kCellTrack | (La << 16) | Lb when the score may
 // have saturated or a sequence is empty.
@@ -66,14 +66,18 @@ struct ScoreTaskSet {
     uint32_t* best_cell;
 };
 
+// Columns per first-maximum block of the score pass: best_cell names the
+// block, the re-run finds the exact column inside it.
+constexpr int kFirstMaxBlock = 4;
+
 // best_cell flag: the pair's score may have saturated int16, or a sequence
 // is empty; the re-run then tracks the first maximum over the whole matrix.
 constexpr uint32_t kCellTrack = 0x80000000u;
 
 // Sort of the block's pairs by first-maximum cell for the re-run: buckets of
 // wr x wc cells on a kBucketDim x kBucketDim grid, plus one list (the last
-// bucket) of flagged pairs; each bucket is padded to whole 64-lane tiles.
-constexpr int kBucketDim = 64;
+// bucket) of flagged pairs, padded to whole 64-lane tiles.
+constexpr int kBucketDim = 128;
 constexpr int kNumBuckets = kBucketDim * kBucketDim + 1;
 
 // meta words (int32 index; the 64-bit counters at meta64[k] = meta + 2k)
@@ -83,6 +87,7 @@ constexpr int kMetaRectCells = 1;      // meta64: cells of the capture rectangle
 constexpr int kMetaFlaggedCells = 2;   // meta64: cells of the flagged pairs
 constexpr int kMetaCaptureRun = 3;     // meta64: cells evaluated by the capture re-run
 constexpr int kMetaFlaggedRun = 4;     // meta64: cells evaluated by the flagged re-run
+constexpr int kMetaSlots = 10;         // slots in use (flagged list padded, then the buckets)
 constexpr int kMetaWords = 64;         // zeroed with the bucket counts before every sort
 
 struct SortSet {

@@ -529,7 +529,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
 // (:211-214) are evaluated: per row, the packed row maximum and the column of
 // its first occurrence (v_pk_* ops, one instruction for two cells).  The
 // diagonal add saturates, so a score that would exceed 32767 pins at 32767 and
-// the pair is sent to the exact int32 re-run with unbounded limits.  Every
+// the pair is sent to the exact int32 re-run over its whole matrix.  Every
 // other operation only lowers values, and all values stay >= -(gap_open+16384).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
@@ -580,10 +580,10 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
     }
 }
 
-// Per row the running maximum and the first 4-column block in which it was
-// reached are tracked (the block maximum is folded in once per 4 columns);
-// the re-run finds the exact column inside that block.  best_cell holds
-// (bi << 16) | jb, jb = the block's first column.
+// Per row the running maximum and the first kFirstMaxBlock-column block in
+// which it was reached are tracked (the block maximum is folded in once per
+// block); the re-run finds the exact column inside that block.  best_cell
+// holds (bi << 16) | jb, jb = the block's first column.
 // Two waves per SIMD: the register budget lets the scheduler keep the
 // look-ahead profile loads in flight (measured faster than three waves).
 template <int R, bool JOINT>
@@ -723,10 +723,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
                     for (int r4 = 0; r4 < R / 4; ++r4) cur[r4] = nx[r4];
                 }
-                // block maximum > row maximum (no wrap: row max >= -1, block
-                // max >= 0): the sign of (row max - block max) per half,
-                // spread to a 16-bit mask, selects the block with one
-                // bitfield insert
+                // block maximum bm > row maximum rb (no wrap: rb >= -1,
+                // bm >= 0): the sign of (rb - bm) per half, spread to a 16-bit
+                // mask, selects the block with one bitfield insert
+                static_assert(kFirstMaxBlock == 4, "a block is one 4-column step");
                 const uint32_t JB = splat16(j);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -769,9 +769,12 @@ __device__ __forceinline__ int wave_max(int v)
 // Re-run of the two-pass schedule, pairs sorted by first-maximum cell.
 //
 // sort_count / sort_scan / sort_scatter put the block's pairs into buckets of
-// similar first-maximum cell (bi, bj), each padded to whole 64-lane tiles,
-// flagged pairs (kCellTrack) first and then the buckets largest first, so the
-// re-run's queue drains the long work first.
+// similar first-maximum cell (bi, bj): flagged pairs (kCellTrack) first,
+// padded to whole 64-lane tiles, then the buckets in serpentine order from
+// the largest (rows of buckets from the last, alternately right-to-left and
+// left-to-right), packed without padding: a tile that straddles two buckets
+// holds neighbours in both coordinates, and the re-run's queue drains the
+// long work first.
 __device__ __forceinline__ int bucket_of(uint32_t cell, int wr, int wc)
 {
     if (cell & kCellTrack) return kNumBuckets - 1;
@@ -781,9 +784,15 @@ __device__ __forceinline__ int bucket_of(uint32_t cell, int wr, int wc)
     return qi * kBucketDim + qj;
 }
 
-// Slot order: flagged list, then bucket kNumBuckets-2 down to 0.
-__device__ __forceinline__ int bucket_rank(int k) { return k == kNumBuckets - 1 ? 0 : kNumBuckets - 1 - k; }
-__device__ __forceinline__ int bucket_at_rank(int q) { return q == 0 ? kNumBuckets - 1 : kNumBuckets - 1 - q; }
+// Slot order: rank 0 = the flagged list, then the serpentine.
+__device__ __forceinline__ int bucket_at_rank(int q)
+{
+    if (q == 0) return kNumBuckets - 1;
+    const int rr = q - 1, band = rr / kBucketDim, pos = rr - band * kBucketDim;
+    const int qi = kBucketDim - 1 - band;
+    const int qj = (band & 1) ? pos : kBucketDim - 1 - pos;
+    return qi * kBucketDim + qj;
+}
 
 // Pairs of the block: row `row` of the score-pass desc holds columns
 // 0 .. ncols(row)-1 (a lower-mirror diagonal block only j < i).
@@ -805,7 +814,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 }
 
 // Also accumulates, for the re-run's accounting, the cells of the rectangles
-// the re-run must evaluate: bi x min(jb + 3, Lb) per captured pair and
+// the re-run must evaluate: bi x min(jb + kFirstMaxBlock - 1, Lb) per captured pair and
 // La x Lb per flagged pair.
 __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
 {
@@ -823,7 +832,7 @@ __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
                 flagged += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)Lb;
             } else {
                 const int jb = (int)(x & 0xFFFFu);
-                rect += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)min(jb + 3, Lb);
+                rect += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)min(jb + kFirstMaxBlock - 1, Lb);
             }
         }
     }
@@ -836,14 +845,21 @@ __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
     }
 }
 
-// One wave: exclusive scan of the padded bucket sizes in slot order.
+// Slots of rank q: the flagged list is padded to whole tiles, the buckets are not.
+__device__ __forceinline__ int rank_slots(const SortSet& P, int q)
+{
+    const int n = P.count[bucket_at_rank(q)];
+    return q == 0 ? (n + kWave - 1) / kWave * kWave : n;
+}
+
+// One wave: exclusive scan of the slot counts in rank order.
 __global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
 {
     const int lane = threadIdx.x;
     constexpr int per = (kNumBuckets + kWave - 1) / kWave;
     const int q0 = lane * per;
     int sum = 0;
-    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) sum += (P.count[bucket_at_rank(q)] + kWave - 1) / kWave;
+    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) sum += rank_slots(P, q);
     // inclusive prefix over lanes
     int incl = sum;
 #pragma unroll
@@ -851,26 +867,32 @@ __global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
         const int y = __shfl_up(incl, s, kWave);
         if (lane >= s) incl += y;
     }
-    int tiles = incl - sum;
+    int slot = incl - sum;
     for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) {
         const int k = bucket_at_rank(q);
-        P.base[k] = tiles * kWave;
+        P.base[k] = slot;
         P.cursor[k] = 0;
-        tiles += (P.count[k] + kWave - 1) / kWave;
+        slot += rank_slots(P, q);
     }
     if (lane == 0) P.meta[kMetaFlaggedTiles] = (P.count[kNumBuckets - 1] + kWave - 1) / kWave;
-    if (lane == kWave - 1) P.meta[kMetaTiles] = incl;
+    if (lane == kWave - 1) {
+        P.meta[kMetaTiles] = (incl + kWave - 1) / kWave;
+        P.meta[kMetaSlots] = incl;
+    }
 }
 
 __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
 {
     const BlockDesc D1 = P.desc[1];
     const bool lower = P.desc[0].mirror != 0;
-    // idle lanes of each bucket's last tile
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < kNumBuckets; k += (int)gridDim.x * 256) {
-        const int n = P.count[k];
+    // idle lanes: the flagged list's last tile and the last tile
+    if (blockIdx.x == 0 && threadIdx.x < kWave) {
+        const int nf = P.count[kNumBuckets - 1];
+        const int endf = (nf + kWave - 1) / kWave * kWave;
+        if (nf + (int)threadIdx.x < endf) P.sorted[nf + threadIdx.x] = ~0u;
+        const int n = P.meta[kMetaSlots];
         const int end = (n + kWave - 1) / kWave * kWave;
-        for (int q = n; q < end; ++q) P.sorted[P.base[k] + q] = ~0u;
+        if (n + (int)threadIdx.x < end) P.sorted[n + threadIdx.x] = ~0u;
     }
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
@@ -885,15 +907,16 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
 
 // Each lane aligns its own pair: s1 differs per lane, so the substitution
 // scores come from the [c1][c2] profile table staged in LDS.  A lane's rows
-// are bottom-aligned to the wave's last row Rmax = max bi: its first Rmax - bi
-// rows are pad rows, whose cells equal the DP's row 0 (V = 0, T = 0, and
-// G = -gap_open below them; this needs gap_open >= 0 and gap_ext >= 0), so
-// every lane reaches its row bi at wave row Rmax and the capture row is
-// wave-uniform.  The score pass gives the first column jb of the 4-column
-// block holding the first maximum of row bi, which is the maximum of that
-// row: its first strict maximum from column jb on is the first maximum
-// (columns past the sequence end hold pad cells, never a new maximum).  Every
-// value in the rectangle [1..bi] x [1..jb+3] depends only on cells above and
+// are bottom-aligned to the wave's last row Rmax (max bi rounded up to whole
+// strips): its first Rmax - bi rows are pad rows, whose cells equal the DP's
+// row 0 (V = 0, T = 0, and G = -gap_open below them; this needs gap_open >= 0
+// and gap_ext >= 0), so every lane's row bi is the last strip's bottom row,
+// which the boundary buffer holds afterwards.  The score pass gives the first
+// column jb of the kFirstMaxBlock-column block holding the first maximum of
+// row bi, which is the maximum of that row: its first strict maximum in that
+// block is the first maximum (columns past the sequence end hold pad cells,
+// never a new maximum).  Every value in the rectangle
+// [1..bi] x [1..jb+kFirstMaxBlock-1] depends only on cells above and
 // to the left, so it equals the full DP's, and the T word captured there is
 // that of the reference's traceback from its first maximum
 // (SmithWatermanGotoh.cs:211-229).
@@ -950,7 +973,8 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
         const int ro = Rmax - bi;  // pad rows before the lane's row 1
         const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (live ? S.off[b] : S.pad_off));
-        const int ncols = (Cmax + 3) & ~3;
+        // capture: the first maximum lies in columns jb .. jb + kFirstMaxBlock - 1
+        const int ncols = ((TRACK ? Cmax : Cmax + kFirstMaxBlock - 1) + 3) & ~3;
 
         for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int4(-o, kNeg, 0, 0);
 
@@ -1043,9 +1067,9 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
         }
 
         if (!TRACK) {
-            // first strict maximum of row bi in columns jb .. jb+3
+            // first strict maximum of row bi in columns jb .. jb + kFirstMaxBlock - 1
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < kFirstMaxBlock; ++q) {
                 const int4 x = bnd[(int64_t)(bj + q) * kWave + lane];
                 const int V = x.x + o;
                 if (V > best) { best = V; bestT = x.z; }
