@@ -69,8 +69,10 @@ struct BlockBufs {
     // score pass
     DevBuf<int32_t> s_task_row, s_task_tile, s_tile_col, s_tile_maxlen;
     DevBuf<uint32_t> best_cell;
-    DevBuf<uint32_t> sorted;      // re-run order (slot -> pair position)
-    DevBuf<int32_t> sortbuf;      // bucket count, cursor, base, then meta
+    DevBuf<int16_t> best_score;
+    DevBuf<uint32_t> start_cell;
+    DevBuf<uint32_t> sorted, sorted2;    // reverse-pass / re-run order (slot -> pair position)
+    DevBuf<int32_t> sortbuf, sortbuf2;   // meta words, then bucket count, cursor, base
     DevBuf<int32_t> queue;        // [0]: first pass, [16]: flagged re-run, [32]: capture re-run
     DevBuf<BlockDesc> desc;       // [0]: output desc, [1]: score-pass desc
     std::vector<int32_t> h_task_row, h_task_tile, h_tile_col, h_tile_maxlen;
@@ -85,7 +87,8 @@ struct BlockBufs {
     {
         task_row.release(); task_tile.release(); tile_col.release(); tile_maxlen.release();
         s_task_row.release(); s_task_tile.release(); s_tile_col.release(); s_tile_maxlen.release();
-        best_cell.release(); sorted.release(); sortbuf.release(); queue.release(); desc.release();
+        best_cell.release(); best_score.release(); start_cell.release(); sorted.release(); sorted2.release();
+        sortbuf.release(); sortbuf2.release(); queue.release(); desc.release();
         if (done) (void)hipEventDestroy(done);
         done = nullptr;
     }
@@ -177,7 +180,8 @@ struct swg_ctx {
     int32_t kt_ids[kMaxKernels] = {};
     int kt_n = 0;
     Plan kt_plan;
-    const int32_t* kt_meta = nullptr;  // device meta words of the two-pass sort
+    const int32_t* kt_meta = nullptr;   // device meta words of the two-pass sorts: by first-maximum cell
+    const int32_t* kt_meta2 = nullptr;  // and by re-run rectangle
 };
 
 namespace {
@@ -197,7 +201,9 @@ void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
     *s2 = c->strip_rows == 32 ? 32 : 16;  // re-run
 }
 
-int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
+// Resident workgroups of the variant's kernels on the lane's share of the
+// CUs: n1 first kernel, n2 re-run, n3 reverse pass (two-pass variant).
+int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2, int* n3 = nullptr)
 {
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
@@ -206,9 +212,11 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
-    if (a <= 0 || (v == kVarI16 && b <= 0)) return fail(SWG_E_DEVICE, "occupancy query failed");
+    int r = v == kVarI16 ? reverse_blocks_per_cu(c->ncode) : 0;
+    if (a <= 0 || (v == kVarI16 && (b <= 0 || r <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
     *n2 = std::max(1, b * c->num_cu / ln.share);
+    if (n3) *n3 = std::max(1, r * c->num_cu / ln.share);
     return SWG_OK;
 }
 
@@ -338,15 +346,16 @@ void append_key(std::vector<unsigned char>& k, const T& v)
 int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld,
                 int32_t* dev_stats, Plan& plan)
 {
-    int s1, s2, n1, n2;
+    int s1, s2, n1, n2, n3;
     variant_strips(c, v, &s1, &s2);
-    int rc = slots_for(c, ln, v, &n1, &n2);
+    int rc = slots_for(c, ln, v, &n1, &n2, &n3);
     if (rc) return rc;
-    int want = std::max(n1, n2);
+    int want = std::max(std::max(n1, n2), v == kVarI16 ? n3 : 0);
     rc = ensure_scratch(c, ln, &want);
     if (rc) return rc;
     n1 = std::min(n1, want);
     n2 = std::min(n2, want);
+    n3 = std::min(n3, want);
 
     const hipStream_t st = ln.stream;
     BlockDesc desc[2] = {};
@@ -359,8 +368,12 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         if (cells + 2 * (size_t)kWave >= (size_t(1) << 31))
             return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^31 pairs)");
         SWG_HIP(B.best_cell.reserve(std::max<size_t>(cells, 1)));
+        SWG_HIP(B.best_score.reserve(std::max<size_t>(cells, 1)));
+        SWG_HIP(B.start_cell.reserve(std::max<size_t>(cells, 1)));
         SWG_HIP(B.sorted.reserve(cells + 2 * (size_t)kWave));
+        SWG_HIP(B.sorted2.reserve(cells + 2 * (size_t)kWave));
         SWG_HIP(B.sortbuf.reserve(kMetaWords + 3 * kNumBuckets));
+        SWG_HIP(B.sortbuf2.reserve(kMetaWords + 3 * kNumBuckets));
         // bucket size: the block's longest sequences over kBucketDim buckets
         // (slots: every pair, the flagged list's padding and the last tile's)
         int32_t max_r = 1, max_c = 1;
@@ -395,17 +408,29 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     T.task_row = B.task_row.p; T.task_tile = B.task_tile.p; T.tile_col = B.tile_col.p;
     T.tile_maxlen = B.tile_maxlen.p; T.desc = B.desc.p; T.queue = B.queue.p;
     ScoreTaskSet ST{};
-    SortSet P{};
+    SortSet P{}, P2{};
+    ReverseSet RV{};
     RerunSet RT{}, RC{};
     const int sort_grid = std::max(1, c->num_cu * 4 / ln.share);
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
-        P.best_cell = B.best_cell.p; P.desc = B.desc.p; P.len = c->len.p;
+        ST.best_score = B.best_score.p;
+        // sort 1: by first-maximum cell, for the reverse pass
+        P.best_cell = B.best_cell.p; P.start_cell = nullptr; P.desc = B.desc.p; P.len = c->len.p;
         P.meta = B.sortbuf.p; P.count = B.sortbuf.p + kMetaWords;
         P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
-        RT.sorted = B.sorted.p; RT.best_cell = B.best_cell.p; RT.desc = B.desc.p; RT.meta = P.meta;
+        RV.sorted = B.sorted.p; RV.best_cell = B.best_cell.p; RV.best_score = B.best_score.p;
+        RV.start_cell = B.start_cell.p; RV.desc = B.desc.p; RV.meta = P.meta; RV.queue = B.queue.p + 48;
+        // sort 2: by re-run rectangle
+        P2 = P;
+        P2.start_cell = B.start_cell.p;
+        P2.meta = B.sortbuf2.p; P2.count = B.sortbuf2.p + kMetaWords;
+        P2.cursor = P2.count + kNumBuckets; P2.base = P2.count + 2 * kNumBuckets; P2.sorted = B.sorted2.p;
+        RT.sorted = B.sorted2.p; RT.best_cell = B.best_cell.p; RT.start_cell = nullptr; RT.desc = B.desc.p;
+        RT.meta = P2.meta;
         RC = RT;
+        RC.start_cell = B.start_cell.p;
         RT.queue = B.queue.p + 16;
         RC.queue = B.queue.p + 32;
     }
@@ -420,7 +445,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             c->kt_ids[nk] = kernel_id;
             return hipEventRecord(c->kt_ev[nk++], st);
         };
-        hipError_t e = launch_fill_i32(B.queue.p, 0, 48, st);
+        hipError_t e = launch_fill_i32(B.queue.p, 0, 64, st);
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
@@ -434,9 +459,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (e == hipSuccess) e = after(SWG_KERNEL_LPP16);
         } else {
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
+            if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_score16(S, ST, W, s1, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
+            if (e == hipSuccess) e = launch_reverse(S, RV, W, n3, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_REVERSE);
+            if (e == hipSuccess) e = launch_sort(P2, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
             if (e == hipSuccess) e = launch_rerun(S, RT, W, s2, true, n2, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_TRACK);
@@ -454,7 +484,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         // re-capture whenever one changed (new sequences, grown buffers)
         std::vector<unsigned char> key;
         append_key(key, S); append_key(key, T); append_key(key, W); append_key(key, ST); append_key(key, P);
-        append_key(key, RT); append_key(key, RC); append_key(key, sort_grid);
+        append_key(key, RT); append_key(key, RC); append_key(key, sort_grid); append_key(key, P2);
+        append_key(key, RV); append_key(key, n3);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         if (!B.graph[v] || key != B.graph_key[v]) {
             if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
@@ -477,6 +508,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         c->kt_n = 0;
         c->kt_plan = plan;
         c->kt_meta = v == kVarI16 ? P.meta : nullptr;
+        c->kt_meta2 = v == kVarI16 ? P2.meta : nullptr;
     }
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
@@ -955,8 +987,9 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
     *n = 0;
     SWG_HIP(hipSetDevice(c->device));
     SWG_HIP(hipStreamSynchronize(c->main.stream));
-    uint64_t meta64[kMetaWords / 2] = {};
+    uint64_t meta64[kMetaWords / 2] = {}, meta64b[kMetaWords / 2] = {};
     if (c->kt_meta) SWG_HIP(hipMemcpy(meta64, c->kt_meta, sizeof(meta64), hipMemcpyDeviceToHost));
+    if (c->kt_meta2) SWG_HIP(hipMemcpy(meta64b, c->kt_meta2, sizeof(meta64b), hipMemcpyDeviceToHost));
     const int cnt = std::min<int>(c->kt_n, max);
     for (int k = 0; k < cnt; ++k) {
         float ms = 0.0f;
@@ -972,13 +1005,17 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
             t.cells = c->kt_plan.cells;
             t.cells_run = c->kt_plan.cells_run;
             break;
-        case SWG_KERNEL_RERUN_TRACK:
-            t.cells = (double)meta64[kMetaFlaggedCells];
-            t.cells_run = (double)meta64[kMetaFlaggedRun];
-            break;
-        case SWG_KERNEL_RERUN_CAPTURE:
+        case SWG_KERNEL_REVERSE:
             t.cells = (double)meta64[kMetaRectCells];
             t.cells_run = (double)meta64[kMetaCaptureRun];
+            break;
+        case SWG_KERNEL_RERUN_TRACK:
+            t.cells = (double)meta64b[kMetaFlaggedCells];
+            t.cells_run = (double)meta64b[kMetaFlaggedRun];
+            break;
+        case SWG_KERNEL_RERUN_CAPTURE:
+            t.cells = (double)meta64b[kMetaRectCells];
+            t.cells_run = (double)meta64b[kMetaCaptureRun];
             break;
         default: break;
         }

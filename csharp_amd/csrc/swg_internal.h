@@ -64,6 +64,7 @@ struct ScoreTaskSet {
     const BlockDesc* desc;        // ntasks, r0, c0, ld (of best_cell)
     int32_t* queue;
     uint32_t* best_cell;
+    int16_t* best_score;          // the first-maximum score (valid where best_cell is not flagged)
 };
 
 // Columns per first-maximum block of the score pass: best_cell names the
@@ -92,6 +93,8 @@ constexpr int kMetaWords = 64;         // zeroed with the bucket counts before e
 
 struct SortSet {
     const uint32_t* best_cell;    // [rows * ld] of the score-pass desc
+    const uint32_t* start_cell;   // null: key = first-maximum cell (bi, jb); else key = the re-run
+                                  // rectangle [si..bi] x [sj..min(jb+kFirstMaxBlock-1, Lb)]
     const BlockDesc* desc;        // desc[0]: output (mirror), desc[1]: score pass (r0, c0, ld, rows, wr, wc)
     const int32_t* len;           // sequence lengths
     int32_t* meta;                // kMetaWords words, then the bucket counts
@@ -101,12 +104,28 @@ struct SortSet {
     uint32_t* sorted;             // slot -> pair position in best_cell, ~0u = idle lane
 };
 
+// Reverse pass: for the pairs of tiles [meta[0], meta[1]) of `sorted`, the
+// topmost row and leftmost column at which an alignment ending in the
+// first-maximum block with the first-maximum score can start:
+// start_cell = (si << 16) | sj.
+struct ReverseSet {
+    const uint32_t* sorted;
+    const uint32_t* best_cell;
+    const int16_t* best_score;
+    uint32_t* start_cell;
+    const BlockDesc* desc;        // as SortSet
+    int32_t* meta;
+    int32_t* queue;
+};
+
 // Re-run: tile t of `sorted` (64 pairs, one per lane; tiles [meta[0],
 // meta[1]) capture the statistics at the known first-maximum cell, tiles
-// [0, meta[0]) track the first maximum).
+// [0, meta[0]) track the first maximum).  start_cell (optional): the capture
+// re-run covers rows si..bi and columns sj.. only.
 struct RerunSet {
     const uint32_t* sorted;
     const uint32_t* best_cell;
+    const uint32_t* start_cell;
     const BlockDesc* desc;        // as SortSet
     int32_t* meta;
     int32_t* queue;
@@ -133,6 +152,9 @@ hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
 hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, int strip_rows, bool track,
                         int nslots, hipStream_t stream);
 int rerun_blocks_per_cu(int strip_rows, int ncode);
+// Reverse pass (strip_rows 16).
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t stream);
+int reverse_blocks_per_cu(int ncode);
 // Sixteen-lanes-per-pair kernel: tiles of kLppTile column sequences;
 // strip_rows (rows per lane) in {4, 8}.
 constexpr int kLppTile = 4;

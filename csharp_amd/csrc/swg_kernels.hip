@@ -749,11 +749,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         }
         if (bA >= 0) {
             const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
-            T.best_cell[(int64_t)(a - D->r0) * D->ld + (bA - D->c0)] = c;
+            const int64_t pos = (int64_t)(a - D->r0) * D->ld + (bA - D->c0);
+            T.best_cell[pos] = c;
+            T.best_score[pos] = (int16_t)bestA;
         }
         if (bB >= 0) {
             const uint32_t c = (La == 0 || LbB == 0 || bestB >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
-            T.best_cell[(int64_t)(a - D->r0) * D->ld + (bB - D->c0)] = c;
+            const int64_t pos = (int64_t)(a - D->r0) * D->ld + (bB - D->c0);
+            T.best_cell[pos] = c;
+            T.best_score[pos] = (int16_t)bestB;
         }
     }
 }
@@ -775,13 +779,42 @@ __device__ __forceinline__ int wave_max(int v)
 // left-to-right), packed without padding: a tile that straddles two buckets
 // holds neighbours in both coordinates, and the re-run's queue drains the
 // long work first.
-__device__ __forceinline__ int bucket_of(uint32_t cell, int wr, int wc)
+// Bucket coordinate of a rectangle side x >= 1: linear in steps of w, or
+// (re-run rectangles, mostly small) 2-cell steps up to 64 and steps of
+// about (w * kBucketDim) / 96 beyond.
+__device__ __forceinline__ int side_bucket(int x, int w, bool fine)
 {
-    if (cell & kCellTrack) return kNumBuckets - 1;
-    const int bi = (int)((cell >> 16) & 0x7FFFu), bj = (int)(cell & 0xFFFFu);
-    const int qi = min(max(bi - 1, 0) / wr, kBucketDim - 1);
-    const int qj = min(max(bj - 1, 0) / wc, kBucketDim - 1);
-    return qi * kBucketDim + qj;
+    x = max(x - 1, 0);
+    if (!fine) return min(x / w, kBucketDim - 1);
+    if (x < 64) return x >> 1;
+    const int wq = max(1, (w * kBucketDim + 95) / 96);
+    return min(32 + (x - 64) / wq, kBucketDim - 1);
+}
+
+// Bucket of the pair at `pos` and the DP cells of the rectangle its next pass
+// evaluates (the reverse pass: bi x wend; the re-run: (bi-si+1) x (wend-sj+1);
+// a flagged pair: La x Lb).
+__device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1, int64_t pos, int Lb,
+                                           uint64_t* cells, bool* flagged)
+{
+    const uint32_t x = P.best_cell[pos];
+    if (x & kCellTrack) {
+        *flagged = true;
+        *cells = (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)Lb;
+        return kNumBuckets - 1;
+    }
+    *flagged = false;
+    const int bi = (int)((x >> 16) & 0x7FFFu), jb = (int)(x & 0xFFFFu);
+    const int wend = min(jb + kFirstMaxBlock - 1, Lb);
+    int h = bi, w = wend;
+    if (P.start_cell) {
+        const uint32_t st = P.start_cell[pos];
+        h = bi - (int)(st >> 16) + 1;
+        w = wend - (int)(st & 0xFFFFu) + 1;
+    }
+    *cells = (uint64_t)h * (uint64_t)w;
+    const bool fine = P.start_cell != nullptr;
+    return side_bucket(h, D1.wr, fine) * kBucketDim + side_bucket(w, D1.wc, fine);
 }
 
 // Slot order: rank 0 = the flagged list, then the serpentine.
@@ -813,9 +846,8 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
     return v;
 }
 
-// Also accumulates, for the re-run's accounting, the cells of the rectangles
-// the re-run must evaluate: bi x min(jb + kFirstMaxBlock - 1, Lb) per captured pair and
-// La x Lb per flagged pair.
+// Also accumulates, for the accounting of the next pass, the cells of the
+// pairs' rectangles (pair_bucket) and of the flagged pairs.
 __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
 {
     const BlockDesc D1 = P.desc[1];
@@ -823,17 +855,12 @@ __global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
     uint64_t rect = 0, flagged = 0;
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
-        const uint32_t* cell = P.best_cell + (int64_t)row * D1.ld;
         for (int64_t c = threadIdx.x; c < n; c += 256) {
-            const uint32_t x = cell[c];
-            atomicAdd(&P.count[bucket_of(x, D1.wr, D1.wc)], 1);
-            const int Lb = P.len[D1.c0 + c];
-            if (x & kCellTrack) {
-                flagged += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)Lb;
-            } else {
-                const int jb = (int)(x & 0xFFFFu);
-                rect += (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)min(jb + kFirstMaxBlock - 1, Lb);
-            }
+            uint64_t cells;
+            bool fl;
+            const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
+            atomicAdd(&P.count[k], 1);
+            if (fl) flagged += cells; else rect += cells;
         }
     }
     rect = wave_sum_u64(rect);
@@ -898,7 +925,9 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
         const int64_t n = row_pairs(D1, lower, row);
         const int64_t first = (int64_t)row * D1.ld;
         for (int64_t c = threadIdx.x; c < n; c += 256) {
-            const int k = bucket_of(P.best_cell[first + c], D1.wr, D1.wc);
+            uint64_t cells;
+            bool fl;
+            const int k = pair_bucket(P, D1, first + c, P.len[D1.c0 + c], &cells, &fl);
             const int slot = P.base[k] + atomicAdd(&P.cursor[k], 1);
             P.sorted[slot] = (uint32_t)(first + c);
         }
@@ -952,29 +981,40 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 
         const uint32_t p = T.sorted[(int64_t)t * kWave + lane];
         const bool live = p != ~0u;
-        int a = 0, b = 0, La = 0, Lb = 0, bi = 0, bj = 0;
+        // the lane's rectangle: rows si..bi of s1, columns sj..sj+w-1 of s2
+        int a = 0, b = 0, La = 0, Lb = 0, bi = 0, bj = 0, si = 1, sj = 1, w = 0, h = 0;
         if (live) {
             a = (int)(r0 + p / ld1);
             b = (int)(c0 + p % ld1);
             La = S.len[a];
             Lb = S.len[b];
             if (TRACK) {
-                bi = La; bj = Lb;
+                bi = La; bj = Lb; w = Lb;
             } else {
                 const uint32_t cell = T.best_cell[p];
                 bi = (int)((cell >> 16) & 0x7FFFu);
                 bj = (int)(cell & 0xFFFFu);
+                if (T.start_cell) {
+                    const uint32_t st = T.start_cell[p];
+                    si = (int)(st >> 16);
+                    sj = (int)(st & 0xFFFFu);
+                }
+                w = min(bj + kFirstMaxBlock - 1, Lb) - sj + 1;
             }
+            h = bi - si + 1;
         }
         // capture: the rows end at a strip boundary, so row bi is the last
         // strip's bottom row, which the boundary buffer holds afterwards
-        const int Rmax0 = uni(wave_max(bi)), Cmax = uni(wave_max(bj));
+        const int Rmax0 = uni(wave_max(h)), Cmax = uni(wave_max(w));
         const int Rmax = TRACK ? Rmax0 : (Rmax0 + R - 1) / R * R;
-        const int ro = Rmax - bi;  // pad rows before the lane's row 1
-        const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
-        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (live ? S.off[b] : S.pad_off));
-        // capture: the first maximum lies in columns jb .. jb + kFirstMaxBlock - 1
-        const int ncols = ((TRACK ? Cmax : Cmax + kFirstMaxBlock - 1) + 3) & ~3;
+        const int ro = Rmax - h;  // pad rows before the lane's first row
+        const uint8_t* s1 = S.res + (live ? S.off[a] + si - 1 : S.pad_off);
+        // s2 from column sj: 4-byte words from the aligned address below it,
+        // shifted by the misalignment
+        const int64_t c2b = live ? S.off[b] + sj - 1 : S.pad_off;
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (c2b & ~int64_t(3)));
+        const int sh = (int)(c2b & 3);
+        const int ncols = (Cmax + 3) & ~3;
 
         for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int4(-o, kNeg, 0, 0);
 
@@ -983,8 +1023,8 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
             int rowb[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int si = i0 + r - ro;
-                rowb[r] = ((si >= 0 && si < La) ? (int)s1[si] : pad) * ncode;
+                const int k = i0 + r - ro;
+                rowb[r] = ((k >= 0 && k < h) ? (int)s1[k] : pad) * ncode;
             }
             int tr[R], Tr[R], Hr[R], THr[R];
             int rb[TRACK ? R : 1], rT[TRACK ? R : 1];
@@ -995,29 +1035,33 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
             }
             int dtb = -o, dTb = 0;
 
-            uint32_t nw = s2w[0];
+            // residue words (4 columns each): current, next, and the one after
+            uint32_t a1 = s2w[1], a2 = s2w[2];
+            uint32_t nw = __builtin_amdgcn_alignbyte(a1, s2w[0], sh);
             int4 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
             int4 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
             // the table entries of a column are loaded one column ahead
             int2 cur[R];
             {
-                const int c2 = Lb >= 1 ? (int)(nw & 0xFFu) : pad;
+                const int c2 = w >= 1 ? (int)(nw & 0xFFu) : pad;
 #pragma unroll
                 for (int r = 0; r < R; ++r) cur[r] = tab[rowb[r] + c2];
             }
-            uint32_t n2 = s2w[1];  // residue word of the block after next
+            uint32_t n2 = __builtin_amdgcn_alignbyte(a2, a1, sh);  // the block after next
             for (int j = 1; j <= ncols; j += 4) {
-                const uint32_t w = nw;
+                const uint32_t wd = nw;
                 int4 cb[4] = {nb0, nb1, nb2, nb3};
                 nw = n2;
-                n2 = s2w[(j + 7) >> 2];
+                a1 = a2;
+                a2 = s2w[((j + 7) >> 2) + 1];
+                n2 = __builtin_amdgcn_alignbyte(a2, a1, sh);
                 const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
                 nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
-                    int c2n = (int)(((q < 3 ? w : nw) >> (8 * ((q + 1) & 3))) & 0xFFu);
-                    c2n = jj + 1 > Lb ? pad : c2n;
+                    int c2n = (int)(((q < 3 ? wd : nw) >> (8 * ((q + 1) & 3))) & 0xFFu);
+                    c2n = jj + 1 > w ? pad : c2n;
                     int2 nxtp[R];
 #pragma unroll
                     for (int r = 0; r < R; ++r) nxtp[r] = tab[rowb[r] + c2n];
@@ -1068,11 +1112,15 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 
         if (!TRACK) {
             // first strict maximum of row bi in columns jb .. jb + kFirstMaxBlock - 1
+            // (rectangle columns jb - sj + 1 .., those inside it)
 #pragma unroll
             for (int q = 0; q < kFirstMaxBlock; ++q) {
-                const int4 x = bnd[(int64_t)(bj + q) * kWave + lane];
-                const int V = x.x + o;
-                if (V > best) { best = V; bestT = x.z; }
+                const int col = bj - sj + 1 + q;
+                if (col >= 1 && col <= w) {
+                    const int4 x = bnd[(int64_t)col * kWave + lane];
+                    const int V = x.x + o;
+                    if (V > best) { best = V; bestT = x.z; }
+                }
             }
         }
         if (live) {
@@ -1085,6 +1133,138 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     if (lane == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + (TRACK ? kMetaFlaggedRun : kMetaCaptureRun),
                   (unsigned long long)run);
+}
+
+// ---------------------------------------------------------------------------
+// Reverse pass: where can an optimal alignment ending in the first-maximum
+// block start?  Per pair (one per lane), the DP of the reversed prefixes
+// s1[1..bi] and s2[1..wend], wend = min(jb + kFirstMaxBlock - 1, Lb), anchored
+// at the block: reverse row 1 is s1[bi], reverse column 1 is s2[wend]; the
+// border cell (0, c) is 0 for c < wend - jb + 1 (the block's columns) and
+// -inf elsewhere, and there is no 0 floor.  V'(i', c') is then the best score
+// of an alignment path from the forward cell (bi - i' + 1, wend - c' + 1) to a
+// cell of the block.  Every forward V is at most the first-maximum score S, so
+// V' <= S, and the start of the reference's traceback path reaches S: it lies
+// in a reverse row and column at most the last ones where V' reaches S.  With
+// si = bi - hit_row + 1 and sj = wend - hit_col + 1, the rectangle
+// [si..bi] x [sj..wend] holds every optimal path into the block, and the local
+// DP over it (0 border) gives the reference's statistics: on the path each
+// value equals the full DP's (its prefix lies in the rectangle and no value can
+// exceed the full DP's), and so does every comparison the traceback depends on
+// there (the diagonal candidate of a DIAGONAL cell and the gap chain of a gap
+// run lie on the path; the other candidates can only be lower).
+// No hit (only possible when S = 0): the whole prefix rectangle.
+template <int R>
+__global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
+{
+    extern __shared__ int2 tab[];  // [ncode][ncode]
+    const int lane = threadIdx.x;
+    const int ncode = S.ncode, pad = ncode - 1;
+    for (int k = lane; k < ncode * ncode; k += kWave) tab[k] = S.prof[k];
+    __syncthreads();
+
+    int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
+    const BlockDesc* D = T.desc;
+    const int64_t r0 = D[1].r0, c0 = D[1].c0;
+    const uint32_t ld1 = (uint32_t)D[1].ld;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int t_lo = T.meta[kMetaFlaggedTiles], t_hi = T.meta[kMetaTiles];
+    uint64_t run = 0;  // cells evaluated, added to meta after the task loop
+
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(T.queue, 1);
+        const int t = uni(v) + t_lo;
+        if (t >= t_hi) break;
+
+        const uint32_t p = T.sorted[(int64_t)t * kWave + lane];
+        const bool live = p != ~0u;
+        int a = 0, b = 0, bi = 0, wend = 0, anc = 0, best = INT_MAX;
+        if (live) {
+            a = (int)(r0 + p / ld1);
+            b = (int)(c0 + p % ld1);
+            const uint32_t cell = T.best_cell[p];
+            bi = (int)((cell >> 16) & 0x7FFFu);
+            const int jb = (int)(cell & 0xFFFFu);
+            wend = min(jb + kFirstMaxBlock - 1, (int)S.len[b]);
+            anc = wend - jb + 1;
+            best = T.best_score[p];
+        }
+        const int Rmax = uni(wave_max(bi)), Cmax = uni(wave_max(wend));
+        const int ncols = (Cmax + 3) & ~3;
+        const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
+        // reverse columns c'..c'+3 are s2[wend-c'-3 .. wend-c'] (0-based),
+        // read as one misaligned word; columns past wend (bytes before the
+        // sequence) are pad, and word indices below the buffer clamp to 0
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res);
+        const int64_t top = live ? S.off[b] + wend - 4 : S.pad_off;  // first byte of reverse columns 1..4
+        const int sh = (int)(top & 3);
+
+        // reverse row 0: V' = 0 on the block's columns (t = V' - o), else -inf
+        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2(j < anc ? -o : kNeg, kNeg);
+
+        int hitr = 0, hitc = 0;
+        for (int i0 = 0; i0 < Rmax; i0 += R) {
+            int rowb[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int k = bi - 1 - (i0 + r);
+                rowb[r] = (k >= 0 ? (int)s1[k] : pad) * ncode;
+            }
+            int tr[R], Hr[R], rm[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) { tr[r] = kNeg; Hr[r] = kNeg; rm[r] = kNeg; }
+            int dtb = i0 == 0 ? -o : kNeg;  // V'(i0, 0) - o
+
+            int64_t wi = top >> 2;  // word of the current 4 columns (may be below 0)
+            uint32_t lo = s2w[wi < 0 ? 0 : wi], hi = s2w[wi + 1 < 0 ? 0 : wi + 1];
+            for (int j = 1; j <= ncols; j += 4) {
+                const uint32_t wd = __builtin_amdgcn_alignbyte(hi, lo, sh);  // byte 3 = column j
+                hi = lo;
+                --wi;
+                lo = s2w[wi < 0 ? 0 : wi];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int jj = j + q;
+                    int c2 = (int)((wd >> (8 * (3 - q))) & 0xFFu);
+                    c2 = jj > wend ? pad : c2;
+                    const int2 bu = bnd[(int64_t)jj * kWave + lane];
+                    int tu = bu.x, Gu = bu.y;
+                    int dt = dtb;
+                    dtb = tu;
+                    int cm = kNeg;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int sc = tab[rowb[r] + c2].x;
+                        const int G = max(Gu - e, tu);
+                        const int H = max(Hr[r] - e, tr[r]);
+                        const int V = max(max(dt + sc, G), H);
+                        rm[r] = max(rm[r], V);
+                        cm = max(cm, V);
+                        const int tn = V - o;
+                        dt = tr[r];
+                        tr[r] = tn;
+                        Hr[r] = H;
+                        tu = tn;
+                        Gu = G;
+                    }
+                    bnd[(int64_t)jj * kWave + lane] = make_int2(tu, Gu);
+                    // strips restart at column 1: keep the largest hit column
+                    hitc = (cm == best && jj <= wend) ? max(hitc, jj) : hitc;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) hitr = (rm[r] == best && i0 + r < bi) ? i0 + r + 1 : hitr;
+        }
+        if (live) {
+            const int si = (hitr > 0 && hitc > 0) ? bi - hitr + 1 : 1;
+            const int sj = (hitr > 0 && hitc > 0) ? wend - hitc + 1 : 1;
+            T.start_cell[p] = ((uint32_t)si << 16) | (uint32_t)sj;
+        }
+        run += (uint64_t)Rmax * (uint64_t)ncols * kWave;
+    }
+    // once, after the task loop (see rerun_kernel)
+    if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n)
@@ -1223,6 +1403,21 @@ hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, in
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t st)
+{
+    const size_t lds = (size_t)S.ncode * S.ncode * sizeof(int2);
+    hipLaunchKernelGGL((reverse_kernel<16>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+    return hipGetLastError();
+}
+
+int reverse_blocks_per_cu(int ncode)
+{
+    const size_t lds = (size_t)ncode * ncode * sizeof(int2);
+    int nb = 0;
+    const hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16>, kWave, lds);
+    return err == hipSuccess ? nb : 0;
 }
 
 int rerun_blocks_per_cu(int strip_rows, int ncode)

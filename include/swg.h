@@ -160,13 +160,16 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
 #define SWG_KERNEL_RERUN_CAPTURE 5  /* re-run up to the known first-maximum cell */
 #define SWG_KERNEL_LPP16         6  /* 16 lanes per pair */
 #define SWG_KERNEL_GLOBAL        7  /* global (NW) alignment */
+#define SWG_KERNEL_REVERSE       8  /* reverse score pass: where the first-maximum alignments can start */
 typedef struct swg_kernel_time {
     int32_t kernel;     /* SWG_KERNEL_* */
     int32_t reserved;
     double  ms;         /* event time from the previous kernel's end (block start for the first) */
     double  cells;      /* DP cells the kernel is charged with: the block's cells (one-pass, score
-                           pass); the rectangles up to each pair's first-maximum block (capture
-                           re-run); the flagged pairs' cells (tracking re-run); 0 for the sort */
+                           pass); the rectangles up to each pair's first-maximum block (reverse
+                           pass); the rectangles from each pair's earliest optimal start to that
+                           block (capture re-run); the flagged pairs' cells (tracking re-run);
+                           0 for the sorts */
     double  cells_run;  /* DP cells the kernel evaluated, idle lanes, pad rows and columns included */
 } swg_kernel_time;
 /* After a block run with SWG_OPT_KERNEL_TIMING (synchronizes the context

@@ -365,7 +365,7 @@ def test_swms_restart_reproduces_block_files(tmp_path, ednafull):
 
 
 @pytest.mark.parametrize("schedule,names", [
-    (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "rerun_track", "rerun_capture"]),
+    (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "reverse", "sort", "rerun_track", "rerun_capture"]),
     (ca.SCHED_PERSISTENT_I32, ["one_pass"]), (ca.SCHED_PLAIN, ["one_pass"])])
 def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
     say("kernel timing schedule=%d" % schedule)
@@ -387,7 +387,7 @@ def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
     for name, _, c, run in kt:
         if name in ("score16", "one_pass"):
             assert c == block
-        if name == "rerun_capture":
+        if name in ("rerun_capture", "reverse"):
             assert 0 < c < block
         assert run >= c
     al.close()
