@@ -29,13 +29,13 @@ PEAK_OPS = {"int32": 256 * 4 * 16 * 2.4e9, "int16x2": 2 * 256 * 4 * 16 * 2.4e9}
 # update (the score pass; the reverse pass tracks where the maximum is
 # reached instead); the global variant carries one more word (lead
 # and trailing gap run), 30.  Mode = the arithmetic the kernel runs in.
-ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int32"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
+ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
            "one_pass": (24, "int32"), "lpp16": (24, "int32"), "global": (30, "int32")}
 # VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
 # unrolled), counted from this build's gfx950 assembly by
 # `python -m csharp_amd.isa_valu_count`; a packed instruction counts once for
 # the two cells it evaluates.  Reported for information.
-VALU_PER_CELL = {"score16": 5.75, "reverse": 9.594, "rerun_capture": 22.797, "rerun_track": 24.969, "one_pass": 24.969,
+VALU_PER_CELL = {"score16": 5.75, "reverse": 7.297, "rerun_capture": 22.797, "rerun_track": 24.969, "one_pass": 24.969,
                  "global": 38.875}
 
 

@@ -411,7 +411,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     SortSet P{}, P2{};
     ReverseSet RV{};
     RerunSet RT{}, RC{};
-    const int sort_grid = std::max(1, c->num_cu * 4 / ln.share);
+    const int sort_grid = std::max(1, c->num_cu * 2 / ln.share);
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;

@@ -848,21 +848,33 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 
 // Also accumulates, for the accounting of the next pass, the cells of the
 // pairs' rectangles (pair_bucket) and of the flagged pairs.
-__global__ __launch_bounds__(256) void sort_count_kernel(SortSet P)
+// Counts per bucket are gathered in LDS per workgroup and added to the
+// global counts once per non-empty bucket: most re-run rectangles are small
+// and fall into a few hundred buckets, so global atomics per pair would
+// serialise on them.
+constexpr int kSortThreads = 512;
+
+__global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
 {
+    __shared__ int hist[kNumBuckets];
     const BlockDesc D1 = P.desc[1];
     const bool lower = P.desc[0].mirror != 0;
+    for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads) hist[k] = 0;
+    __syncthreads();
     uint64_t rect = 0, flagged = 0;
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
-        for (int64_t c = threadIdx.x; c < n; c += 256) {
+        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
             uint64_t cells;
             bool fl;
             const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
-            atomicAdd(&P.count[k], 1);
+            atomicAdd(&hist[k], 1);
             if (fl) flagged += cells; else rect += cells;
         }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads)
+        if (hist[k]) atomicAdd(&P.count[k], hist[k]);
     rect = wave_sum_u64(rect);
     flagged = wave_sum_u64(flagged);
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -908,8 +920,11 @@ __global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
     }
 }
 
-__global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
+// Each workgroup counts its pairs per bucket in LDS, reserves one slot range
+// per non-empty bucket, then places its pairs with LDS atomics.
+__global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(SortSet P)
 {
+    __shared__ int hist[kNumBuckets];
     const BlockDesc D1 = P.desc[1];
     const bool lower = P.desc[0].mirror != 0;
     // idle lanes: the flagged list's last tile and the last tile
@@ -921,15 +936,28 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(SortSet P)
         const int end = (n + kWave - 1) / kWave * kWave;
         if (n + (int)threadIdx.x < end) P.sorted[n + threadIdx.x] = ~0u;
     }
+    for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads) hist[k] = 0;
+    __syncthreads();
+    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
+        const int64_t n = row_pairs(D1, lower, row);
+        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
+            uint64_t cells;
+            bool fl;
+            atomicAdd(&hist[pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl)], 1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads)
+        if (hist[k]) hist[k] = P.base[k] + atomicAdd(&P.cursor[k], hist[k]);
+    __syncthreads();
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
         const int64_t first = (int64_t)row * D1.ld;
-        for (int64_t c = threadIdx.x; c < n; c += 256) {
+        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
             uint64_t cells;
             bool fl;
             const int k = pair_bucket(P, D1, first + c, P.len[D1.c0 + c], &cells, &fl);
-            const int slot = P.base[k] + atomicAdd(&P.cursor[k], 1);
-            P.sorted[slot] = (uint32_t)(first + c);
+            P.sorted[atomicAdd(&hist[k], 1)] = (uint32_t)(first + c);
         }
     }
 }
@@ -1154,13 +1182,22 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 // there (the diagonal candidate of a DIAGONAL cell and the gap chain of a gap
 // run lie on the path; the other candidates can only be lower).
 // No hit (only possible when S = 0): the whole prefix rectangle.
+//
+// Two pairs per lane in the int16 halves of one register (v_pk_* with
+// saturation): pair A = slot 128t + lane, pair B = slot 128t + 64 + lane of
+// the sorted list.  Saturation cannot create a hit: a path through a value
+// clamped at -32768 would need a segment scoring >= 32768 + S to climb back
+// to S, more than any local alignment of the pair scores (S < 32766).
 template <int R>
 __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
 {
-    extern __shared__ int2 tab[];  // [ncode][ncode]
+    extern __shared__ short tab16[];  // [ncode][ncode]: score + gap_open, clamped to int16
     const int lane = threadIdx.x;
     const int ncode = S.ncode, pad = ncode - 1;
-    for (int k = lane; k < ncode * ncode; k += kWave) tab[k] = S.prof[k];
+    for (int k = lane; k < ncode * ncode; k += kWave) {
+        const int v = S.prof[k].x;
+        tab16[k] = (short)(v < kPad16 ? kPad16 : v);
+    }
     __syncthreads();
 
     int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
@@ -1168,100 +1205,170 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
     const int64_t r0 = D[1].r0, c0 = D[1].c0;
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
-    const int t_lo = T.meta[kMetaFlaggedTiles], t_hi = T.meta[kMetaTiles];
+    const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
+    const uint32_t NEG2 = 0x80008000u, T0 = splat16(-o);
+    const int64_t slot0 = (int64_t)T.meta[kMetaFlaggedTiles] * kWave, slot1 = (int64_t)T.meta[kMetaTiles] * kWave;
+    const int ntasks = (int)((slot1 - slot0 + 2 * kWave - 1) / (2 * kWave));
     uint64_t run = 0;  // cells evaluated, added to meta after the task loop
 
     for (;;) {
         int v = 0;
         if (lane == 0) v = atomicAdd(T.queue, 1);
-        const int t = uni(v) + t_lo;
-        if (t >= t_hi) break;
+        const int t = uni(v);
+        if (t >= ntasks) break;
 
-        const uint32_t p = T.sorted[(int64_t)t * kWave + lane];
-        const bool live = p != ~0u;
-        int a = 0, b = 0, bi = 0, wend = 0, anc = 0, best = INT_MAX;
-        if (live) {
-            a = (int)(r0 + p / ld1);
-            b = (int)(c0 + p % ld1);
-            const uint32_t cell = T.best_cell[p];
-            bi = (int)((cell >> 16) & 0x7FFFu);
-            const int jb = (int)(cell & 0xFFFFu);
-            wend = min(jb + kFirstMaxBlock - 1, (int)S.len[b]);
-            anc = wend - jb + 1;
-            best = T.best_score[p];
+        // per half (h = 0: pair A, 1: pair B)
+        int a[2], b[2], bi[2], wend[2], anc[2], best[2], sh[2];
+        bool live[2];
+        uint32_t p[2];
+        int64_t wi[2];
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const int64_t slot = slot0 + (int64_t)t * 2 * kWave + hh * kWave + lane;
+            p[hh] = slot < slot1 ? T.sorted[slot] : ~0u;
+            live[hh] = p[hh] != ~0u;
+            a[hh] = 0; b[hh] = 0; bi[hh] = 0; wend[hh] = 0; anc[hh] = 0; best[hh] = 32767;
+            if (live[hh]) {
+                a[hh] = (int)(r0 + p[hh] / ld1);
+                b[hh] = (int)(c0 + p[hh] % ld1);
+                const uint32_t cell = T.best_cell[p[hh]];
+                bi[hh] = (int)((cell >> 16) & 0x7FFFu);
+                const int jb = (int)(cell & 0xFFFFu);
+                wend[hh] = min(jb + kFirstMaxBlock - 1, (int)S.len[b[hh]]);
+                anc[hh] = wend[hh] - jb + 1;
+                best[hh] = T.best_score[p[hh]];
+            }
+            // reverse columns c'..c'+3 are s2[wend-c'-3 .. wend-c'] (0-based),
+            // one misaligned word; columns past wend are pad, word indices
+            // below the buffer clamp to 0
+            const int64_t top = live[hh] ? S.off[b[hh]] + wend[hh] - 4 : S.pad_off;
+            sh[hh] = (int)(top & 3);
+            wi[hh] = top >> 2;
         }
-        const int Rmax = uni(wave_max(bi)), Cmax = uni(wave_max(wend));
+        const int Rmax = uni(wave_max(max(bi[0], bi[1]))), Cmax = uni(wave_max(max(wend[0], wend[1])));
         const int ncols = (Cmax + 3) & ~3;
-        const uint8_t* s1 = S.res + (live ? S.off[a] : S.pad_off);
-        // reverse columns c'..c'+3 are s2[wend-c'-3 .. wend-c'] (0-based),
-        // read as one misaligned word; columns past wend (bytes before the
-        // sequence) are pad, and word indices below the buffer clamp to 0
+        const uint8_t* s1a = S.res + (live[0] ? S.off[a[0]] : S.pad_off);
+        const uint8_t* s1b = S.res + (live[1] ? S.off[a[1]] : S.pad_off);
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res);
-        const int64_t top = live ? S.off[b] + wend - 4 : S.pad_off;  // first byte of reverse columns 1..4
-        const int sh = (int)(top & 3);
+        const uint32_t S2 = ((uint32_t)best[0] & 0xFFFFu) | ((uint32_t)best[1] << 16);
+        const uint32_t BI2 = ((uint32_t)bi[0] & 0xFFFFu) | ((uint32_t)bi[1] << 16);
+        const uint32_t WE2 = ((uint32_t)wend[0] & 0xFFFFu) | ((uint32_t)wend[1] << 16);
 
-        // reverse row 0: V' = 0 on the block's columns (t = V' - o), else -inf
-        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2(j < anc ? -o : kNeg, kNeg);
+        // reverse row 0: V' = 0 on each pair's block columns (t = V' - o), else -inf
+        for (int j = 1; j <= ncols; ++j) {
+            const uint32_t tl = j < anc[0] ? (T0 & 0xFFFFu) : 0x8000u;
+            const uint32_t th = j < anc[1] ? (T0 & 0xFFFF0000u) : 0x80000000u;
+            bnd[(int64_t)j * kWave + lane] = make_int2((int)(tl | th), (int)NEG2);
+        }
 
-        int hitr = 0, hitc = 0;
+        uint32_t hitr = 0, hitc = 0;
         for (int i0 = 0; i0 < Rmax; i0 += R) {
-            int rowb[R];
+            int rba[R], rbb[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int k = bi - 1 - (i0 + r);
-                rowb[r] = (k >= 0 ? (int)s1[k] : pad) * ncode;
+                const int ka = bi[0] - 1 - (i0 + r), kb = bi[1] - 1 - (i0 + r);
+                rba[r] = (ka >= 0 ? (int)s1a[ka] : pad) * ncode;
+                rbb[r] = (kb >= 0 ? (int)s1b[kb] : pad) * ncode;
             }
-            int tr[R], Hr[R], rm[R];
+            uint32_t tr[R], Hr[R], rm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = kNeg; Hr[r] = kNeg; rm[r] = kNeg; }
-            int dtb = i0 == 0 ? -o : kNeg;  // V'(i0, 0) - o
+            for (int r = 0; r < R; ++r) { tr[r] = NEG2; Hr[r] = NEG2; rm[r] = NEG2; }
+            uint32_t dtb = i0 == 0 ? T0 : NEG2;  // V'(i0, 0) - o
 
-            int64_t wi = top >> 2;  // word of the current 4 columns (may be below 0)
-            uint32_t lo = s2w[wi < 0 ? 0 : wi], hi = s2w[wi + 1 < 0 ? 0 : wi + 1];
+            // residue words of the current and the next 4 columns (byte 3 = the
+            // first of the 4), rolling down through s2
+            int64_t wa = wi[0], wb = wi[1];
+            uint32_t loa = s2w[wa < 0 ? 0 : wa], hia = s2w[wa + 1 < 0 ? 0 : wa + 1];
+            uint32_t lob = s2w[wb < 0 ? 0 : wb], hib = s2w[wb + 1 < 0 ? 0 : wb + 1];
+            uint32_t wda = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
+            uint32_t wdb = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
+            hia = loa; hib = lob; --wa; --wb;
+            loa = s2w[wa < 0 ? 0 : wa];
+            lob = s2w[wb < 0 ? 0 : wb];
+            uint32_t wdaN = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
+            uint32_t wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
+            // both pairs' scores of a column, packed, loaded one column ahead
+            auto col_scores = [&](int ca, int cb, uint32_t* out) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    out[r] = (uint32_t)(uint16_t)tab16[rba[r] + ca] | ((uint32_t)(uint16_t)tab16[rbb[r] + cb] << 16);
+            };
+            uint32_t cur[R];
+            col_scores(wend[0] >= 1 ? (int)(wda >> 24) : pad, wend[1] >= 1 ? (int)(wdb >> 24) : pad, cur);
             for (int j = 1; j <= ncols; j += 4) {
-                const uint32_t wd = __builtin_amdgcn_alignbyte(hi, lo, sh);  // byte 3 = column j
-                hi = lo;
-                --wi;
-                lo = s2w[wi < 0 ? 0 : wi];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
-                    int c2 = (int)((wd >> (8 * (3 - q))) & 0xFFu);
-                    c2 = jj > wend ? pad : c2;
+                    int ca = (int)(((q < 3 ? wda : wdaN) >> (8 * (3 - ((q + 1) & 3)))) & 0xFFu);
+                    int cb = (int)(((q < 3 ? wdb : wdbN) >> (8 * (3 - ((q + 1) & 3)))) & 0xFFu);
+                    ca = jj + 1 > wend[0] ? pad : ca;
+                    cb = jj + 1 > wend[1] ? pad : cb;
+                    uint32_t nx[R];
+                    col_scores(ca, cb, nx);
+                    __builtin_amdgcn_sched_barrier(0);
                     const int2 bu = bnd[(int64_t)jj * kWave + lane];
-                    int tu = bu.x, Gu = bu.y;
-                    int dt = dtb;
+                    uint32_t tu = (uint32_t)bu.x, Gu = (uint32_t)bu.y;
+                    uint32_t dt = dtb;
                     dtb = tu;
-                    int cm = kNeg;
+                    s16x2 cm = v2(NEG2);
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const int sc = tab[rowb[r] + c2].x;
-                        const int G = max(Gu - e, tu);
-                        const int H = max(Hr[r] - e, tr[r]);
-                        const int V = max(max(dt + sc, G), H);
-                        rm[r] = max(rm[r], V);
-                        cm = max(cm, V);
-                        const int tn = V - o;
+                        const uint32_t sp = cur[r];
+                        const s16x2 G = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Gu), E2), v2(tu));
+                        const s16x2 H = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Hr[r]), E2),
+                                                                  v2(tr[r]));
+                        const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G), H);
+                        rm[r] = u2(__builtin_elementwise_max(v2(rm[r]), V));
+                        cm = __builtin_elementwise_max(cm, V);
+                        const uint32_t tn = u2(__builtin_elementwise_sub_sat(V, O2));
                         dt = tr[r];
                         tr[r] = tn;
-                        Hr[r] = H;
+                        Hr[r] = u2(H);
                         tu = tn;
-                        Gu = G;
+                        Gu = u2(G);
                     }
-                    bnd[(int64_t)jj * kWave + lane] = make_int2(tu, Gu);
-                    // strips restart at column 1: keep the largest hit column
-                    hitc = (cm == best && jj <= wend) ? max(hitc, jj) : hitc;
+                    bnd[(int64_t)jj * kWave + lane] = make_int2((int)tu, (int)Gu);
+                    // column hit: cm == S (cm <= S, so S - cm >= 0 is 0 only on a
+                    // hit) inside the pair's columns; the largest column wins
+                    const uint32_t JJ = splat16(jj);
+                    uint32_t miss, out;
+                    const uint32_t dc = u2(__builtin_elementwise_sub_sat(cm, v2(S2)));   // <= 0
+                    const uint32_t dw = u2(__builtin_elementwise_sub_sat(v2(WE2), v2(JJ)));  // >= 0 inside
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(miss) : "s"(0x000F000Fu), "v"(dc));
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(out) : "s"(0x000F000Fu), "v"(dw));
+                    hitc = u2(__builtin_elementwise_max(v2(hitc), v2(JJ & ~miss & ~out)));
+#pragma unroll
+                    for (int r = 0; r < R; ++r) cur[r] = nx[r];
                 }
+                wda = wdaN; wdb = wdbN;
+                hia = loa; hib = lob; --wa; --wb;
+                loa = s2w[wa < 0 ? 0 : wa];
+                lob = s2w[wb < 0 ? 0 : wb];
+                wdaN = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
+                wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
             }
 #pragma unroll
-            for (int r = 0; r < R; ++r) hitr = (rm[r] == best && i0 + r < bi) ? i0 + r + 1 : hitr;
+            for (int r = 0; r < R; ++r) {
+                const uint32_t RR = splat16(i0 + r + 1);
+                uint32_t miss, out;
+                const uint32_t dr = u2(__builtin_elementwise_sub_sat(v2(rm[r]), v2(S2)));
+                const uint32_t dv = u2(__builtin_elementwise_sub_sat(v2(BI2), v2(RR)));
+                asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(miss) : "s"(0x000F000Fu), "v"(dr));
+                asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(out) : "s"(0x000F000Fu), "v"(dv));
+                hitr = u2(__builtin_elementwise_max(v2(hitr), v2(RR & ~miss & ~out)));
+            }
         }
-        if (live) {
-            const int si = (hitr > 0 && hitc > 0) ? bi - hitr + 1 : 1;
-            const int sj = (hitr > 0 && hitc > 0) ? wend - hitc + 1 : 1;
-            T.start_cell[p] = ((uint32_t)si << 16) | (uint32_t)sj;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            if (live[hh]) {
+                const int hr = (int)((hitr >> (16 * hh)) & 0xFFFFu), hc = (int)((hitc >> (16 * hh)) & 0xFFFFu);
+                const int si = (hr > 0 && hc > 0) ? bi[hh] - hr + 1 : 1;
+                const int sj = (hr > 0 && hc > 0) ? wend[hh] - hc + 1 : 1;
+                T.start_cell[p[hh]] = ((uint32_t)si << 16) | (uint32_t)sj;
+            }
         }
-        run += (uint64_t)Rmax * (uint64_t)ncols * kWave;
+        run += 2ull * (uint64_t)Rmax * (uint64_t)ncols * kWave;
     }
     // once, after the task loop (see rerun_kernel)
     if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
@@ -1379,12 +1486,12 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
 
 hipError_t launch_sort(const SortSet& P, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(kSortThreads), 0, st, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kWave), 0, st, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(kSortThreads), 0, st, P);
     return hipGetLastError();
 }
 
@@ -1407,14 +1514,14 @@ hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, in
 
 hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t st)
 {
-    const size_t lds = (size_t)S.ncode * S.ncode * sizeof(int2);
+    const size_t lds = (size_t)S.ncode * S.ncode * sizeof(short);
     hipLaunchKernelGGL((reverse_kernel<16>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
     return hipGetLastError();
 }
 
 int reverse_blocks_per_cu(int ncode)
 {
-    const size_t lds = (size_t)ncode * ncode * sizeof(int2);
+    const size_t lds = (size_t)ncode * ncode * sizeof(short);
     int nb = 0;
     const hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16>, kWave, lds);
     return err == hipSuccess ? nb : 0;

@@ -20,7 +20,7 @@ KERNELS = {
     "score16_kernelILi16ELb1E": ("score16", 4 * 16 * 2),       # 4 columns x 16 rows x 2 pairs per lane
     "score16_kernelILi16ELb0E": ("score16 (large alphabet)", 4 * 16 * 2),
     "rerun_kernelILi16ELb0E": ("rerun_capture", 4 * 16),
-    "reverse_kernelILi16E": ("reverse", 4 * 16),
+    "reverse_kernelILi16E": ("reverse", 4 * 16 * 2),           # 2 pairs per lane
     "sw_kernelILi16ELb1E": ("one_pass", 4 * 16),
     "nw_kernelILi16ELb1E": ("global", 16),
 }
