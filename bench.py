@@ -35,7 +35,11 @@ ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture"
 # unrolled), counted from this build's gfx950 assembly by
 # `python -m csharp_amd.isa_valu_count`; a packed instruction counts once for
 # the two cells it evaluates.  Reported for information.
-VALU_PER_CELL = {"score16": 5.75, "reverse": 7.297, "rerun_capture": 22.797, "rerun_track": 24.969, "one_pass": 24.969,
+# The reverse pass evaluates a data-dependent band (the DP cells that can lie
+# on an optimal path back from the maximum), so its work is the cells it
+# evaluates, not the prefix rectangles it decides.
+EVALUATED_IS_WORK = {"reverse"}
+VALU_PER_CELL = {"score16": 5.75, "reverse": 7.469, "rerun_capture": 22.797, "rerun_track": 24.969, "one_pass": 24.969,
                  "global": 38.875}
 
 
@@ -227,7 +231,7 @@ def main():
     dom = max(ktimes, key=lambda k: k[1])
     dname, dms, dcells, drun = dom
     w, mode = ALG_OPS.get(dname, (None, "int32"))
-    achieved = dcells * w / (dms / 1e3) / 1e12 if w else None
+    achieved = (drun if dname in EVALUATED_IS_WORK else dcells) * w / (dms / 1e3) / 1e12 if w else None
     peak = PEAK_OPS[mode] / 1e12
     # HBM bytes per launch of that kernel from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), if one exists
@@ -279,7 +283,8 @@ def main():
             "valu_instr_per_cell": VALU_PER_CELL.get(dname),
         },
         "kernels": [{"kernel": k[0], "ms": round(k[1], 2), "cells": k[2], "cells_evaluated": k[3],
-                     "frac": (round(k[2] * ALG_OPS[k[0]][0] / (k[1] / 1e3) / PEAK_OPS[ALG_OPS[k[0]][1]], 4)
+                     "frac": (round((k[3] if k[0] in EVALUATED_IS_WORK else k[2]) * ALG_OPS[k[0]][0] / (k[1] / 1e3)
+                                    / PEAK_OPS[ALG_OPS[k[0]][1]], 4)
                               if k[0] in ALG_OPS and k[1] > 0 else None)}
                     for k in ktimes],
         "cpu_baseline": cpu,

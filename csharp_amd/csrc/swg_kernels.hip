@@ -1189,7 +1189,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 // clamped at -32768 would need a segment scoring >= 32768 + S to climb back
 // to S, more than any local alignment of the pair scores (S < 32766).
 template <int R>
-__global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
 {
     extern __shared__ short tab16[];  // [ncode][ncode]: score + gap_open, clamped to int16
     const int lane = threadIdx.x;
@@ -1254,13 +1254,34 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
         const uint32_t BI2 = ((uint32_t)bi[0] & 0xFFFFu) | ((uint32_t)bi[1] << 16);
         const uint32_t WE2 = ((uint32_t)wend[0] & 0xFFFFu) | ((uint32_t)wend[1] << 16);
 
-        // reverse row 0: V' = 0 on each pair's block columns (t = V' - o), else -inf
-        for (int j = 1; j <= ncols; ++j) {
+        // reverse row 0: V' = 0 on each pair's block columns (t = V' - o), else
+        // -inf; only columns 0 .. kFirstMaxBlock - 1 hold a 0
+        for (int j = 1; j <= kFirstMaxBlock; ++j) {
             const uint32_t tl = j < anc[0] ? (T0 & 0xFFFFu) : 0x8000u;
             const uint32_t th = j < anc[1] ? (T0 & 0xFFFF0000u) : 0x80000000u;
             bnd[(int64_t)j * kWave + lane] = make_int2((int)(tl | th), (int)NEG2);
         }
 
+        // Band (X-drop).  Call a cell live when V' >= min(0, o - e).  Every
+        // cell x of an optimal path from a hit to the block is live: the part
+        // of the path before x is a local alignment ending at a forward cell
+        // whose value is at most S, and the part from x on scores at most
+        // V'(x); the two parts add up to S, or to S + o - e when x splits a
+        // gap run (both parts then charge the run's opening), so V'(x) >=
+        // min(0, o - e).  So a cell none of whose three predecessors is live
+        // is left out (-inf), which keeps every hit and, values only
+        // dropping, creates none; and once a row has no live cell, no later
+        // row holds a hit (the path from such a hit would cross that row at a
+        // live cell).  Per 16-row strip the wave evaluates columns jlo .. jend:
+        // jlo = the first 4-column group with a live cell in the row above the
+        // strip (earlier columns have no live predecessor, by induction along
+        // the row), and the strip stops after a group with no live cell in any
+        // row that lies past the last live column of the row above (later
+        // columns again have no live predecessor).  bnd holds the row above for
+        // columns jlo .. jtop; later columns read -inf.
+        const uint32_t LIVE2 = splat16(min(0, o - e));
+        int jlo = 1, jtop = kFirstMaxBlock, live_hi = kFirstMaxBlock - 1;
+        uint32_t done = 0;  // per half: 0xFFFF once a row without live cells (or past bi) was seen
         uint32_t hitr = 0, hitc = 0;
         for (int i0 = 0; i0 < Rmax; i0 += R) {
             int rba[R], rbb[R];
@@ -1273,11 +1294,11 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
             uint32_t tr[R], Hr[R], rm[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) { tr[r] = NEG2; Hr[r] = NEG2; rm[r] = NEG2; }
-            uint32_t dtb = i0 == 0 ? T0 : NEG2;  // V'(i0, 0) - o
+            uint32_t dtb = i0 == 0 ? T0 : NEG2;  // V'(i0, jlo - 1) - o (jlo = 1 in the first strip)
 
             // residue words of the current and the next 4 columns (byte 3 = the
             // first of the 4), rolling down through s2
-            int64_t wa = wi[0], wb = wi[1];
+            int64_t wa = wi[0] - ((jlo - 1) >> 2), wb = wi[1] - ((jlo - 1) >> 2);
             uint32_t loa = s2w[wa < 0 ? 0 : wa], hia = s2w[wa + 1 < 0 ? 0 : wa + 1];
             uint32_t lob = s2w[wb < 0 ? 0 : wb], hib = s2w[wb + 1 < 0 ? 0 : wb + 1];
             uint32_t wda = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
@@ -1294,8 +1315,10 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                     out[r] = (uint32_t)(uint16_t)tab16[rba[r] + ca] | ((uint32_t)(uint16_t)tab16[rbb[r] + cb] << 16);
             };
             uint32_t cur[R];
-            col_scores(wend[0] >= 1 ? (int)(wda >> 24) : pad, wend[1] >= 1 ? (int)(wdb >> 24) : pad, cur);
-            for (int j = 1; j <= ncols; j += 4) {
+            col_scores(wend[0] >= jlo ? (int)(wda >> 24) : pad, wend[1] >= jlo ? (int)(wdb >> 24) : pad, cur);
+            int lo_next = 0, hi_next = 0, jend = jlo - 1;
+            for (int j = jlo; j <= ncols; j += 4) {
+                uint32_t gm = NEG2, lm = NEG2;  // group maxima: all rows, last row
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
@@ -1306,8 +1329,8 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                     uint32_t nx[R];
                     col_scores(ca, cb, nx);
                     __builtin_amdgcn_sched_barrier(0);
-                    const int2 bu = bnd[(int64_t)jj * kWave + lane];
-                    uint32_t tu = (uint32_t)bu.x, Gu = (uint32_t)bu.y;
+                    const int2 bu = bnd[(int64_t)jj * kWave + lane];  // columns past jtop: stale, read as -inf
+                    uint32_t tu = jj <= jtop ? (uint32_t)bu.x : NEG2, Gu = jj <= jtop ? (uint32_t)bu.y : NEG2;
                     uint32_t dt = dtb;
                     dtb = tu;
                     s16x2 cm = v2(NEG2);
@@ -1321,6 +1344,7 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                         const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G), H);
                         rm[r] = u2(__builtin_elementwise_max(v2(rm[r]), V));
                         cm = __builtin_elementwise_max(cm, V);
+                        if (r == R - 1) lm = u2(__builtin_elementwise_max(v2(lm), V));
                         const uint32_t tn = u2(__builtin_elementwise_sub_sat(V, O2));
                         dt = tr[r];
                         tr[r] = tn;
@@ -1338,6 +1362,7 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                     asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(miss) : "s"(0x000F000Fu), "v"(dc));
                     asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(out) : "s"(0x000F000Fu), "v"(dw));
                     hitc = u2(__builtin_elementwise_max(v2(hitc), v2(JJ & ~miss & ~out)));
+                    gm = u2(__builtin_elementwise_max(v2(gm), cm));
 #pragma unroll
                     for (int r = 0; r < R; ++r) cur[r] = nx[r];
                 }
@@ -1347,17 +1372,41 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                 lob = s2w[wb < 0 ? 0 : wb];
                 wdaN = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
                 wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
+                jend = j + 3;
+                // halves with V' >= -o: V' - (-o) >= 0, sign mask 0
+                uint32_t dg, dl;
+                const uint32_t xg = u2(__builtin_elementwise_sub_sat(v2(gm), v2(LIVE2)));
+                const uint32_t xl = u2(__builtin_elementwise_sub_sat(v2(lm), v2(LIVE2)));
+                asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(dg) : "s"(0x000F000Fu), "v"(xg));
+                asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(dl) : "s"(0x000F000Fu), "v"(xl));
+                const bool any_live = __builtin_amdgcn_ballot_w64(dg != 0xFFFFFFFFu) != 0;
+                const bool any_last = __builtin_amdgcn_ballot_w64(dl != 0xFFFFFFFFu) != 0;
+                if (any_last) {
+                    if (lo_next == 0) lo_next = j;
+                    hi_next = j + 3;
+                }
+                if (!any_live && j + 3 > live_hi) break;
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t RR = splat16(i0 + r + 1);
-                uint32_t miss, out;
+                uint32_t miss, out, dead;
                 const uint32_t dr = u2(__builtin_elementwise_sub_sat(v2(rm[r]), v2(S2)));
                 const uint32_t dv = u2(__builtin_elementwise_sub_sat(v2(BI2), v2(RR)));
+                const uint32_t dd = u2(__builtin_elementwise_sub_sat(v2(rm[r]), v2(LIVE2)));
                 asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(miss) : "s"(0x000F000Fu), "v"(dr));
                 asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(out) : "s"(0x000F000Fu), "v"(dv));
+                asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(dead) : "s"(0x000F000Fu), "v"(dd));
                 hitr = u2(__builtin_elementwise_max(v2(hitr), v2(RR & ~miss & ~out)));
+                done |= dead | out;
             }
+            run += 2ull * R * (uint64_t)(jend - jlo + 1) * kWave;
+            // no live cell in the strip's last row, or every pair past a row
+            // without one: no later row holds a hit
+            if (lo_next == 0 || __builtin_amdgcn_ballot_w64(done != 0xFFFFFFFFu) == 0) break;
+            jlo = lo_next;
+            live_hi = hi_next;
+            jtop = jend;
         }
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
@@ -1368,7 +1417,6 @@ __global__ __launch_bounds__(64) void reverse_kernel(SeqSet S, ReverseSet T, Scr
                 T.start_cell[p[hh]] = ((uint32_t)si << 16) | (uint32_t)sj;
             }
         }
-        run += 2ull * (uint64_t)Rmax * (uint64_t)ncols * kWave;
     }
     // once, after the task loop (see rerun_kernel)
     if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);

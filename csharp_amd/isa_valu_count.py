@@ -2,7 +2,8 @@
 
 Compiles csrc/swg_kernels.hip for gfx950 to assembly (this build's own
 code), takes each kernel's largest basic block (the unrolled 4-column x
-R-row loop body) and divides its VALU instruction count by the cells that
+R-row loop body, with the block that closes the loop when the compiler
+splits it in two) and divides its VALU instruction count by the cells that
 body evaluates.  bench.py's roofline uses these figures as the kernels'
 operations per cell.
 
@@ -35,11 +36,23 @@ def count(asm_text):
         start = m.end()
         end = asm_text.index(".Lfunc_end", start)
         body = asm_text[start:end]
-        blocks = re.split(r"\n\.LBB[0-9_]+:", body)
-        best = 0
-        for b in blocks:
-            n = sum(1 for line in b.split("\n") if line.strip().startswith("v_"))
-            best = max(best, n)
+        parts = re.split(r"\n(\.LBB[0-9_]+):", body)
+        blocks = {parts[k]: parts[k + 1] for k in range(1, len(parts), 2)}
+
+        def valu(txt):
+            return sum(1 for line in txt.split("\n") if line.strip().startswith("v_"))
+
+        def branch_target(txt):
+            lines = [ln.strip() for ln in txt.split("\n") if ln.strip() and not ln.strip().startswith((";", "."))]
+            m2 = re.match(r"s_branch\s+(\.LBB[0-9_]+)", lines[-1]) if lines else None
+            return m2.group(1) if m2 else None
+
+        label = max(blocks, key=lambda lb: valu(blocks[lb]))
+        best = valu(blocks[label])
+        # a loop body the compiler closed with a second block that jumps back
+        nxt = branch_target(blocks[label])
+        if nxt in blocks and nxt != label and re.search(r"s_c?branch\S*\s+%s\b" % re.escape(label), blocks[nxt]):
+            best += valu(blocks[nxt])
         out[name] = round(best / cells, 3)
     return out
 

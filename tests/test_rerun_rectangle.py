@@ -9,7 +9,10 @@ reaches the first-maximum score.  This restates that reverse DP in Python and
 checks, against the oracle's literal aligner on the full pair, that the local
 DP of the rectangle alone (the oracle run on the two substrings) gives the
 same score, aligned length and identity/similarity -- including low
-complexity sequences, zero gap penalties (many ties) and protein.
+complexity sequences, zero gap penalties (many ties), extension above
+opening, and protein.  The GPU evaluates the reverse DP only on a band (cells
+with a predecessor whose value can still lie on an optimal path); the banded
+restatement here must find the same rectangle as the full reverse DP.
 """
 import numpy as np
 import pytest
@@ -47,6 +50,87 @@ def reverse_start(s1, s2, M, o, e, bi, jb, best):
     return bi - hitr + 1, wend - hitc + 1, wend
 
 
+PAD = -16384  # score of a pad row / column past the pair's prefix
+
+
+def reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R, live_min=None, stats=None):
+    """The same reverse DP evaluated on the GPU kernel's band: strips of R
+    rows, columns in 4-column groups from jlo (the first group with a live
+    cell, V' >= min(0, o - e), in the row above the strip) until a group with no live
+    cell past the last live column of the row above; cells outside are -inf;
+    stop after a strip whose last row has no live cell or once every row
+    seen so far includes one without a live cell.  Returns (si, sj)."""
+    live_min = min(0, o - e) if live_min is None else live_min
+    wend = min(jb + BLOCK - 1, len(s2))
+    anc = wend - jb + 1
+    r1 = s1[:bi][::-1]
+    r2 = s2[:wend][::-1]
+    NEG = -(1 << 40)
+    ncols = (wend + 3) // 4 * 4
+    evaluated = 0
+    top_v = [NEG] * (ncols + 1)   # row above the strip: V' and the vertical-gap state
+    top_g = [NEG] * (ncols + 1)
+    for c in range(anc):
+        top_v[c] = 0
+    jlo, jtop, live_hi = 1, BLOCK, BLOCK - 1
+    hitr = hitc = 0
+    seen_dead = False
+    for i0 in range(0, bi, R):
+        rows = range(i0 + 1, i0 + R + 1)
+        left_v = {i: NEG for i in rows}
+        left_h = {i: NEG for i in rows}
+        rmax = {i: NEG for i in rows}
+        new_v = [NEG] * (ncols + 1)
+        new_g = [NEG] * (ncols + 1)
+        diag_top = top_v[0] if (i0 == 0 and jlo == 1) else NEG
+        lo_next = hi_next = 0
+        j = jlo
+        while j <= ncols:
+            group_live = last_live = False
+            for jj in range(j, j + 4):
+                up_v = top_v[jj] if jj <= jtop else NEG
+                up_g = top_g[jj] if jj <= jtop else NEG
+                diag = diag_top
+                diag_top = up_v
+                for i in rows:
+                    s = int(M[r1[i - 1]][r2[jj - 1]]) if (i <= bi and jj <= wend) else PAD
+                    g = max(up_g - e, up_v - o)
+                    h = max(left_h[i] - e, left_v[i] - o)
+                    v = max(diag + s if diag > NEG // 2 else NEG, g, h)
+                    v = max(v, NEG)
+                    assert v <= best
+                    if v == best and i <= bi and jj <= wend:
+                        hitr, hitc = max(hitr, i), max(hitc, jj)
+                    rmax[i] = max(rmax[i], v)
+                    group_live |= v >= live_min
+                    if i == rows[-1]:
+                        last_live |= v >= live_min
+                    evaluated += 1
+                    diag = left_v[i]
+                    left_v[i], left_h[i] = v, h
+                    up_v, up_g = v, g
+                new_v[jj], new_g[jj] = up_v, up_g
+            jend = j + 3
+            if last_live:
+                lo_next = lo_next or j
+                hi_next = j + 3
+            if not group_live and j + 3 > live_hi:
+                break
+            j += 4
+        for i in rows:
+            seen_dead |= rmax[i] < live_min or i > bi
+        if lo_next == 0 or seen_dead:
+            break
+        jlo, live_hi, jtop = lo_next, hi_next, jend
+        top_v, top_g = new_v, new_g
+    if stats is not None:
+        stats["evaluated"] = stats.get("evaluated", 0) + evaluated
+        stats["prefix"] = stats.get("prefix", 0) + bi * wend
+    if hitr == 0:
+        return 1, 1
+    return bi - hitr + 1, wend - hitc + 1
+
+
 def check_pair(s1, s2, M, o, e):
     full = O.sw_literal(s1, s2, M, o, e)
     if len(s1) == 0 or len(s2) == 0:
@@ -55,6 +139,8 @@ def check_pair(s1, s2, M, o, e):
     bi, bj = full["best_row"], full["best_col"]
     jb = (bj - 1) // BLOCK * BLOCK + 1
     si, sj, wend = reverse_start(s1, s2, M, o, e, bi, jb, best)
+    for R in (2, 16):
+        assert reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R) == (si, sj), (R, s1, s2, o, e)
     assert 1 <= si <= bi and 1 <= sj <= bj <= wend
     sub = O.sw_literal(s1[si - 1:bi], s2[sj - 1:wend], M, o, e)
     assert int(sub["score"]) == best
@@ -75,7 +161,7 @@ def mutate(rng, s, sub, indel, alphabet):
     return bytes(out)
 
 
-@pytest.mark.parametrize("o,e", [(16, 4), (5, 2), (1, 1), (0, 0)])
+@pytest.mark.parametrize("o,e", [(16, 4), (5, 2), (1, 1), (0, 0), (2, 5)])
 def test_rectangle_dna(ednafull, o, e):
     rng = np.random.default_rng(11 + o)
     A = b"ACGT"
@@ -105,3 +191,51 @@ def test_rectangle_protein(blosum62):
         s1 = bytes(A[k] for k in rng.integers(20, size=n1))
         s2 = mutate(rng, s1, 0.3, 0.1, A) if rng.random() < 0.5 else bytes(A[k] for k in rng.integers(20, size=n2))
         check_pair(s1, s2 or b"W", blosum62, 10, 1)
+
+
+def _long_pairs(rng, n, lo, hi):
+    A = b"ACGT"
+    out = []
+    for _ in range(n):
+        n1, n2 = rng.integers(lo, hi, size=2)
+        anc = bytes(A[k] for k in rng.integers(4, size=max(n1, n2)))
+        s1 = mutate(rng, anc, 0.15, 0.03, A)[:n1]
+        s2 = mutate(rng, anc, 0.15, 0.03, A)[:n2] if rng.random() < 0.3 else bytes(A[k] for k in rng.integers(4, size=n2))
+        out.append((s1, s2))
+    return out
+
+
+def _rect(s1, s2, M, o, e):
+    full = O.sw_literal(s1, s2, M, o, e)
+    best = int(full["score"])
+    bi, bj = full["best_row"], full["best_col"]
+    return best, bi, (bj - 1) // BLOCK * BLOCK + 1
+
+
+def test_band_prunes_and_keeps_rectangle(ednafull):
+    """Longer pairs (120..260 residues, as the block workloads' scale with
+    related and unrelated pairs): the band evaluates a fraction of the
+    prefix rectangle and still finds the unbanded DP's rectangle."""
+    rng = np.random.default_rng(23)
+    stats = {}
+    for s1, s2 in _long_pairs(rng, 12, 120, 260):
+        best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
+        if best == 0:
+            continue
+        si, sj, _ = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best)
+        assert reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 16, stats=stats) == (si, sj)
+    assert stats["evaluated"] < 0.8 * stats["prefix"], stats
+
+
+def test_band_threshold_is_not_slack(ednafull):
+    """The comparison catches a band that is too narrow: with liveness
+    V' >= e instead of V' >= min(0, o - e) the band loses hits."""
+    rng = np.random.default_rng(29)
+    lost = 0
+    for s1, s2 in _long_pairs(rng, 40, 30, 90):
+        best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
+        if best == 0:
+            continue
+        ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best)[:2]
+        lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, live_min=4) != ref
+    assert lost > 0
