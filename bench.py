@@ -30,6 +30,7 @@ PEAK_OPS = {"int32": 256 * 4 * 16 * 2.4e9, "int16x2": 2 * 256 * 4 * 16 * 2.4e9}
 # reached instead); the global variant carries one more word (lead
 # and trailing gap run), 30.  Mode = the arithmetic the kernel runs in.
 ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
+           "rect64": (24, "int32"),
            "one_pass": (24, "int32"), "lpp16": (24, "int32"), "global": (30, "int32")}
 # VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
 # unrolled), counted from this build's gfx950 assembly by

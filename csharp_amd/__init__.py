@@ -36,7 +36,7 @@ DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
 OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING = 1, 2
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
-                7: "global", 8: "reverse"}
+                7: "global", 8: "reverse", 9: "rect64"}
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (

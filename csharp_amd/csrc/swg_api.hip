@@ -175,7 +175,7 @@ struct swg_ctx {
 
     // kernel-timing mode: direct launches with an event after each kernel
     bool kernel_timing = false;
-    static constexpr int kMaxKernels = 8;
+    static constexpr int kMaxKernels = 10;
     hipEvent_t kt_ev[kMaxKernels] = {};
     int32_t kt_ids[kMaxKernels] = {};
     int kt_n = 0;
@@ -202,8 +202,9 @@ void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
 }
 
 // Resident workgroups of the variant's kernels on the lane's share of the
-// CUs: n1 first kernel, n2 re-run, n3 reverse pass (two-pass variant).
-int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2, int* n3 = nullptr)
+// CUs: n1 first kernel, n2 re-run, n3 reverse pass and n4 big-rectangle
+// re-run (two-pass variant).
+int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2, int* n3 = nullptr, int* n4 = nullptr)
 {
     int s1, s2;
     variant_strips(c, v, &s1, &s2);
@@ -213,10 +214,12 @@ int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2, int* n3
                          : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
     int r = v == kVarI16 ? reverse_blocks_per_cu(c->ncode) : 0;
-    if (a <= 0 || (v == kVarI16 && (b <= 0 || r <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
+    int q = v == kVarI16 ? rect64_blocks_per_cu(c->ncode) : 0;
+    if (a <= 0 || (v == kVarI16 && (b <= 0 || r <= 0 || q <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
     *n2 = std::max(1, b * c->num_cu / ln.share);
     if (n3) *n3 = std::max(1, r * c->num_cu / ln.share);
+    if (n4) *n4 = std::max(1, q * c->num_cu / ln.share);
     return SWG_OK;
 }
 
@@ -346,16 +349,17 @@ void append_key(std::vector<unsigned char>& k, const T& v)
 int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld,
                 int32_t* dev_stats, Plan& plan)
 {
-    int s1, s2, n1, n2, n3;
+    int s1, s2, n1, n2, n3, n4;
     variant_strips(c, v, &s1, &s2);
-    int rc = slots_for(c, ln, v, &n1, &n2, &n3);
+    int rc = slots_for(c, ln, v, &n1, &n2, &n3, &n4);
     if (rc) return rc;
-    int want = std::max(std::max(n1, n2), v == kVarI16 ? n3 : 0);
+    int want = std::max(std::max(n1, n2), v == kVarI16 ? std::max(n3, n4) : 0);
     rc = ensure_scratch(c, ln, &want);
     if (rc) return rc;
     n1 = std::min(n1, want);
     n2 = std::min(n2, want);
     n3 = std::min(n3, want);
+    n4 = std::min(n4, want);
 
     const hipStream_t st = ln.stream;
     BlockDesc desc[2] = {};
@@ -365,13 +369,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)))
             return rc;
         const size_t cells = (size_t)(g.r1 - g.r0 + 1) * (size_t)(g.c1 - g.c0 + 1);
-        if (cells + 2 * (size_t)kWave >= (size_t(1) << 31))
+        // slots: every pair, the paddings of the flagged and big lists and the last tile's
+        if (cells + 3 * (size_t)kWave >= (size_t(1) << 31))
             return fail(SWG_E_CAPACITY, "block too large for the two-pass schedule (2^31 pairs)");
         SWG_HIP(B.best_cell.reserve(std::max<size_t>(cells, 1)));
         SWG_HIP(B.best_score.reserve(std::max<size_t>(cells, 1)));
         SWG_HIP(B.start_cell.reserve(std::max<size_t>(cells, 1)));
-        SWG_HIP(B.sorted.reserve(cells + 2 * (size_t)kWave));
-        SWG_HIP(B.sorted2.reserve(cells + 2 * (size_t)kWave));
+        SWG_HIP(B.sorted.reserve(cells + 3 * (size_t)kWave));
+        SWG_HIP(B.sorted2.reserve(cells + 3 * (size_t)kWave));
         SWG_HIP(B.sortbuf.reserve(kMetaWords + 3 * kNumBuckets));
         SWG_HIP(B.sortbuf2.reserve(kMetaWords + 3 * kNumBuckets));
         // bucket size: the block's longest sequences over kBucketDim buckets
@@ -397,7 +402,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     desc[0].r0 = g.r0; desc[0].c0 = g.c0; desc[0].ld = ld;
     desc[0].out = dev_out; desc[0].stats = dev_stats;
     SWG_HIP(B.desc.reserve(2));
-    SWG_HIP(B.queue.reserve(64));
+    SWG_HIP(B.queue.reserve(80));
     SWG_HIP(hipMemcpyAsync(B.desc.p, desc, sizeof(desc), hipMemcpyHostToDevice, st));
 
     const SeqSet S = make_seqset(c);
@@ -410,7 +415,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     ScoreTaskSet ST{};
     SortSet P{}, P2{};
     ReverseSet RV{};
-    RerunSet RT{}, RC{};
+    RerunSet RT{}, RC{}, RB{};
     const int sort_grid = std::max(1, c->num_cu * 2 / ln.share);
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
@@ -433,6 +438,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         RC.start_cell = B.start_cell.p;
         RT.queue = B.queue.p + 16;
         RC.queue = B.queue.p + 32;
+        RB = RC;
+        RB.queue = B.queue.p + 64;
     }
 
     // the block's kernels in stream order; with `marks`, an event is recorded
@@ -445,7 +452,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             c->kt_ids[nk] = kernel_id;
             return hipEventRecord(c->kt_ev[nk++], st);
         };
-        hipError_t e = launch_fill_i32(B.queue.p, 0, 64, st);
+        hipError_t e = launch_fill_i32(B.queue.p, 0, 80, st);
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
@@ -472,6 +479,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_TRACK);
             if (e == hipSuccess) e = launch_rerun(S, RC, W, s2, false, n2, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_CAPTURE);
+            if (e == hipSuccess) e = launch_rect64(S, RB, W, n4, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_RECT64);
         }
         if (mark) c->kt_n = nk;
         return e;
@@ -485,7 +494,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         std::vector<unsigned char> key;
         append_key(key, S); append_key(key, T); append_key(key, W); append_key(key, ST); append_key(key, P);
         append_key(key, RT); append_key(key, RC); append_key(key, sort_grid); append_key(key, P2);
-        append_key(key, RV); append_key(key, n3);
+        append_key(key, RV); append_key(key, n3); append_key(key, RB); append_key(key, n4);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         if (!B.graph[v] || key != B.graph_key[v]) {
             if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
@@ -1016,6 +1025,10 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
         case SWG_KERNEL_RERUN_CAPTURE:
             t.cells = (double)meta64b[kMetaRectCells];
             t.cells_run = (double)meta64b[kMetaCaptureRun];
+            break;
+        case SWG_KERNEL_RECT64:
+            t.cells = (double)meta64b[kMetaBigCells];
+            t.cells_run = (double)meta64b[kMetaBigRun];
             break;
         default: break;
         }

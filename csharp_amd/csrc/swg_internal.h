@@ -76,10 +76,15 @@ constexpr int kFirstMaxBlock = 4;
 constexpr uint32_t kCellTrack = 0x80000000u;
 
 // Sort of the block's pairs by first-maximum cell for the re-run: buckets of
-// wr x wc cells on a kBucketDim x kBucketDim grid, plus one list (the last
-// bucket) of flagged pairs, padded to whole 64-lane tiles.
+// wr x wc cells on a kBucketDim x kBucketDim grid, plus two lists, each
+// padded to whole 64-lane tiles: flagged pairs (the last bucket) and, in the
+// sort by re-run rectangle, rectangles of more than kBigRect cells (the one
+// before it), which one wave per pair re-runs.
 constexpr int kBucketDim = 128;
-constexpr int kNumBuckets = kBucketDim * kBucketDim + 1;
+constexpr int kNumBuckets = kBucketDim * kBucketDim + 2;
+constexpr int kFlaggedBucket = kNumBuckets - 1;
+constexpr int kBigBucket = kNumBuckets - 2;
+constexpr int64_t kBigRect = 65536;
 
 // meta words (int32 index; the 64-bit counters at meta64[k] = meta + 2k)
 constexpr int kMetaFlaggedTiles = 0;   // tiles of the flagged list (slots first)
@@ -89,6 +94,9 @@ constexpr int kMetaFlaggedCells = 2;   // meta64: cells of the flagged pairs
 constexpr int kMetaCaptureRun = 3;     // meta64: cells evaluated by the capture re-run
 constexpr int kMetaFlaggedRun = 4;     // meta64: cells evaluated by the flagged re-run
 constexpr int kMetaSlots = 10;         // slots in use (flagged list padded, then the buckets)
+constexpr int kMetaBigTiles = 11;      // tiles of the flagged and big lists (the buckets' first tile)
+constexpr int kMetaBigCells = 6;       // meta64: cells of the big rectangles
+constexpr int kMetaBigRun = 7;         // meta64: cells evaluated by the big-rectangle re-run
 constexpr int kMetaWords = 64;         // zeroed with the bucket counts before every sort
 
 struct SortSet {
@@ -118,10 +126,12 @@ struct ReverseSet {
     int32_t* queue;
 };
 
-// Re-run: tile t of `sorted` (64 pairs, one per lane; tiles [meta[0],
-// meta[1]) capture the statistics at the known first-maximum cell, tiles
-// [0, meta[0]) track the first maximum).  start_cell (optional): the capture
-// re-run covers rows si..bi and columns sj.. only.
+// Re-run: tile t of `sorted` (64 pairs, one per lane; tiles
+// [meta[kMetaBigTiles], meta[kMetaTiles]) capture the statistics at the known
+// first-maximum cell, tiles [0, meta[kMetaFlaggedTiles]) track the first
+// maximum).  start_cell (optional): the capture re-run covers rows si..bi and
+// columns sj.. only.  The big-rectangle re-run takes the pairs of tiles
+// [meta[kMetaFlaggedTiles], meta[kMetaBigTiles]), one per wave.
 struct RerunSet {
     const uint32_t* sorted;
     const uint32_t* best_cell;
@@ -152,6 +162,9 @@ hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
 hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, int strip_rows, bool track,
                         int nslots, hipStream_t stream);
 int rerun_blocks_per_cu(int strip_rows, int ncode);
+// Re-run of the big rectangles, one wave per pair.
+hipError_t launch_rect64(const SeqSet& S, const RerunSet& T, const Scratch& W, int nslots, hipStream_t stream);
+int rect64_blocks_per_cu(int ncode);
 // Reverse pass (strip_rows 16).
 hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t stream);
 int reverse_blocks_per_cu(int ncode);

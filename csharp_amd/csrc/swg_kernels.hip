@@ -801,7 +801,7 @@ __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1
     if (x & kCellTrack) {
         *flagged = true;
         *cells = (uint64_t)((x >> 16) & 0x7FFFu) * (uint64_t)Lb;
-        return kNumBuckets - 1;
+        return kFlaggedBucket;
     }
     *flagged = false;
     const int bi = (int)((x >> 16) & 0x7FFFu), jb = (int)(x & 0xFFFFu);
@@ -814,14 +814,17 @@ __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1
     }
     *cells = (uint64_t)h * (uint64_t)w;
     const bool fine = P.start_cell != nullptr;
+    if (fine && (int64_t)*cells > kBigRect) return kBigBucket;
     return side_bucket(h, D1.wr, fine) * kBucketDim + side_bucket(w, D1.wc, fine);
 }
 
-// Slot order: rank 0 = the flagged list, then the serpentine.
+// Slot order: rank 0 = the flagged list, rank 1 = the big rectangles, then
+// the serpentine.
 __device__ __forceinline__ int bucket_at_rank(int q)
 {
-    if (q == 0) return kNumBuckets - 1;
-    const int rr = q - 1, band = rr / kBucketDim, pos = rr - band * kBucketDim;
+    if (q == 0) return kFlaggedBucket;
+    if (q == 1) return kBigBucket;
+    const int rr = q - 2, band = rr / kBucketDim, pos = rr - band * kBucketDim;
     const int qi = kBucketDim - 1 - band;
     const int qj = (band & 1) ? pos : kBucketDim - 1 - pos;
     return qi * kBucketDim + qj;
@@ -861,7 +864,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
     const bool lower = P.desc[0].mirror != 0;
     for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads) hist[k] = 0;
     __syncthreads();
-    uint64_t rect = 0, flagged = 0;
+    uint64_t rect = 0, flagged = 0, big = 0;
     for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
         const int64_t n = row_pairs(D1, lower, row);
         for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
@@ -869,7 +872,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
             bool fl;
             const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
             atomicAdd(&hist[k], 1);
-            if (fl) flagged += cells; else rect += cells;
+            if (fl) flagged += cells;
+            else if (k == kBigBucket) big += cells;
+            else rect += cells;
         }
     }
     __syncthreads();
@@ -877,18 +882,21 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
         if (hist[k]) atomicAdd(&P.count[k], hist[k]);
     rect = wave_sum_u64(rect);
     flagged = wave_sum_u64(flagged);
+    big = wave_sum_u64(big);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         unsigned long long* m = reinterpret_cast<unsigned long long*>(P.meta);
         atomicAdd(&m[kMetaRectCells], (unsigned long long)rect);
         atomicAdd(&m[kMetaFlaggedCells], (unsigned long long)flagged);
+        atomicAdd(&m[kMetaBigCells], (unsigned long long)big);
     }
 }
 
-// Slots of rank q: the flagged list is padded to whole tiles, the buckets are not.
+// Slots of rank q: the flagged and big lists are padded to whole tiles, the
+// buckets are not.
 __device__ __forceinline__ int rank_slots(const SortSet& P, int q)
 {
     const int n = P.count[bucket_at_rank(q)];
-    return q == 0 ? (n + kWave - 1) / kWave * kWave : n;
+    return q <= 1 ? (n + kWave - 1) / kWave * kWave : n;
 }
 
 // One wave: exclusive scan of the slot counts in rank order.
@@ -913,7 +921,11 @@ __global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
         P.cursor[k] = 0;
         slot += rank_slots(P, q);
     }
-    if (lane == 0) P.meta[kMetaFlaggedTiles] = (P.count[kNumBuckets - 1] + kWave - 1) / kWave;
+    if (lane == 0) {
+        const int tf = (P.count[kFlaggedBucket] + kWave - 1) / kWave;
+        P.meta[kMetaFlaggedTiles] = tf;
+        P.meta[kMetaBigTiles] = tf + (P.count[kBigBucket] + kWave - 1) / kWave;
+    }
     if (lane == kWave - 1) {
         P.meta[kMetaTiles] = (incl + kWave - 1) / kWave;
         P.meta[kMetaSlots] = incl;
@@ -927,11 +939,14 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(SortSet P)
     __shared__ int hist[kNumBuckets];
     const BlockDesc D1 = P.desc[1];
     const bool lower = P.desc[0].mirror != 0;
-    // idle lanes: the flagged list's last tile and the last tile
+    // idle lanes: the last tiles of the flagged and big lists, and the last tile
     if (blockIdx.x == 0 && threadIdx.x < kWave) {
-        const int nf = P.count[kNumBuckets - 1];
+        const int nf = P.count[kFlaggedBucket];
         const int endf = (nf + kWave - 1) / kWave * kWave;
         if (nf + (int)threadIdx.x < endf) P.sorted[nf + threadIdx.x] = ~0u;
+        const int nb = P.base[kBigBucket] + P.count[kBigBucket];
+        const int endb = P.base[kBigBucket] + (P.count[kBigBucket] + kWave - 1) / kWave * kWave;
+        if (nb + (int)threadIdx.x < endb) P.sorted[nb + threadIdx.x] = ~0u;
         const int n = P.meta[kMetaSlots];
         const int end = (n + kWave - 1) / kWave * kWave;
         if (n + (int)threadIdx.x < end) P.sorted[n + threadIdx.x] = ~0u;
@@ -994,7 +1009,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     const int64_t r0 = D[1].r0, c0 = D[1].c0;
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
-    const int t_lo = TRACK ? 0 : T.meta[kMetaFlaggedTiles];
+    const int t_lo = TRACK ? 0 : T.meta[kMetaBigTiles];
     const int t_hi = TRACK ? T.meta[kMetaFlaggedTiles] : T.meta[kMetaTiles];
     // cells evaluated, added to meta once after the task loop: a second
     // lane-0 branch inside the loop lets the compiler merge it with the
@@ -1161,6 +1176,170 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     if (lane == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + (TRACK ? kMetaFlaggedRun : kMetaCaptureRun),
                   (unsigned long long)run);
+}
+
+// ---------------------------------------------------------------------------
+// Re-run of the big rectangles (more than kBigRect cells), one wave per pair.
+// One lane per pair would leave a single lane with millions of cells while the
+// rest of the GPU idles.  Lane k owns R consecutive rows of a stripe of 64*R
+// rows of the rectangle and runs one column behind lane k-1 (anti-diagonal
+// wavefront, as lpp16_kernel with the whole wave on one pair): the row above
+// its first row (V-o, G, T, TG) and the column residue come from lane k-1
+// with a DPP wave shift, lane 0 reads the previous stripe's bottom row from
+// the boundary buffer and lane 63 writes this stripe's.  Pre-columns and pad
+// rows/columns as in lpp16_kernel.
+// The local DP of the rectangle [si..bi] x [sj..wend] is run with its first
+// strict maximum tracked over all of it: the rectangle's values are at most
+// the full DP's and equal to them along the reference's traceback path, so
+// every cell before (bi, bj) in row-major order stays below S and the first
+// maximum is (bi, bj), with the reference's statistics.
+constexpr int kRectLanes = 64;
+
+__device__ __forceinline__ int wave_shr1(int old, int src)
+{
+    return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratch W)
+{
+    constexpr int H = kRectLanes * R;  // stripe height
+    constexpr int PS = H / 2 + 1;      // int4 per residue code in the stripe profile
+    extern __shared__ int4 dyn_lds[];
+    int4* lprof = dyn_lds;                                           // ncode * PS int4
+    int2* lmerge = reinterpret_cast<int2*>(dyn_lds + S.ncode * PS);  // H (best, T)
+
+    const int lane = threadIdx.x;
+    int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;  // [column]
+    const BlockDesc* D = T.desc;
+    const int64_t r0 = D[1].r0, c0 = D[1].c0;
+    const uint32_t ld1 = (uint32_t)D[1].ld;
+    const int o = S.gap_open, e = S.gap_ext;
+    const int pad = S.ncode - 1;
+    const int64_t s_lo = (int64_t)T.meta[kMetaFlaggedTiles] * kWave;
+    const int64_t s_hi = (int64_t)T.meta[kMetaBigTiles] * kWave;
+    uint64_t run = 0;  // cells evaluated, added to meta once after the task loop
+
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(T.queue, 1);
+        const int64_t slot = s_lo + uni(v);
+        if (slot >= s_hi) break;
+        const uint32_t p = (uint32_t)uni((int)T.sorted[slot]);
+        if (p == ~0u) continue;  // the list's padding
+
+        const int a = (int)(r0 + p / ld1), b = (int)(c0 + p % ld1);
+        const uint32_t cell = (uint32_t)uni((int)T.best_cell[p]), st = (uint32_t)uni((int)T.start_cell[p]);
+        const int bi = (int)((cell >> 16) & 0x7FFFu), jb = (int)(cell & 0xFFFFu);
+        const int si = (int)(st >> 16), sj = (int)(st & 0xFFFFu);
+        const int wend = min(jb + kFirstMaxBlock - 1, uni(S.len[b]));
+        const int h = bi - si + 1, w = wend - sj + 1;
+        const uint8_t* s1 = S.res + S.off[a] + si - 1;
+        const int64_t c2b = S.off[b] + sj - 1;
+        const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res + (c2b & ~int64_t(3)));
+        const int sh = (int)(c2b & 3);
+        const int nsteps = (w + (kRectLanes - 1) + 3) & ~3;  // lane 63 reaches column w
+
+        // row 0 of the rectangle (V = 0, G = -inf, T = 0), also past the last
+        // column so that lane 0 always reads defined values
+        for (int j = 1 + lane; j <= nsteps + 8; j += kWave) bnd[j] = make_int4(-o, kNeg, 0, 0);
+        __syncthreads();  // other lanes read these columns
+
+        int best = INT_MIN, bestT = 0;
+        for (int i0 = 0; i0 < h; i0 += H) {
+            __syncthreads();
+            {
+                int2* lp = reinterpret_cast<int2*>(lprof);
+                for (int idx = lane; idx < S.ncode * H; idx += kWave) {
+                    const int c2 = idx / H, r = idx % H;
+                    const int row = i0 + r;
+                    const int c1 = row < h ? (int)s1[row] : pad;
+                    lp[c2 * (H + 2) + r] = S.prof[c1 * S.ncode + c2];
+                }
+            }
+            __syncthreads();
+
+            int tr[R], Tr[R], Hr[R], THr[R], rb[R], rT[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
+            }
+            int ot = -o, oG = kNeg, oT = 0, oTG = 0;  // this lane's last-row output of the previous step
+            int pt = -o, pT = 0;                      // up-values received at the previous step (diagonal)
+            int cprev = pad;
+
+            uint32_t wlo = s2w[0], whi = s2w[1];
+            int4 nb0 = bnd[1], nb1 = bnd[2], nb2 = bnd[3], nb3 = bnd[4];
+            for (int t = 0; t < nsteps; t += 4) {
+                const uint32_t cw = __builtin_amdgcn_alignbyte(whi, wlo, sh);  // columns t+1 .. t+4
+                wlo = whi;
+                whi = s2w[(t >> 2) + 2];
+                int4 cb[4] = {nb0, nb1, nb2, nb3};
+                nb0 = bnd[t + 5]; nb1 = bnd[t + 6]; nb2 = bnd[t + 7]; nb3 = bnd[t + 8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int tt = t + q;
+                    int cl = (int)((cw >> (8 * q)) & 0xFFu);
+                    cl = tt + 1 > w ? pad : cl;
+                    const int c2 = wave_shr1(cl, cprev);
+                    cprev = c2;
+                    int tu = wave_shr1(cb[q].x, ot), Gu = wave_shr1(cb[q].y, oG);
+                    int Tu = wave_shr1(cb[q].z, oT), TGu = wave_shr1(cb[q].w, oTG);
+                    int dt = pt, dT = pT;
+                    pt = tu; pT = Tu;
+                    const int4* pr = lprof + c2 * PS + lane * (R / 2);
+#pragma unroll
+                    for (int r2 = 0; r2 < R / 2; ++r2) {
+                        const int4 pp = pr[r2];
+#pragma unroll
+                        for (int hh = 0; hh < 2; ++hh) {
+                            const int r = 2 * r2 + hh;
+                            const int sc = hh ? pp.z : pp.x;
+                            const int inc = hh ? pp.w : pp.y;
+                            const int g1 = Gu - e;
+                            const bool cg = g1 > tu;
+                            const int G = cg ? g1 : tu;
+                            const int TG = (cg ? TGu : Tu) + 0x10000;
+                            const int h1 = Hr[r] - e;
+                            const bool ch = h1 > tr[r];
+                            Hr[r] = ch ? h1 : tr[r];
+                            THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                            const int f = dt + sc;
+                            int V = max(max(f, G), Hr[r]);
+                            V = max(V, 0);
+                            const int Td = dT + inc;
+                            int x = (V == G) ? TG : THr[r];
+                            x = (V == f) ? Td : x;
+                            const int Tn = (V == 0) ? 0 : x;
+                            const bool cb2 = V > rb[r];
+                            rb[r] = cb2 ? V : rb[r];
+                            rT[r] = cb2 ? Tn : rT[r];
+                            const int tn = V - o;
+                            dt = tr[r]; dT = Tr[r];
+                            tr[r] = tn; Tr[r] = Tn;
+                            tu = tn; Gu = G; Tu = Tn; TGu = TG;
+                        }
+                    }
+                    ot = tu; oG = Gu; oT = Tu; oTG = TGu;
+                    const int j = tt - lane + 1;
+                    if (lane == kRectLanes - 1 && j >= 1) bnd[j] = make_int4(ot, oG, oT, oTG);
+                }
+            }
+            // first strict maximum in row-major order over the stripe's rows
+#pragma unroll
+            for (int r = 0; r < R; ++r) lmerge[lane * R + r] = make_int2(rb[r], rT[r]);
+            __syncthreads();
+            if (lane == 0) {
+                for (int r = 0; r < H && i0 + r < h; ++r) {
+                    const int2 m = lmerge[r];
+                    if (m.x > best) { best = m.x; bestT = m.y; }
+                }
+            }
+            run += (uint64_t)H * (uint64_t)nsteps;
+        }
+        if (lane == 0) write_distance(S, D[0], a, b, best, bestT);
+    }
+    if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaBigRun, (unsigned long long)run);
 }
 
 // ---------------------------------------------------------------------------
@@ -1558,6 +1737,34 @@ hipError_t launch_rerun(const SeqSet& S, const RerunSet& T, const Scratch& W, in
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Rows per lane of the big-rectangle re-run: 8 when the stripe profile of
+// 512 rows fits in 64 KiB of LDS with the merge rows, else 4.
+static int rect64_rows(int ncode) { return ncode <= 12 ? 8 : 4; }
+
+static size_t rect64_lds(int ncode, int R)
+{
+    return (size_t)ncode * (size_t)(kRectLanes * R / 2 + 1) * sizeof(int4) + (size_t)kRectLanes * R * sizeof(int2);
+}
+
+hipError_t launch_rect64(const SeqSet& S, const RerunSet& T, const Scratch& W, int nslots, hipStream_t st)
+{
+    const int R = rect64_rows(S.ncode);
+    const size_t lds = rect64_lds(S.ncode, R);
+    if (R == 8) hipLaunchKernelGGL((rect64_kernel<8>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+    else hipLaunchKernelGGL((rect64_kernel<4>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+    return hipGetLastError();
+}
+
+int rect64_blocks_per_cu(int ncode)
+{
+    const int R = rect64_rows(ncode);
+    const size_t lds = rect64_lds(ncode, R);
+    int nb = 0;
+    const hipError_t err = R == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rect64_kernel<8>, kWave, lds)
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rect64_kernel<4>, kWave, lds);
+    return err == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t st)

@@ -161,6 +161,7 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
 #define SWG_KERNEL_LPP16         6  /* 16 lanes per pair */
 #define SWG_KERNEL_GLOBAL        7  /* global (NW) alignment */
 #define SWG_KERNEL_REVERSE       8  /* reverse score pass: where the first-maximum alignments can start */
+#define SWG_KERNEL_RECT64        9  /* re-run of the big rectangles, one wave per pair */
 typedef struct swg_kernel_time {
     int32_t kernel;     /* SWG_KERNEL_* */
     int32_t reserved;
@@ -168,7 +169,8 @@ typedef struct swg_kernel_time {
     double  cells;      /* DP cells the kernel is charged with: the block's cells (one-pass, score
                            pass); the rectangles up to each pair's first-maximum block (reverse
                            pass); the rectangles from each pair's earliest optimal start to that
-                           block (capture re-run); the flagged pairs' cells (tracking re-run);
+                           block (capture re-run: those of at most 65536 cells; big-rectangle
+                           re-run: the others); the flagged pairs' cells (tracking re-run);
                            0 for the sorts */
     double  cells_run;  /* DP cells the kernel evaluated, idle lanes, pad rows and columns included */
 } swg_kernel_time;

@@ -207,6 +207,58 @@ def test_long_sequences(ednafull, schedule):
     al.close()
 
 
+def _mutated_family(rng, alphabet, n, lo, hi, sub, indel):
+    """n copies of one random ancestor with substitutions and indels, plus
+    as many unrelated sequences: related pairs align over most of their
+    length, so their re-run rectangles exceed the big-rectangle threshold."""
+    anc = rng.choice(list(alphabet), size=hi)
+    out = []
+    for _ in range(n):
+        s = []
+        for ch in anc[: rng.integers(lo, hi + 1)]:
+            r = rng.random()
+            if r < indel / 2:
+                continue
+            if r < indel:
+                s.append(rng.choice(list(alphabet)))
+            s.append(rng.choice(list(alphabet)) if rng.random() < sub else ch)
+        out.append(bytes(s))
+    for _ in range(n):
+        out.append(bytes(rng.choice(list(alphabet), size=int(rng.integers(lo, hi + 1)))))
+    return ca.SequenceSet.from_strings(out)
+
+
+@pytest.mark.parametrize("kind", ["dna", "protein"])
+def test_big_rectangles_one_wave_per_pair(ednafull, blosum62, kind):
+    """Related pairs whose re-run rectangles exceed 65536 cells go to the
+    one-wave-per-pair kernel (8 rows per lane for DNA, 4 for the protein
+    alphabet); their distances and statistics must match the oracle."""
+    say("big rectangles %s" % kind)
+    rng = np.random.default_rng(31 if kind == "dna" else 37)
+    if kind == "dna":
+        s = _mutated_family(rng, b"ACGT", 24, 600, 1300, 0.10, 0.03)
+        M, o, e, nucleic = ednafull, 16, 4, True
+    else:
+        s = _mutated_family(rng, b"ACDEFGHIKLMNPQRSTVWY", 24, 300, 700, 0.25, 0.03)
+        M, o, e, nucleic = blosum62, 10, 1, False
+    al = ca.Aligner(M, o, e, schedule=ca.SCHED_PERSISTENT_I16,
+                    alignment_type=ca.ALIGN_NUCLEIC if nucleic else ca.ALIGN_PROTEIN)
+    al.set_schedule(ca.SCHED_PERSISTENT_I16, 16)
+    al.set_sequences(s)
+    n = len(s)
+    al.set_kernel_timing(True)
+    got, gst, _ = al.compute_block(0, 23, 0, n - 1, False, stats=True)
+    kt = {k[0]: k for k in al.kernel_times()}
+    al.set_kernel_timing(False)
+    want, wst = oracle_block(s, 0, 23, 0, n - 1, False, M, o, e, nucleic=nucleic)
+    assert_same(got, want, gst, wst, "big rectangles")
+    assert kt["rect64"][2] > 0 and kt["rect64"][3] >= kt["rect64"][2]
+    # the graph path gives the same block
+    got2, gst2, _ = al.compute_block(0, 23, 0, n - 1, False, stats=True)
+    assert np.array_equal(got2, got) and np.array_equal(gst2, gst)
+    al.close()
+
+
 def test_repeated_blocks_reuse_graph(dna_set, ednafull):
     say("repeat")
     al = ca.Aligner(ednafull, 16, 4)
@@ -365,7 +417,7 @@ def test_swms_restart_reproduces_block_files(tmp_path, ednafull):
 
 
 @pytest.mark.parametrize("schedule,names", [
-    (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "reverse", "sort", "rerun_track", "rerun_capture"]),
+    (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "reverse", "sort", "rerun_track", "rerun_capture", "rect64"]),
     (ca.SCHED_PERSISTENT_I32, ["one_pass"]), (ca.SCHED_PLAIN, ["one_pass"])])
 def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
     say("kernel timing schedule=%d" % schedule)
