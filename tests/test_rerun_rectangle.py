@@ -19,10 +19,12 @@ import pytest
 
 from oracle import oracle as O
 
-BLOCK = 4  # columns per first-maximum block of the score pass
+# columns per first-maximum block of the score pass (16); the rectangle and
+# band rules hold for any block width, 4 is checked as well
+BLOCKS = (4, 16)
 
 
-def reverse_start(s1, s2, M, o, e, bi, jb, best):
+def reverse_start(s1, s2, M, o, e, bi, jb, best, BLOCK):
     """Reverse DP anchored on the block (no 0 floor); returns (si, sj)."""
     wend = min(jb + BLOCK - 1, len(s2))
     anc = wend - jb + 1
@@ -53,7 +55,7 @@ def reverse_start(s1, s2, M, o, e, bi, jb, best):
 PAD = -16384  # score of a pad row / column past the pair's prefix
 
 
-def reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R, live_min=None, stats=None):
+def reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R, BLOCK, live_min=None, stats=None):
     """The same reverse DP evaluated on the GPU kernel's band: strips of R
     rows, columns in 4-column groups from jlo (the first group with a live
     cell, V' >= min(0, o - e), in the row above the strip) until a group with no live
@@ -137,16 +139,17 @@ def check_pair(s1, s2, M, o, e):
         return
     best = int(full["score"])
     bi, bj = full["best_row"], full["best_col"]
-    jb = (bj - 1) // BLOCK * BLOCK + 1
-    si, sj, wend = reverse_start(s1, s2, M, o, e, bi, jb, best)
-    for R in (2, 16):
-        assert reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R) == (si, sj), (R, s1, s2, o, e)
-    assert 1 <= si <= bi and 1 <= sj <= bj <= wend
-    sub = O.sw_literal(s1[si - 1:bi], s2[sj - 1:wend], M, o, e)
-    assert int(sub["score"]) == best
-    assert (sub["best_row"] + si - 1, sub["best_col"] + sj - 1) == (bi, bj)
-    for k in ("columns", "identity", "similarity", "gaps"):
-        assert sub[k] == full[k], (k, s1, s2, si, sj)
+    for BLOCK in BLOCKS:
+        jb = (bj - 1) // BLOCK * BLOCK + 1
+        si, sj, wend = reverse_start(s1, s2, M, o, e, bi, jb, best, BLOCK)
+        for R in (2, 16):
+            assert reverse_start_banded(s1, s2, M, o, e, bi, jb, best, R, BLOCK) == (si, sj), (R, s1, s2, o, e)
+        assert 1 <= si <= bi and 1 <= sj <= bj <= wend
+        sub = O.sw_literal(s1[si - 1:bi], s2[sj - 1:wend], M, o, e)
+        assert int(sub["score"]) == best
+        assert (sub["best_row"] + si - 1, sub["best_col"] + sj - 1) == (bi, bj)
+        for k in ("columns", "identity", "similarity", "gaps"):
+            assert sub[k] == full[k], (k, s1, s2, si, sj)
 
 
 def mutate(rng, s, sub, indel, alphabet):
@@ -205,7 +208,7 @@ def _long_pairs(rng, n, lo, hi):
     return out
 
 
-def _rect(s1, s2, M, o, e):
+def _rect(s1, s2, M, o, e, BLOCK=16):
     full = O.sw_literal(s1, s2, M, o, e)
     best = int(full["score"])
     bi, bj = full["best_row"], full["best_col"]
@@ -222,8 +225,8 @@ def test_band_prunes_and_keeps_rectangle(ednafull):
         best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
         if best == 0:
             continue
-        si, sj, _ = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best)
-        assert reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 16, stats=stats) == (si, sj)
+        si, sj, _ = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, 16)
+        assert reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 16, 16, stats=stats) == (si, sj)
     assert stats["evaluated"] < 0.8 * stats["prefix"], stats
 
 
@@ -236,6 +239,6 @@ def test_band_threshold_is_not_slack(ednafull):
         best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
         if best == 0:
             continue
-        ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best)[:2]
-        lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, live_min=4) != ref
+        ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, 16)[:2]
+        lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, 16, live_min=4) != ref
     assert lost > 0
