@@ -49,6 +49,7 @@ HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
     "salsa_write_matrix", "salsa_write_swms_block", "salsa_load_scoring_matrix",
     "salsa_synth_dna", "salsa_synth_protein", "salsa_swms_compute_rank", "salsa_read_restart_file",
+    "salsa_read_matrix_rows", "salsa_read_partial_stripes", "salsa_distance_value",
 )
 
 
@@ -148,6 +149,11 @@ def host():
                                               ctypes.POINTER(i64)]
         H.salsa_read_restart_file.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                               ctypes.POINTER(i32)]
+        H.salsa_read_matrix_rows.argtypes = [ctypes.c_char_p, i32, i64, i32, i32, vp]
+        H.salsa_read_partial_stripes.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i32, i32, vp, vp,
+                                                 i32, i32, vp]
+        H.salsa_distance_value.argtypes = [i32, i32, ctypes.c_double, ctypes.c_double, i32]
+        H.salsa_distance_value.restype = ctypes.c_double
         H._typed = True
     return H
 
@@ -468,6 +474,49 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def write_matrix(path, data):
+    """A row-major TDistance matrix file (WriteFullMatrixOnRank0, or a rank's
+    partial stripe as PartialMatrixBinaryWriter writes it)."""
+    data = np.ascontiguousarray(data)
+    if host().salsa_write_matrix(path.encode(), _ptr(data), data.nbytes) != SWG_OK:
+        raise SwgError(SWG_E_ARGUMENT, "cannot write %s" % path)
+
+
+def read_matrix_rows(path, n_cols, start_row, row_count, dtype=np.int16):
+    """MatrixBinaryReader.ReadInt16/ReadDouble(fileName, startRowIndex, rowCount)
+    (Salsa.CoreTPL/Blas/MatrixBinaryReader.cs:224-311), as PWC and MDS load
+    their distances."""
+    out = np.zeros((row_count, n_cols), dtype=dtype)
+    rc = host().salsa_read_matrix_rows(path.encode(), n_cols, start_row, row_count, out.itemsize, _ptr(out))
+    if rc != SWG_OK:
+        raise SwgError(rc, "cannot read %s" % path)
+    return out
+
+
+def read_partial_stripes(directory, prefix, ext, start_index, end_index, ranges, n_cols, dtype=np.int16):
+    """MatrixBinaryReader.ReadInt16(path, prefix, ext, startIndex, endIndex, ranges)
+    (MatrixBinaryReader.cs:191-217): the rank stripes "<prefix>_<i><ext>"."""
+    starts = np.array([r[0] for r in ranges], dtype=np.int32)
+    ends = np.array([r[1] for r in ranges], dtype=np.int32)
+    rows = int(sum(ends[i] - starts[i] + 1 for i in range(start_index, end_index + 1)))
+    out = np.zeros((rows, n_cols), dtype=dtype)
+    rc = host().salsa_read_partial_stripes(directory.encode(), prefix.encode(), ext.encode(), start_index, end_index,
+                                           _ptr(starts), _ptr(ends), n_cols, out.itemsize, _ptr(out))
+    if rc != SWG_OK:
+        raise SwgError(rc, "cannot read stripes %s_*%s" % (prefix, ext))
+    return out
+
+
+PWC, MDS = 0, 1
+
+
+def distance_value(value, consumer=PWC, distance_cut=-1.0, undefined_value=-1.0, processing_option=0):
+    """The distance a consumer derives from a stored Int16 value:
+    PWCParallelism.getDistanceValue (PWCParallelism.cs:211-218) or
+    SALSAParallelism.getDistanceValue (MDSasCHISQ/SALSAParallelism.cs:318-370)."""
+    return host().salsa_distance_value(int(value), consumer, distance_cut, undefined_value, processing_option)
 
 
 def read_restart_file(path, process_rank):

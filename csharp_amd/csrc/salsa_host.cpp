@@ -110,6 +110,69 @@ int salsa_write_matrix(const char* path, const void* data, int64_t nbytes)
     return w == (size_t)nbytes ? SWG_OK : SWG_E_ARGUMENT;
 }
 
+// MatrixBinaryReader.ReadInt16/ReadDouble(fileName, startRowIndex, rowCount)
+// (Salsa.CoreTPL/Blas/MatrixBinaryReader.cs:224-311, 290-309): seek to
+// startRow * n_cols elements, then row_count rows of n_cols TDistance.
+int salsa_read_matrix_rows(const char* path, int32_t n_cols, int64_t start_row, int32_t row_count, int32_t elem,
+                           void* out)
+{
+    if (!path || !out || n_cols < 0 || start_row < 0 || row_count < 0 || (elem != 2 && elem != 8))
+        return SWG_E_ARGUMENT;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return SWG_E_ARGUMENT;
+    const int64_t off = start_row * (int64_t)n_cols * elem;
+    const size_t want = (size_t)row_count * (size_t)n_cols * (size_t)elem;
+    size_t got = 0;
+    if (fseeko(f, (off_t)off, SEEK_SET) == 0) got = std::fread(out, 1, want, f);
+    std::fclose(f);
+    return got == want ? SWG_OK : SWG_E_ARGUMENT;  // a short file is an error (ReadInt16 throws at EOF)
+}
+
+// MatrixBinaryReader.ReadInt16(path, prefix, ext, startIndex, endIndex, ranges)
+// (MatrixBinaryReader.cs:191-217): the partial stripes "<prefix>_<i><ext>"
+// that SWMPI writes per rank (Salsa.SmithWatermanMPI/Program.cs:145-151,
+// PartialMatrixBinaryWriter: the rank's rows x N, row-major), for i =
+// start_index..end_index, ranges[i] rows each, one after the other.
+int salsa_read_partial_stripes(const char* dir, const char* prefix, const char* ext, int32_t start_index,
+                               int32_t end_index, const int32_t* range_starts, const int32_t* range_ends,
+                               int32_t n_cols, int32_t elem, void* out)
+{
+    if (!dir || !prefix || !ext || !range_starts || !range_ends || !out || start_index < 0 || end_index < start_index)
+        return SWG_E_ARGUMENT;
+    uint8_t* dst = static_cast<uint8_t*>(out);
+    for (int32_t i = start_index; i <= end_index; ++i) {
+        const int32_t rows = range_ends[i] - range_starts[i] + 1;
+        const std::string name = std::string(dir) + "/" + prefix + "_" + std::to_string(i) + ext;
+        const int rc = salsa_read_matrix_rows(name.c_str(), n_cols, 0, rows, elem, dst);
+        if (rc) return rc;
+        dst += (size_t)rows * (size_t)n_cols * (size_t)elem;
+    }
+    return SWG_OK;
+}
+
+// The consumers' distance of a stored Int16 value.
+//  consumer 0: PWCParallelism.getDistanceValue
+//    (Salsa.PairwiseClusteringTPL/PWCParallelism.cs:211-218): value / 32767.0.
+//  consumer 1: SALSAParallelism.getDistanceValue (MDSasCHISQ/SALSAParallelism.cs:
+//    318-370, Int16 build): 32767 means "undefined" when distance_cut > 0 (then
+//    -1.0 if undefined_value < 0, else undefined_value, squared when
+//    processing_option == 2) and 1.0 otherwise; a negative value is undefined
+//    the same way; else value / 32767.0, squared when processing_option == 2.
+double salsa_distance_value(int32_t value, int32_t consumer, double distance_cut, double undefined_value,
+                            int32_t processing_option)
+{
+    const double scale = 32767.0;
+    if (consumer == 0) return value / scale;
+    auto undefined = [&]() {
+        if (undefined_value < 0.0) return -1.0;
+        return processing_option == 2 ? undefined_value * undefined_value : undefined_value;
+    };
+    if (value == 32767) return distance_cut > 0.0 ? undefined() : 1.0;
+    if (value < 0) return undefined();
+    const double d = value / scale;
+    return processing_option == 2 ? d * d : d;
+}
+
 static void put_u16(std::vector<uint8_t>& b, uint32_t v) { b.push_back(v & 0xFF); b.push_back((v >> 8) & 0xFF); }
 static void put_u32(std::vector<uint8_t>& b, uint32_t v)
 {

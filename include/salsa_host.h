@@ -74,6 +74,34 @@ int salsa_swms_compute_rank(swg_ctx* ctx, int32_t n, int32_t world, int32_t rank
                             const char* const* labels, int32_t elem_size, int32_t stop_after,
                             int32_t* last_saved, int64_t* pairs);
 
+/* MatrixBinaryReader.ReadInt16 / ReadDouble(fileName, startRowIndex, rowCount)
+ * (Salsa.CoreTPL/Blas/MatrixBinaryReader.cs:224-311): rows start_row ..
+ * start_row+row_count-1 of a row-major file of n_cols TDistance per row, as the
+ * downstream programs (PWC, MDS) load the matrix this path writes.  elem_size
+ * 2 (Int16) or 8 (Double); a short file is an error. */
+int salsa_read_matrix_rows(const char* path, int32_t n_cols, int64_t start_row, int32_t row_count,
+                           int32_t elem_size, void* out);
+
+/* MatrixBinaryReader.ReadInt16(path, prefix, ext, startIndex, endIndex, ranges)
+ * (MatrixBinaryReader.cs:191-217): the partial stripe files
+ * "<dir>/<prefix>_<i><ext>" (SWMPI's per-rank PartialMatrixBinaryWriter output,
+ * Salsa.SmithWatermanMPI/Program.cs:145-151) for i = start_index..end_index,
+ * range i holding range_ends[i]-range_starts[i]+1 rows, concatenated into out. */
+int salsa_read_partial_stripes(const char* dir, const char* prefix, const char* ext, int32_t start_index,
+                               int32_t end_index, const int32_t* range_starts, const int32_t* range_ends,
+                               int32_t n_cols, int32_t elem_size, void* out);
+
+/* The distance a downstream consumer derives from a stored Int16 value.
+ * consumer 0: PWCParallelism.getDistanceValue (Salsa.PairwiseClusteringTPL/
+ * PWCParallelism.cs:211-218): value / 32767.0.  consumer 1:
+ * SALSAParallelism.getDistanceValue (MDSasCHISQ/SALSAParallelism.cs:318-370,
+ * Int16 build): 32767 is undefined when distance_cut > 0 and 1.0 otherwise,
+ * a negative value is undefined; undefined is -1.0 if undefined_value < 0,
+ * else undefined_value (squared when processing_option == 2); a defined value
+ * is value / 32767.0, squared when processing_option == 2. */
+double salsa_distance_value(int32_t value, int32_t consumer, double distance_cut, double undefined_value,
+                            int32_t processing_option);
+
 /* ReadRestartFile (Salsa.SmithWatermanMS/Program.cs:245-262): the first line
  * is the world size; line 1+k holds "<original rank> <last saved block>" for
  * the k-th restarted process. */

@@ -195,6 +195,30 @@ def test_full_matrix_multi_rank_layout(dna_set, ednafull):
     al.close()
 
 
+def test_full_matrix_read_by_consumers(tmp_path, dna_set, ednafull):
+    """The matrix this path produces, written as WriteFullMatrixOnRank0 and as
+    SWMPI's per-rank partial stripes, loaded the way PWC and MDS load it
+    (MatrixBinaryReader) and turned into their distances."""
+    say("downstream consumers")
+    n = 90
+    sub = dna_set.subset(range(n))
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(sub)
+    full, _ = al.compute_full_matrix(world=3)
+    al.close()
+    path = str(tmp_path / "dist.bin")
+    ca.write_matrix(path, full)
+    ranges = ca.range_partition(n, 3)
+    for i, (r0, r1) in enumerate(ranges):
+        ca.write_matrix(str(tmp_path / ("dist_%d.bin" % i)), full[r0:r1 + 1])
+        assert np.array_equal(ca.read_matrix_rows(path, n, r0, r1 - r0 + 1), full[r0:r1 + 1])
+    assert np.array_equal(ca.read_partial_stripes(str(tmp_path), "dist", ".bin", 0, 2, ranges, n), full)
+    pwc = np.array([[ca.distance_value(v, ca.PWC) for v in row] for row in full[:20]])
+    assert np.array_equal(pwc, full[:20] / 32767.0)
+    assert (pwc >= 0).all() and (pwc <= 1).all() and (np.diag(pwc[:, :20]) == 0).all()
+    assert np.array_equal(full, full.T)
+
+
 @pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16])
 def test_long_sequences(ednafull, schedule):
     say("long sequences schedule=%d" % schedule)
