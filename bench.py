@@ -393,6 +393,11 @@ def workload_run(args):
     return out
 
 
+# queries per serving batch: batches of a few queries run 16 lanes per pair
+# (latency), deep queues fill batches large enough for the two-pass schedule
+SERVE_MAX_BATCH = 32
+
+
 def serve_bench(args):
     """Serving workload: a device-resident panel of 20,000 DNA sequences
     (lengths 300-1500) and Poisson query arrivals (lengths 50-2000) at each
@@ -405,13 +410,13 @@ def serve_bench(args):
     qset = WL.sequences("c5_queries")
     M, o, e = WL.scoring("c5_panel")
     qmax = int(qset.lengths.max())
-    srv = ca.Server(panel, M, o, e, max_query_len=qmax, max_batch=8, lanes=3, hbm_budget=2 << 30, stats=False)
+    srv = ca.Server(panel, M, o, e, max_query_len=qmax, max_batch=SERVE_MAX_BATCH, lanes=3, hbm_budget=4 << 30, stats=False)
     # warm-up: scratch allocation and graph capture on every lane
     warm = [srv.submit(qset.seq(i)) for i in range(6)]
     for t in warm:
         srv.wait(t)
         srv.release(t)
-    out = {"panel": len(panel), "max_batch": 8, "lanes": 3, "results": []}
+    out = {"panel": len(panel), "max_batch": SERVE_MAX_BATCH, "lanes": 3, "results": []}
     base = 6
     for rate in [float(x) for x in args.serve_rates.split(",") if x]:
         n = int(min(200, max(20, rate * 2)))

@@ -559,8 +559,11 @@ int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
     for (int32_t i = g.r0; i <= g.r1; ++i) { sum += L[i]; ++cnt; }
     for (int32_t i = g.c0; i <= g.c1; ++i) { sum += L[i]; ++cnt; }
     const int len_class = cnt ? (int)std::lround(2.0 * std::log2(1.0 + sum / (double)cnt)) : 0;
+    // two size classes only: the host entry point splits a big block into
+    // row chunks of different sizes, and each chunk should not re-open the
+    // trials (the first chunks time one variant each, the rest use the faster)
     const double pairs = (double)(g.r1 - g.r0 + 1) * (double)(g.c1 - g.c0 + 1) * (g.diag_lower ? 0.5 : 1.0);
-    const int size_class = (int)std::lround(std::log2(1.0 + pairs));
+    const int size_class = pairs >= 4e6 ? 1 : 0;
     return ((int64_t)c->ncode << 40) | ((int64_t)len_class << 20) | (int64_t)size_class;
 }
 
@@ -605,7 +608,12 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant,
         t->key = key;
     }
     if (t->chosen >= 0) { *variant = t->chosen; return SWG_OK; }
-    for (int v : {kVarI16, kVarI32}) {
+    // trial order: the two-pass first for small alphabets (DNA), the one-pass
+    // first for large ones (protein, where cheap gap extension makes the
+    // re-run rectangles a large share of the cells): a run of a single block
+    // then gets the variant that measured faster on such data
+    const bool small_alphabet = c->ncode <= 8;
+    for (int v : {small_alphabet ? kVarI16 : kVarI32, small_alphabet ? kVarI32 : kVarI16}) {
         if (!t->tried[v]) {
             *variant = v;
             *trial = t;
