@@ -137,28 +137,36 @@ def main():
 
     torch.cuda.synchronize()
     tot_cells, tot_kms, tot_pairs = 0.0, 0.0, 0
+    # per-kernel times of every timed block: the timed blocks run in the
+    # library's kernel-timing mode (direct launches, an event after each kernel
+    # on the stream the kernels run on; the same kernels as the block's graph)
+    ksum = []
+    al.set_kernel_timing(True)
     t_start = time.perf_counter()
     for k in range(args.steps):
         p, c, ms, strip, var = run_block(args.warmup + k)
         tot_cells += c; tot_kms += ms; tot_pairs += p
+        kt = al.kernel_times()
+        if len(ksum) != len(kt) or any(a[0] != b[0] for a, b in zip(ksum, kt)):
+            ksum = [[name, 0.0, 0.0, 0.0, 0] for name, _, _, _ in kt]
+        for acc, (name, kms, kc, krun) in zip(ksum, kt):
+            acc[1] += kms; acc[2] += kc; acc[3] += krun; acc[4] += 1
         log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (strip %d, %s)" % (
             k, p, c, ms, c / ms / 1e6, strip, VNAME[var]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    al.set_kernel_timing(False)
     gcups = tot_cells / wall / 1e9
     kernel_gcups = tot_cells / (tot_kms / 1e3) / 1e9
     last_block = blocks[(args.warmup + args.steps - 1) % len(blocks)]
     result_dev = dev_out[: (last_block[0][1] - last_block[0][0] + 1) * (last_block[1][1] - last_block[1][0] + 1)]
     result = result_dev.cpu().numpy().reshape(last_block[0][1] - last_block[0][0] + 1, -1)
 
-    # per-kernel times of one more block (direct launches, an event after each
-    # kernel) for the roofline of the dominant kernel
-    al.set_kernel_timing(True)
-    run_block(args.warmup + args.steps - 1)
-    ktimes = al.kernel_times()
-    al.set_kernel_timing(False)
+    # average per-kernel time over the timed blocks, for the roofline of the
+    # dominant kernel
+    ktimes = [(name, ms / n, kc / n, krun / n) for name, ms, kc, krun, n in ksum if n]
     for name, kms, kc, krun in ktimes:
-        log("kernel %s: %.2f ms, %.3e cells (%.3e evaluated)" % (name, kms, kc, krun))
+        log("kernel %s: %.2f ms, %.3e cells (%.3e evaluated) per timed block" % (name, kms, kc, krun))
 
     # PCIe-inclusive rate of the host-memory entry point (one block)
     (r0, r1), (c0, c1) = blocks[0]
