@@ -36,7 +36,7 @@ DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
 OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING = 1, 2
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
-                7: "global", 8: "reverse", 9: "rect64"}
+                7: "global", 8: "reverse", 9: "rect64", 10: "distance"}
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (
@@ -64,6 +64,11 @@ class SwgError(RuntimeError):
 class SmithWatermanGotohException(SwgError):
     """Raised where the reference raises SmithWatermanGotohException
     (Salsa.CoreTPL/Bio/Algorithms/SmithWaterman/SmithWatermanGotohException.cs:30-47)."""
+
+
+class JukesCantorUndefined(SwgError):
+    """Raised where the reference throws Exception("Jukes and Cantor Distance
+    Undefined - Log(Zero)") (Salsa.CoreTPL/Bio/Algorithms/Alignment.cs:340-345)."""
 
 
 class swg_kernel_time(ctypes.Structure):
@@ -161,6 +166,8 @@ def host():
 def _check(rc, exc=SwgError):
     if rc != SWG_OK:
         msg = lib().swg_last_error()
+        if rc == SWG_E_JC_UNDEFINED and exc is SwgError:
+            exc = JukesCantorUndefined
         raise exc(rc, msg.decode() if msg else "")
 
 
@@ -322,9 +329,9 @@ class Aligner:
     def kernel_times(self):
         """Per-kernel [(name, ms, cells, cells_run)] of the last block run with
         kernel timing on."""
-        buf = (swg_kernel_time * 8)()
+        buf = (swg_kernel_time * 16)()
         n = ctypes.c_int32()
-        _check(lib().swg_last_kernel_times(self._h, buf, 8, ctypes.byref(n)))
+        _check(lib().swg_last_kernel_times(self._h, buf, 16, ctypes.byref(n)))
         return [(KERNEL_NAMES.get(buf[k].kernel, str(buf[k].kernel)), buf[k].ms, buf[k].cells, buf[k].cells_run)
                 for k in range(n.value)]
 

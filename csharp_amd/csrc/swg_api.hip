@@ -61,6 +61,16 @@ enum Variant : int {
 
 constexpr int kScoreTile = 2 * kWave;  // pairs per score-pass task
 
+// Work counters of a block (int32 words of BlockBufs::queue, zeroed by a fill
+// kernel at the start of every block): one per persistent launch.
+constexpr int kQueueFirst = 0;       // first kernel (one-pass, 16-lane or score pass)
+constexpr int kQueueTrack = 16;      // flagged re-run
+constexpr int kQueueCapture = 32;    // capture re-run
+constexpr int kQueueReverse = 48;    // reverse pass
+constexpr int kQueueRect = 64;       // big-rectangle re-run
+constexpr int kQueueSecond = 80;     // Kimura2 second pass: + kQueueFirst / kQueueTrack / kQueueCapture / kQueueRect
+constexpr int kQueueWords = 160;
+
 // One set of per-block device buffers; two of them alternate so that the
 // host can plan block k+1 while block k runs.
 struct BlockBufs {
@@ -73,8 +83,9 @@ struct BlockBufs {
     DevBuf<uint32_t> start_cell;
     DevBuf<uint32_t> sorted, sorted2;    // reverse-pass / re-run order (slot -> pair position)
     DevBuf<int32_t> sortbuf, sortbuf2;   // meta words, then bucket count, cursor, base
-    DevBuf<int32_t> queue;        // [0]: first pass, [16]: flagged re-run, [32]: capture re-run
+    DevBuf<int32_t> queue;        // work counters, see kQueue*
     DevBuf<BlockDesc> desc;       // [0]: output desc, [1]: score-pass desc
+    DevBuf<uint32_t> aux;         // Kimura2: first-pass count word per pair of the block
     std::vector<int32_t> h_task_row, h_task_tile, h_tile_col, h_tile_maxlen;
     hipEvent_t done = nullptr;
     bool pending = false;
@@ -88,7 +99,7 @@ struct BlockBufs {
         task_row.release(); task_tile.release(); tile_col.release(); tile_maxlen.release();
         s_task_row.release(); s_task_tile.release(); s_tile_col.release(); s_tile_maxlen.release();
         best_cell.release(); best_score.release(); start_cell.release(); sorted.release(); sorted2.release();
-        sortbuf.release(); sortbuf2.release(); queue.release(); desc.release();
+        sortbuf.release(); sortbuf2.release(); queue.release(); desc.release(); aux.release();
         if (done) (void)hipEventDestroy(done);
         done = nullptr;
     }
@@ -121,6 +132,10 @@ struct Lane {
     int64_t bnd_stride = 0;
     int bnd_slots = 0;
     int share = 1;  // the lane's persistent grids use 1/share of the resident workgroups
+    // Jukes-Cantor undefined: [0] sticky flag of the lane, [1 + row - r0] per
+    // row of a serving batch
+    DevBuf<int32_t> undef;
+    bool undef_rows = false;
 };
 
 }  // namespace
@@ -133,6 +148,7 @@ struct swg_ctx {
     int32_t iscores[SWG_MATRIX_SIZE * SWG_MATRIX_SIZE];
     int32_t gap_open = 0, gap_ext = 0;
     bool count_similarity = false;  // protein: PercentSimilarity
+    int32_t dist_kind = kDistFraction;  // kDistFraction, kDistJukesCantor or kDistKimura
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
     bool lpp_ok = false;            // non-negative penalties (pre-column cells of the 16-lane kernel)
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
@@ -175,7 +191,7 @@ struct swg_ctx {
 
     // kernel-timing mode: direct launches with an event after each kernel
     bool kernel_timing = false;
-    static constexpr int kMaxKernels = 10;
+    static constexpr int kMaxKernels = 12;
     hipEvent_t kt_ev[kMaxKernels] = {};
     int32_t kt_ids[kMaxKernels] = {};
     int kt_n = 0;
@@ -327,14 +343,28 @@ int upload(hipStream_t st, DevBuf<T>& d, const std::vector<T>& h)
     return SWG_OK;
 }
 
-SeqSet make_seqset(const swg_ctx* c)
+// The kernels' view of the sequence set.  Kimura2 runs the statistics
+// kernels twice: the first pass (this SeqSet) stores the transitions' count
+// words into aux plane 0, the second (`second`: the profile's second table)
+// the transversions' into plane 1.
+SeqSet make_seqset(const swg_ctx* c, const Lane& ln, SeqSet* second)
 {
-    SeqSet S;
+    SeqSet S{};
     S.res = c->res.p; S.off = c->off.p; S.len = c->len.p; S.pad_off = c->pad_off;
     S.prof = c->prof.p; S.ncode = c->ncode;
     S.gap_open = c->gap_open; S.gap_ext = c->gap_ext;
     S.tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
     S.nw_region = c->nw && !c->count_similarity ? 1 : 0;
+    S.gap_inc = c->dist_kind == kDistFraction ? 0x10000 : 0;
+    S.dist_kind = c->dist_kind;
+    S.aux_plane = 0;
+    S.undefined = ln.undef.p;
+    S.undefined_rows = ln.undef_rows ? ln.undef.p + 1 : nullptr;
+    *second = S;
+    if (c->dist_kind == kDistKimura) {
+        second->aux_plane = 1;
+        second->prof = c->prof.p + (size_t)c->ncode * c->ncode;
+    }
     return S;
 }
 
@@ -401,11 +431,21 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     desc[0].mirror = g.diag_lower ? 1 : 0;
     desc[0].r0 = g.r0; desc[0].c0 = g.c0; desc[0].ld = ld;
     desc[0].out = dev_out; desc[0].stats = dev_stats;
+    const bool kimura = c->dist_kind == kDistKimura;
+    const bool words = c->dist_kind != kDistFraction;  // distance_kernel after the alignment kernels
+    const int brows = g.r1 - g.r0 + 1, bcols = g.c1 - g.c0 + 1;
+    if (words) {
+        SWG_HIP(B.aux.reserve(std::max<size_t>((size_t)(kimura ? 2 : 1) * brows * bcols, 1)));
+        desc[0].aux = B.aux.p;
+        desc[0].aux_ld = bcols;
+        desc[0].rows = brows;
+    }
     SWG_HIP(B.desc.reserve(2));
-    SWG_HIP(B.queue.reserve(80));
+    SWG_HIP(B.queue.reserve(kQueueWords));
     SWG_HIP(hipMemcpyAsync(B.desc.p, desc, sizeof(desc), hipMemcpyHostToDevice, st));
 
-    const SeqSet S = make_seqset(c);
+    SeqSet S2;
+    const SeqSet S = make_seqset(c, ln, &S2);
     Scratch W;
     W.bnd = ln.bnd.p;
     W.bnd_stride = ln.bnd_stride;
@@ -426,7 +466,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         P.meta = B.sortbuf.p; P.count = B.sortbuf.p + kMetaWords;
         P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
         RV.sorted = B.sorted.p; RV.best_cell = B.best_cell.p; RV.best_score = B.best_score.p;
-        RV.start_cell = B.start_cell.p; RV.desc = B.desc.p; RV.meta = P.meta; RV.queue = B.queue.p + 48;
+        RV.start_cell = B.start_cell.p; RV.desc = B.desc.p; RV.meta = P.meta; RV.queue = B.queue.p + kQueueReverse;
         // sort 2: by re-run rectangle
         P2 = P;
         P2.start_cell = B.start_cell.p;
@@ -436,11 +476,18 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         RT.meta = P2.meta;
         RC = RT;
         RC.start_cell = B.start_cell.p;
-        RT.queue = B.queue.p + 16;
-        RC.queue = B.queue.p + 32;
+        RT.queue = B.queue.p + kQueueTrack;
+        RC.queue = B.queue.p + kQueueCapture;
         RB = RC;
-        RB.queue = B.queue.p + 64;
+        RB.queue = B.queue.p + kQueueRect;
     }
+    // Kimura2 second pass: the same launches on fresh work counters
+    TaskSet T2 = T;
+    T2.queue = B.queue.p + kQueueSecond + kQueueFirst;
+    RerunSet RT2 = RT, RC2 = RC, RB2 = RB;
+    RT2.queue = B.queue.p + kQueueSecond + kQueueTrack;
+    RC2.queue = B.queue.p + kQueueSecond + kQueueCapture;
+    RB2.queue = B.queue.p + kQueueSecond + kQueueRect;
 
     // the block's kernels in stream order; with `marks`, an event is recorded
     // after each kernel (kernel-timing mode, direct launches)
@@ -452,7 +499,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             c->kt_ids[nk] = kernel_id;
             return hipEventRecord(c->kt_ev[nk++], st);
         };
-        hipError_t e = launch_fill_i32(B.queue.p, 0, 80, st);
+        hipError_t e = launch_fill_i32(B.queue.p, 0, kQueueWords, st);
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
@@ -461,9 +508,13 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         } else if (v == kVarI32 || v == kVarPlain) {
             e = launch_sw(S, T, W, s1, persist, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_ONE_PASS);
+            if (e == hipSuccess && kimura) e = launch_sw(S2, T2, W, s1, persist, n1, st);
+            if (e == hipSuccess && kimura) e = after(SWG_KERNEL_ONE_PASS);
         } else if (v == kVarLpp) {
             e = launch_lpp16(S, T, W, s1, true, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_LPP16);
+            if (e == hipSuccess && kimura) e = launch_lpp16(S2, T2, W, s1, true, n1, st);
+            if (e == hipSuccess && kimura) e = after(SWG_KERNEL_LPP16);
         } else {
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
@@ -481,6 +532,18 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_CAPTURE);
             if (e == hipSuccess) e = launch_rect64(S, RB, W, n4, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_RECT64);
+            if (kimura) {
+                if (e == hipSuccess) e = launch_rerun(S2, RT2, W, s2, true, n2, st);
+                if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_TRACK);
+                if (e == hipSuccess) e = launch_rerun(S2, RC2, W, s2, false, n2, st);
+                if (e == hipSuccess) e = after(SWG_KERNEL_RERUN_CAPTURE);
+                if (e == hipSuccess) e = launch_rect64(S2, RB2, W, n4, st);
+                if (e == hipSuccess) e = after(SWG_KERNEL_RECT64);
+            }
+        }
+        if (words && !c->nw) {
+            if (e == hipSuccess) e = launch_distance(S, B.desc.p, brows, bcols, st);
+            if (e == hipSuccess) e = after(SWG_KERNEL_DISTANCE);
         }
         if (mark) c->kt_n = nk;
         return e;
@@ -496,6 +559,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         append_key(key, RT); append_key(key, RC); append_key(key, sort_grid); append_key(key, P2);
         append_key(key, RV); append_key(key, n3); append_key(key, RB); append_key(key, n4);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
+        append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         if (!B.graph[v] || key != B.graph_key[v]) {
             if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
             hipGraph_t gr = nullptr;
@@ -703,6 +767,16 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
         for (int x = 0; x < SWG_MATRIX_SIZE; ++x)
             for (int y = 0; y < SWG_MATRIX_SIZE; ++y)
                 if (c->iscores[x * SWG_MATRIX_SIZE + y] != 0) seen[x] = seen[y] = true;
+    if (c->dist_kind != kDistFraction) {
+        // Kimura2 / Jukes-Cantor count a column holding the gap character as a
+        // gap whatever the pointer: residues '-' would need per-residue gap
+        // increments, which the kernels do not carry
+        for (int32_t i = 0; i < n; ++i)
+            for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k)
+                if (residues[k] == '-')
+                    return fail(SWG_E_UNSUPPORTED, "Kimura2/JukesCantor with the gap character '-' in a sequence");
+        seen[(int)'-'] = false;
+    }
     int32_t ncode = 0;
     for (int s = 0; s < 256; ++s) {
         c->code_of[s] = -1;
@@ -717,25 +791,34 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     std::vector<uint8_t> sym_of(c->ncode, 0);
     for (int s = 0; s < 256; ++s) if (c->code_of[s] >= 0) sym_of[c->code_of[s]] = (uint8_t)s;
 
-    // profile table (score + gap_open, increment of the packed count word)
-    std::vector<int2> prof((size_t)c->ncode * c->ncode);
-    for (int x = 0; x < c->ncode; ++x)
-        for (int y = 0; y < c->ncode; ++y) {
-            int2 v;
-            if (x == pad || y == pad) { v.x = kPadScore; v.y = 0; }
-            else {
-                const int sx = sym_of[x], sy = sym_of[y];
-                const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
-                int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
-                if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
-                // global protein distance: non-gap columns scoring > 0 only
-                // (Salsa.NeedlemanWunschTPL/Program.cs:583-604)
-                if (c->nw && c->count_similarity) cnt = sc > 0 ? 1 : 0;
-                v.x = sc + c->gap_open;
-                v.y = 0x10000 + cnt;
+    // profile table (score + gap_open, increment of the packed count word);
+    // Kimura2 has a second table for its second pass
+    const int ntables = c->dist_kind == kDistKimura ? 2 : 1;
+    std::vector<int2> prof((size_t)ntables * c->ncode * c->ncode);
+    for (int tb = 0; tb < ntables; ++tb)
+        for (int x = 0; x < c->ncode; ++x)
+            for (int y = 0; y < c->ncode; ++y) {
+                int2 v;
+                if (x == pad || y == pad) { v.x = kPadScore; v.y = 0; }
+                else {
+                    const int sx = sym_of[x], sy = sym_of[y];
+                    const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
+                    int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
+                    if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
+                    // global protein distance: non-gap columns scoring > 0 only
+                    // (Salsa.NeedlemanWunschTPL/Program.cs:583-604)
+                    if (c->nw && c->count_similarity) cnt = sc > 0 ? 1 : 0;
+                    // Jukes-Cantor: differences; Kimura2: transitions (A<->G,
+                    // C<->T), then transversions (Alignment.cs:268-283, 328-337)
+                    const bool ts = (sx == 'A' && sy == 'G') || (sx == 'G' && sy == 'A') ||
+                                    (sx == 'C' && sy == 'T') || (sx == 'T' && sy == 'C');
+                    if (c->dist_kind == kDistJukesCantor) cnt = sx != sy ? 1 : 0;
+                    if (c->dist_kind == kDistKimura) cnt = tb == 0 ? (ts ? 1 : 0) : (sx != sy && !ts ? 1 : 0);
+                    v.x = sc + c->gap_open;
+                    v.y = 0x10000 + cnt;
+                }
+                prof[((size_t)tb * c->ncode + x) * c->ncode + y] = v;
             }
-            prof[(size_t)x * c->ncode + y] = v;
-        }
 
     // residue codes, each sequence 4-byte aligned, then the slots, then a
     // tail of pad codes
@@ -777,6 +860,19 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     return SWG_OK;
 }
 
+// Jukes-Cantor pairs with d <= double.Epsilon since the last check of the
+// main lane (the lane's stream must be idle): the reference throws
+// Exception("Jukes and Cantor Distance Undefined - Log(Zero)") for them.
+int take_undefined(swg_ctx* c)
+{
+    if (c->dist_kind != kDistJukesCantor || !c->main.undef.p) return SWG_OK;
+    int32_t flag = 0;
+    SWG_HIP(hipMemcpy(&flag, c->main.undef.p, sizeof(flag), hipMemcpyDeviceToHost));
+    if (!flag) return SWG_OK;
+    SWG_HIP(hipMemset(c->main.undef.p, 0, sizeof(flag)));
+    return fail(SWG_E_JC_UNDEFINED, "Jukes and Cantor Distance Undefined - Log(Zero)");
+}
+
 }  // namespace
 
 extern "C" {
@@ -795,8 +891,13 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (p->diag_mode != SWG_DIAG_LOWER_MIRROR && p->diag_mode != SWG_DIAG_FULL) return fail(SWG_E_ARGUMENT, "bad diag_mode");
     if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPP16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
-    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
-        return fail(SWG_E_UNSUPPORTED, "only PercentIdentity is implemented for nucleic alignments");
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY &&
+        p->distance_fn != SWG_DIST_KIMURA2 && p->distance_fn != SWG_DIST_JUKES_CANTOR)
+        return fail(p->distance_fn == SWG_DIST_MINMAX_NORM ? SWG_E_UNSUPPORTED : SWG_E_ARGUMENT,
+                    "nucleic distances: PercentIdentity, Kimura2 or JukesCantor");
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->mode == SWG_MODE_NW_GLOBAL &&
+        p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
+        return fail(SWG_E_UNSUPPORTED, "the global alignment implements PercentIdentity only for nucleic sequences");
     if (!is_integral(p->gap_open) || !is_integral(p->gap_ext))
         return fail(SWG_E_NONINTEGRAL, "gap penalties must be integers below 32768 for the integer kernels");
     for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k)
@@ -810,6 +911,12 @@ int swg_create(swg_ctx** out, const swg_params* p)
     c->gap_open = (int32_t)p->gap_open;
     c->gap_ext = (int32_t)p->gap_ext;
     c->count_similarity = p->alignment_type == SWG_ALIGN_PROTEIN;
+    // ComputeDistance (Salsa.SmithWatermanMPI/Program.cs:259-285): the
+    // distance function applies to nucleic alignments; protein uses
+    // PercentSimilarity whatever it says
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_KIMURA2) c->dist_kind = kDistKimura;
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_JUKES_CANTOR)
+        c->dist_kind = kDistJukesCantor;
     c->nw = p->mode == SWG_MODE_NW_GLOBAL;
     c->schedule = p->schedule;
     {
@@ -839,6 +946,8 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if ((e = hipEventCreate(&c->main.ev1)) != hipSuccess) return bail(e, "hipEventCreate");
     for (auto& ev : c->kt_ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = c->main.undef.reserve(1)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemset(c->main.undef.p, 0, sizeof(int32_t))) != hipSuccess) return bail(e, "hipMemset");
     for (auto& B : c->main.bufs)
         if ((e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     *out = c;
@@ -866,7 +975,7 @@ int swg_synchronize(swg_ctx* c)
     for (auto& B : c->main.bufs) B.pending = false;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, c->main.ev0, c->main.ev1) == hipSuccess) c->last_kernel_ms = ms;
-    return SWG_OK;
+    return take_undefined(c);
 }
 
 }  // extern "C"
@@ -935,7 +1044,7 @@ int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32
     for (auto& B : c->main.bufs) B.pending = false;
     c->last_cells = cells;
     if (pairs_done) *pairs_done = total_pairs;
-    return SWG_OK;
+    return take_undefined(c);
 }
 
 }  // namespace
@@ -1040,6 +1149,10 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
             break;
         default: break;
         }
+        // Kimura2 runs each statistics kernel twice; their run counters hold both passes
+        if (c->dist_kind == kDistKimura && (t.kernel == SWG_KERNEL_RERUN_TRACK ||
+                                            t.kernel == SWG_KERNEL_RERUN_CAPTURE || t.kernel == SWG_KERNEL_RECT64))
+            t.cells_run *= 0.5;
     }
     *n = cnt;
     return SWG_OK;
@@ -1070,6 +1183,7 @@ void swg_destroy(swg_ctx* c)
         B.release();
     }
     c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
+    c->main.undef.release();
     c->out_stage.release(); c->stats_stage.release();
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     for (int b = 0; b < 2; ++b) {
@@ -1131,6 +1245,7 @@ struct ServeLane {
     uint8_t* h_out = nullptr;     // pinned staging
     int32_t* h_stats = nullptr;
     uint8_t* h_codes = nullptr;   // pinned staging for query residues + lengths
+    int32_t* h_undef = nullptr;   // pinned copy of the lane's Jukes-Cantor flags
     hipEvent_t done = nullptr;
     bool active = false;
     int64_t footprint = 0;
@@ -1192,6 +1307,8 @@ int launch_batch(swg_server* s, ServeLane& L)
                                hipMemcpyHostToDevice, st));
     }
     SWG_HIP(hipMemcpyAsync(c->len.p + L.slot_base, L.h_codes + pos, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (c->dist_kind == kDistJukesCantor)
+        SWG_HIP(launch_fill_i32(L.lane.undef.p, 0, 1 + nb, st));
     // batches too small to fill the GPU one lane per pair use 16 lanes per
     // pair; larger ones the int16 two-pass (or int32) variant -- no timing
     // trials on the serving path
@@ -1209,6 +1326,8 @@ int launch_batch(swg_server* s, ServeLane& L)
     SWG_HIP(hipMemcpyAsync(L.h_out, L.dout.p, (size_t)nb * s->npanel * elem_size(c), hipMemcpyDeviceToHost, st));
     if (s->want_stats)
         SWG_HIP(hipMemcpyAsync(L.h_stats, L.dstats.p, (size_t)nb * s->npanel * 16, hipMemcpyDeviceToHost, st));
+    if (c->dist_kind == kDistJukesCantor)
+        SWG_HIP(hipMemcpyAsync(L.h_undef, L.lane.undef.p, (size_t)(1 + nb) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SWG_HIP(hipEventRecord(L.done, st));
     L.active = true;
     return SWG_OK;
@@ -1226,6 +1345,11 @@ void finish_batch(swg_server* s, ServeLane& L)
             std::memcpy(q->stats, L.h_stats + k * (size_t)s->npanel * 4, (size_t)s->npanel * 16);
         q->t_done = now;
         q->state = 2;
+        if (s->c->dist_kind == kDistJukesCantor && L.h_undef[1 + k]) {
+            // a pair of this query has no Jukes-Cantor distance: the query fails
+            q->state = SWG_E_JC_UNDEFINED;
+            s->error_msg = "Jukes and Cantor Distance Undefined - Log(Zero)";
+        }
         ++s->completed;
     }
     s->used -= L.footprint;
@@ -1354,6 +1478,11 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
         const size_t codes_bytes = (size_t)max_batch * (((size_t)max_query_len + 3) & ~size_t(3)) + max_batch * 4;
         if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_codes), codes_bytes, 0)) != hipSuccess)
             return bail("hipHostMalloc", e);
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_undef), (size_t)(1 + max_batch) * 4, 0)) != hipSuccess)
+            return bail("hipHostMalloc", e);
+        if ((e = L->lane.undef.reserve(1 + max_batch)) != hipSuccess) return bail("hipMalloc", e);
+        if ((e = hipMemset(L->lane.undef.p, 0, (size_t)(1 + max_batch) * 4)) != hipSuccess) return bail("hipMemset", e);
+        L->lane.undef_rows = true;
         L->slot_base = npanel + i * max_batch;
         s->lanes.push_back(std::move(L));
     }
@@ -1460,6 +1589,8 @@ void swg_serve_destroy(swg_server* s)
         if (L.h_out) (void)hipHostFree(L.h_out);
         if (L.h_stats) (void)hipHostFree(L.h_stats);
         if (L.h_codes) (void)hipHostFree(L.h_codes);
+        if (L.h_undef) (void)hipHostFree(L.h_undef);
+        L.lane.undef.release();
         if (L.done) (void)hipEventDestroy(L.done);
         if (L.lane.ev0) (void)hipEventDestroy(L.lane.ev0);
         if (L.lane.ev1) (void)hipEventDestroy(L.lane.ev1);

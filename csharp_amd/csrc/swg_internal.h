@@ -20,9 +20,22 @@ struct BlockDesc {
     int64_t r0, c0, ld;    // output position of pair (a,b): (a-r0)*ld + (b-c0)
     void* out;             // TDistance array (int16 or double)
     int32_t* stats;        // optional swg_pair_stats (4 x int32) at the same position
-    int32_t rows;          // score-pass desc: rows of the block
+    int32_t rows;          // rows of the block (score-pass desc; output desc of word-mode distances)
     int32_t wr, wc;        // score-pass desc: bucket size (first-maximum rows, columns) of the re-run sort
-    int32_t pad_;
+    int32_t aux_ld;        // output desc: row pitch of aux (the block's columns)
+    uint32_t* aux;         // output desc: count words of the block's pairs, plane p of pair (a,b) at
+                           // p*rows*aux_ld + (a-r0)*aux_ld + (b-c0)
+};
+
+// What the carried count word T = (hi << 16) | lo holds.  hi grows by one per
+// diagonal column and by gap_inc >> 16 per gap column; lo by the profile's
+// count flag.  For kDistFraction the alignment kernels write the distance
+// 1 - lo/hi themselves; for the others they store T into aux plane
+// `aux_plane` and distance_kernel turns the planes into the distance.
+enum DistKind : int32_t {
+    kDistFraction = 0,     // hi = columns, lo = identity / similarity
+    kDistJukesCantor = 1,  // hi = diagonal columns, lo = differences (plane 0)
+    kDistKimura = 2,       // hi = diagonal columns, lo = transitions (plane 0) or transversions (plane 1)
 };
 
 // Read-only data of a context (sequence set + scoring).
@@ -36,6 +49,11 @@ struct SeqSet {
     int32_t gap_open, gap_ext;
     int32_t tdist;         // 0 int16, 1 double
     int32_t nw_region;     // global variant: distance over the aligned region (nucleic identity)
+    int32_t gap_inc;       // increment of the count word per gap column (0x10000 or 0)
+    int32_t dist_kind;     // DistKind
+    int32_t aux_plane;     // word-mode distances: the aux plane this pass stores
+    int32_t* undefined;    // set to 1 where Jukes-Cantor is undefined (d <= double.Epsilon)
+    int32_t* undefined_rows;  // optional: also set [a - r0] for the row sequence a of such a pair
 };
 
 // Task lists: task t aligns s1 = task_row[t] against the 64 sequences of
@@ -178,6 +196,9 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode);
 int sw_blocks_per_cu(int strip_rows, bool nw);
 int score16_blocks_per_cu(int strip_rows, int ncode);
 int nw_blocks_per_cu(int strip_rows);
+// Distances of a block from the count words in aux (Jukes-Cantor, Kimura2):
+// desc[0] is the output desc (rows, aux_ld = columns).
+hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int cols, hipStream_t stream);
 // Fill kernels (graph-capturable replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_fill_bytes_zero(void* p, int64_t nbytes, hipStream_t stream);

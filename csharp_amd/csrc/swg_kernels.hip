@@ -16,7 +16,10 @@
 // forward recurrence: T(i,j) = statistics of the traceback path that starts by
 // reading the pointer at (i,j), TG/TH those of the vertical/horizontal gap
 // states.  T packs (columns << 16) | count, count = identity (nucleic) or
-// similarity (protein); the increments come from the profile.
+// similarity (protein); the increments come from the profile.  For the
+// Jukes-Cantor and Kimura2 distances the high half counts diagonal columns
+// only (gap columns add 0) and the count is differences, transitions or
+// transversions.
 #include "swg_internal.h"
 
 #include <climits>
@@ -58,11 +61,32 @@ __device__ __forceinline__ void write_fraction(const SeqSet& S, const BlockDesc&
     }
 }
 
+// double -> Int16 as the x64 CLR converts it (conv.i2: cvttsd2si, then the
+// low 16 bits; NaN and values outside the int32 range give 0x80000000)
+__device__ __forceinline__ short f64_to_i16(double x)
+{
+    const int xi = (x > -2147483649.0 && x < 2147483648.0) ? (int)x : INT_MIN;
+    return (short)xi;
+}
+
+// double.Epsilon of the CLR: the smallest positive subnormal
+constexpr double kDoubleEpsilon = 4.9406564584124654e-324;
+
 __device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc& D, int a, int b,
                                                int score, int Tb)
 {
-    const uint32_t cols = (uint32_t)Tb >> 16, cnt = (uint32_t)Tb & 0xFFFFu;
-    write_fraction(S, D, a, b, cnt, cols, make_int4(score, (int)cols, (int)cnt, 0));
+    const uint32_t hi = (uint32_t)Tb >> 16, lo = (uint32_t)Tb & 0xFFFFu;
+    if (S.dist_kind == kDistFraction) {
+        write_fraction(S, D, a, b, lo, hi, make_int4(score, (int)hi, (int)lo, 0));
+        return;
+    }
+    // word mode: distance_kernel computes the distance
+    D.aux[((int64_t)S.aux_plane * D.rows + (a - D.r0)) * D.aux_ld + (b - D.c0)] = (uint32_t)Tb;
+    if (D.stats) {
+        const int4 st = make_int4(score, (int)hi, (int)lo, 0);
+        reinterpret_cast<int4*>(D.stats)[(int64_t)(a - D.r0) * D.ld + (b - D.c0)] = st;
+        if (D.mirror) reinterpret_cast<int4*>(D.stats)[(int64_t)(b - D.r0) * D.ld + (a - D.c0)] = st;
+    }
 }
 
 // Stage the strip's query profile: lprof[c2][r] = prof[s1[i0+r]][c2] for
@@ -91,6 +115,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
     int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
     const int ntasks = T.desc->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
+    const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
 
     for (int iter = 0;; ++iter) {
@@ -160,12 +185,12 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
                             const int g1 = Gu - e;
                             const bool cg = g1 > tu;
                             const int G = cg ? g1 : tu;
-                            const int TG = (cg ? TGu : Tu) + 0x10000;
+                            const int TG = (cg ? TGu : Tu) + ginc;
                             // horizontal gap (:174-187)
                             const int h1 = Hr[r] - e;
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
-                            THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                            THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             // diagonal and V (:154-157, :190)
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
@@ -240,6 +265,7 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
     int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
     const int ntasks = T.desc->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
+    const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
 
     for (int iter = 0;; ++iter) {
@@ -326,11 +352,11 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
                             const int g1 = Gu - e;
                             const bool cg = g1 > tu;
                             const int G = cg ? g1 : tu;
-                            const int TG = (cg ? TGu : Tu) + 0x10000;
+                            const int TG = (cg ? TGu : Tu) + ginc;
                             const int h1 = Hr[r] - e;
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
-                            THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                            THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
                             V = max(V, 0);
@@ -1009,6 +1035,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     const int64_t r0 = D[1].r0, c0 = D[1].c0;
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
+    const int ginc = S.gap_inc;  // count-word increment per gap column
     const int t_lo = TRACK ? 0 : T.meta[kMetaBigTiles];
     const int t_hi = TRACK ? T.meta[kMetaFlaggedTiles] : T.meta[kMetaTiles];
     // cells evaluated, added to meta once after the task loop: a second
@@ -1119,11 +1146,11 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                         const int g1 = Gu - e;
                         const bool cg = g1 > tu;
                         const int G = cg ? g1 : tu;
-                        const int TG = (cg ? TGu : Tu) + 0x10000;
+                        const int TG = (cg ? TGu : Tu) + ginc;
                         const int h1 = Hr[r] - e;
                         const bool ch = h1 > tr[r];
                         Hr[r] = ch ? h1 : tr[r];
-                        THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                        THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                         const int f = dt + sc;
                         int V = max(max(f, G), Hr[r]);
                         V = max(V, 0);
@@ -1215,6 +1242,7 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
     const int64_t r0 = D[1].r0, c0 = D[1].c0;
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
+    const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
     const int64_t s_lo = (int64_t)T.meta[kMetaFlaggedTiles] * kWave;
     const int64_t s_hi = (int64_t)T.meta[kMetaBigTiles] * kWave;
@@ -1299,11 +1327,11 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
                             const int g1 = Gu - e;
                             const bool cg = g1 > tu;
                             const int G = cg ? g1 : tu;
-                            const int TG = (cg ? TGu : Tu) + 0x10000;
+                            const int TG = (cg ? TGu : Tu) + ginc;
                             const int h1 = Hr[r] - e;
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
-                            THr[r] = (ch ? THr[r] : Tr[r]) + 0x10000;
+                            THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
                             V = max(V, 0);
@@ -1601,6 +1629,64 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
 }
 
+// Jukes-Cantor and Kimura2 distances of a block from the count words the
+// alignment kernels stored (hi = length - gapCount of the aligned pair):
+//   PairwiseAlignment.ComputeJukesCantorDistance (Alignment.cs:314-348)
+//   PairwiseAlignment.ComputeKimuraDistance (Alignment.cs:254-302)
+// in binary64 (built with -ffp-contract=off), then Int16 as (short)(d * 32767)
+// or Double as is.  One workgroup per 256 columns of a row.
+constexpr int kDistThreads = 256;
+
+__global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const BlockDesc* desc)
+{
+    const BlockDesc D = desc[0];
+    const int ntile = (D.aux_ld + kDistThreads - 1) / kDistThreads;
+    const int row = blockIdx.x / ntile;
+    const int col = (blockIdx.x % ntile) * kDistThreads + threadIdx.x;
+    if (row >= D.rows || col >= D.aux_ld) return;
+    const int a = (int)D.r0 + row, b = (int)D.c0 + col;
+    if (D.mirror && b >= a) return;  // lower-mirror block: pairs j < i only
+    const int64_t k = (int64_t)row * D.aux_ld + col;
+    const uint32_t w = D.aux[k];
+    const uint32_t hi = w >> 16, lo = w & 0xFFFFu;
+    double v;
+    if (S.dist_kind == kDistJukesCantor) {
+        const double d = 1.0 - ((4.0 / 3.0) * ((double)lo / (double)hi));
+        if (d <= kDoubleEpsilon) {  // the reference throws here
+            v = 0.0;
+            *S.undefined = 1;
+            if (S.undefined_rows) S.undefined_rows[row] = 1;
+        } else {
+            v = -0.75 * log(d);
+        }
+    } else {
+        const uint32_t tv = D.aux[(int64_t)D.rows * D.aux_ld + k] & 0xFFFFu;
+        const double P = (double)lo / (double)hi;
+        const double Q = (double)tv / (double)hi;
+        if (1.0 - (2.0 * P + Q) <= kDoubleEpsilon) v = 10.0;
+        else if (1.0 - (2.0 * Q) <= kDoubleEpsilon) v = 10.0;
+        else v = (-0.5 * log(1.0 - 2.0 * P - Q) - 0.25 * log(1.0 - 2.0 * Q));
+        if (D.stats) {
+            // the second pass left (score, hi, transversions, 0)
+            const int64_t pos = (int64_t)row * D.ld + col;
+            int4* st = reinterpret_cast<int4*>(D.stats);
+            const int4 x = make_int4(st[pos].x, (int)hi, (int)lo, (int)tv);
+            st[pos] = x;
+            if (D.mirror) st[(int64_t)(b - D.r0) * D.ld + (a - D.c0)] = x;
+        }
+    }
+    const int64_t pos = (int64_t)row * D.ld + col;
+    const int64_t mpos = (int64_t)(b - D.r0) * D.ld + (a - D.c0);
+    if (S.tdist) {
+        static_cast<double*>(D.out)[pos] = v;
+        if (D.mirror) static_cast<double*>(D.out)[mpos] = v;
+    } else {
+        const short sv = f64_to_i16(v * 32767.0);
+        static_cast<short*>(D.out)[pos] = sv;
+        if (D.mirror) static_cast<short*>(D.out)[mpos] = sv;
+    }
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1825,6 +1911,15 @@ int sw_blocks_per_cu(int strip_rows, bool /*nw*/)
     default: break;
     }
     return err == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int cols, hipStream_t st)
+{
+    const int64_t blocks = (int64_t)rows * ((cols + kDistThreads - 1) / kDistThreads);
+    if (blocks <= 0) return hipSuccess;
+    if (blocks > (int64_t(1) << 31) / kDistThreads) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(distance_kernel, dim3((unsigned)blocks), dim3(kDistThreads), 0, st, S, desc);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t st)

@@ -74,7 +74,11 @@ typedef struct swg_ctx swg_ctx;
 typedef struct swg_params {
     int32_t mode;            /* SWG_MODE_* */
     int32_t alignment_type;  /* SWG_ALIGN_*  (SmithWatermanSection.AlignmentType) */
-    int32_t distance_fn;     /* SWG_DIST_*   (SmithWatermanSection.DistanceFunctionType) */
+    int32_t distance_fn;     /* SWG_DIST_*   (SmithWatermanSection.DistanceFunctionType); nucleic
+                                local alignments: PercentIdentity, Kimura2 or JukesCantor
+                                (ComputeDistance, Salsa.SmithWatermanMPI/Program.cs:259-314);
+                                protein: PercentSimilarity whatever is set; MinMaxNormScore and
+                                Kimura2/JukesCantor on the global alignment: SWG_E_UNSUPPORTED */
     int32_t tdistance;       /* SWG_TDIST_* */
     int32_t diag_mode;       /* SWG_DIAG_* */
     int32_t schedule;        /* SWG_SCHED_* */
@@ -87,7 +91,10 @@ typedef struct swg_params {
 
 /* Per-pair statistics (optional output, same row-major position as the
  * distance): the reference's PairwiseAlignment.Score, Sequence1.Length and
- * Identity (nucleic) or Similarity (protein). */
+ * Identity (nucleic) or Similarity (protein).  With Kimura2 / JukesCantor the
+ * counts behind those distances (Alignment.cs:254-348) instead: columns =
+ * length - gapCount, count = transitions (Kimura2) or differenceCount
+ * (JukesCantor), reserved = transversions (Kimura2). */
 typedef struct swg_pair_stats {
     int32_t score;    /* INT32_MIN where the reference's Score is -Infinity (empty sequence) */
     int32_t columns;  /* aligned length */
@@ -103,7 +110,9 @@ int  swg_create(swg_ctx** ctx, const swg_params* p);
 
 /* Hand over the sequence set (the reference's static _sequences,
  * Salsa.SmithWatermanMPI/Program.cs:65,94-96): residues concatenated,
- * upper-case ASCII, offsets[n+1].  Copied and kept resident in HBM. */
+ * upper-case ASCII, offsets[n+1].  Copied and kept resident in HBM.
+ * Kimura2 / JukesCantor refuse the gap character '-' as a residue
+ * (SWG_E_UNSUPPORTED). */
 int  swg_set_sequences(swg_ctx* ctx, const uint8_t* residues, const int64_t* offsets, int32_t n);
 
 /* ComputeBlock (Salsa.SmithWatermanMPI/Program.cs:203-257): align every pair
@@ -114,7 +123,9 @@ int  swg_set_sequences(swg_ctx* ctx, const uint8_t* residues, const int64_t* off
  * SWG_DIAG_LOWER_MIRROR only j<i is aligned (s1 = sequence i) and both (i,j)
  * and (j,i) are written; (i,i) is written as 0.  `stats` (optional, host) gets
  * one swg_pair_stats per computed (i,j) at the same position.  *pairs_done
- * gets the reference's return value (number of alignments). */
+ * gets the reference's return value (number of alignments).  JukesCantor:
+ * SWG_E_JC_UNDEFINED where a pair's d <= double.Epsilon (the reference throws;
+ * that pair's distance is written as 0, the others as usual). */
 int  swg_compute_block(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                        int32_t col_start, int32_t col_end, int32_t is_diagonal,
                        void* out, int64_t out_ld, swg_pair_stats* stats, int64_t* pairs_done);
@@ -127,6 +138,8 @@ int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
                               int32_t col_start, int32_t col_end, int32_t is_diagonal,
                               void* dev_out, int64_t out_ld, swg_pair_stats* dev_stats,
                               int64_t* pairs_done);
+/* Wait for the enqueued blocks.  JukesCantor: SWG_E_JC_UNDEFINED if a pair
+ * of any block since the last check had d <= double.Epsilon. */
 int  swg_synchronize(swg_ctx* ctx);
 
 /* Schedule control (SWG_SCHED_*) and strip height of the int32 alignment
@@ -162,6 +175,8 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
 #define SWG_KERNEL_GLOBAL        7  /* global (NW) alignment */
 #define SWG_KERNEL_REVERSE       8  /* reverse score pass: where the first-maximum alignments can start */
 #define SWG_KERNEL_RECT64        9  /* re-run of the big rectangles, one wave per pair */
+#define SWG_KERNEL_DISTANCE     10  /* Jukes-Cantor / Kimura2 distances from the carried counts
+                                       (Kimura2 runs the statistics kernels twice before it) */
 typedef struct swg_kernel_time {
     int32_t kernel;     /* SWG_KERNEL_* */
     int32_t reserved;

@@ -20,6 +20,7 @@ class OracleResult(ctypes.Structure):
         ("identity", ctypes.c_int32), ("similarity", ctypes.c_int32), ("gaps", ctypes.c_int32),
         ("transitions", ctypes.c_int32), ("transversions", ctypes.c_int32),
         ("nw_region_len", ctypes.c_int32), ("nw_region_identity", ctypes.c_int32), ("nw_similarity", ctypes.c_int32),
+        ("kimura_gaps", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -124,6 +125,14 @@ def kimura2(columns, gaps, ts, tv):
 
 def f64_to_i16(x):
     return int(lib().oracle_f64_to_i16(x))
+
+
+def jukes_cantor(columns, gaps, differences):
+    undefined = ctypes.c_int32(0)
+    v = float(lib().oracle_jukes_cantor(columns, gaps, differences, ctypes.byref(undefined)))
+    if undefined.value:
+        raise ArithmeticError("Jukes and Cantor Distance Undefined - Log(Zero)")
+    return v
 
 
 def sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True, dist_fn=0, tdist_double=False,

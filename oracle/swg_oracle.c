@@ -56,16 +56,17 @@ static int is_transition(char a, char b)
  * over Sequence1/Sequence2 (Alignment.cs:261-284). */
 static void kimura_counts(const char* r1, const char* r2, int32_t len, oracle_result* res)
 {
-    int32_t ts = 0, tv = 0;
+    int32_t ts = 0, tv = 0, gaps = 0;
     for (int32_t k = 0; k < len; ++k) {
         char a = r1[k], b = r2[k];
         if (a != b) {
-            if (a == GAP_CHAR || b == GAP_CHAR) continue;
+            if (a == GAP_CHAR || b == GAP_CHAR) { ++gaps; continue; }
             if (is_transition(a, b)) ++ts; else ++tv;
         }
     }
     res->transitions = ts;
     res->transversions = tv;
+    res->kimura_gaps = gaps;
 }
 
 /* Literal restatement of SmithWatermanGotoh.Align, Construct and Traceback,
@@ -193,14 +194,15 @@ int oracle_sw_literal(const uint8_t* s1, int32_t n1, const uint8_t* s2, int32_t 
  * path that starts by reading the pointer at (i,j); TG/TH hold those of the
  * vertical/horizontal gap states.  Follows the recurrence of
  * SmithWatermanGotoh.cs:147-215 and the traceback rules of :262-315. */
-typedef struct { int32_t cols, ident, sim, gaps, ts, tv; } cstats;
+typedef struct { int32_t cols, ident, sim, gaps, ts, tv, kgaps; } cstats;
 
-static cstats cs_zero(void) { cstats z = {0, 0, 0, 0, 0, 0}; return z; }
+static cstats cs_zero(void) { cstats z = {0, 0, 0, 0, 0, 0, 0}; return z; }
 
+/* a gap column pairs `residue` with '-': a Kimura gap unless the residue is '-' */
 static cstats cs_gapcol(cstats t, uint8_t residue)
 {
     t.cols += 1; t.gaps += 1;
-    (void)residue;
+    if (residue != GAP_CHAR) t.kgaps += 1;
     return t;
 }
 
@@ -212,6 +214,8 @@ static cstats cs_diag(cstats t, uint8_t c1, uint8_t c2, const int32_t* M)
         if (M[(size_t)c1 * ORACLE_MATRIX_SIZE + c2] > 0) t.sim += 1;
         if (c1 != GAP_CHAR && c2 != GAP_CHAR) {
             if (is_transition((char)c1, (char)c2)) t.ts += 1; else t.tv += 1;
+        } else {
+            t.kgaps += 1;
         }
     }
     return t;
@@ -267,6 +271,7 @@ int oracle_sw_carry(const uint8_t* s1, int32_t n1, const uint8_t* s2, int32_t n2
     res->best_row = bi; res->best_col = bj;
     res->columns = bestT.cols; res->identity = bestT.ident; res->similarity = bestT.sim;
     res->gaps = bestT.gaps; res->transitions = bestT.ts; res->transversions = bestT.tv;
+    res->kimura_gaps = bestT.kgaps;
     res->start1 = -1; res->start2 = -1;  /* not carried */
     free(Vp); free(Gp); free(Tp); free(TGp);
     return 0;
@@ -495,20 +500,22 @@ static void* mt_worker(void* arg)
                 d16 = oracle_dist_i16_fraction(r.identity, r.columns);
                 d = oracle_dist_f64_fraction(r.identity, r.columns);
             } else if (J->dist_fn == 1) {
-                d = oracle_kimura2(r.columns, r.gaps, r.transitions, r.transversions);
+                d = oracle_kimura2(r.columns, r.kimura_gaps, r.transitions, r.transversions);
                 d16 = oracle_f64_to_i16(d * 32767);
             } else {
                 int32_t und = 0;
-                d = oracle_jukes_cantor(r.columns, r.gaps, r.transitions + r.transversions, &und);
+                d = oracle_jukes_cantor(r.columns, r.kimura_gaps, r.transitions + r.transversions, &und);
                 if (und) J->err = -3;
                 d16 = oracle_f64_to_i16(d * 32767);
             }
             if (J->tdist_double) ((double*)J->out)[p] = d; else ((int16_t*)J->out)[p] = d16;
             if (J->out_stats) {
+                const int kim = J->nucleic && (J->dist_fn == 1 || J->dist_fn == 2);
                 J->out_stats[4 * p + 0] = isinf(r.score) ? INT32_MIN : (int32_t)r.score;
-                J->out_stats[4 * p + 1] = r.columns;
-                J->out_stats[4 * p + 2] = r.identity;
-                J->out_stats[4 * p + 3] = r.similarity;
+                J->out_stats[4 * p + 1] = kim ? r.columns - r.kimura_gaps : r.columns;
+                J->out_stats[4 * p + 2] = !kim ? r.identity
+                                        : J->dist_fn == 1 ? r.transitions : r.transitions + r.transversions;
+                J->out_stats[4 * p + 3] = !kim ? r.similarity : J->dist_fn == 1 ? r.transversions : 0;
             }
         }
     }

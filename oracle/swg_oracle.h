@@ -36,6 +36,8 @@ typedef struct oracle_result {
     int32_t nw_region_len;      /* lastNonGap - firstNonGap + 1 */
     int32_t nw_region_identity; /* identity counted inside that region */
     int32_t nw_similarity;      /* non-gap columns with M[a][b] > 0 */
+    int32_t kimura_gaps;    /* gapCount of Alignment.cs:270-273, 330-333: columns whose
+                               two characters differ and one is the gap '-' */
 } oracle_result;
 
 /* Literal restatement of SmithWatermanGotoh.Align
@@ -88,7 +90,9 @@ int16_t oracle_f64_to_i16(double x);
  * Salsa.SmithWatermanMPI/Program.cs:259-314 for distance function `dist_fn`
  * (0 PercentIdentity, 1 Kimura2, 2 JukesCantor) and alignment type
  * `nucleic` (0 = protein -> PercentSimilarity).  `out_stats` (optional) gets
- * (score, columns, identity, similarity) as 4 int32 per pair.
+ * (score, columns, identity, similarity) as 4 int32 per pair; for Kimura2
+ * (score, columns - gapCount, transitions, transversions), for JukesCantor
+ * (score, columns - gapCount, differences, 0).
  * dist_fn = -1 selects the global aligner oracle_nw_literal and the distance
  * of Salsa.NeedlemanWunschTPL/Program.cs:269-300 (nucleic: identity over the
  * aligned region; protein: positive-score columns over all columns);

@@ -41,9 +41,18 @@ def test_nonintegral_scores_rejected(ednafull):
 
 
 def test_unsupported_distance_rejected(ednafull):
+    # checked before any device call: MinMaxNormScore (a global-aligner
+    # distance) and Kimura2 / Jukes-Cantor on the global alignment
     with pytest.raises(ca.SwgError) as ei:
-        ca.Aligner(ednafull, 16, 4, distance_fn=ca.DIST_KIMURA2)
+        ca.Aligner(ednafull, 16, 4, distance_fn=ca.DIST_MINMAX_NORM)
     assert ei.value.code == ca.SWG_E_UNSUPPORTED
+    for d in (ca.DIST_KIMURA2, ca.DIST_JUKES_CANTOR):
+        with pytest.raises(ca.SwgError) as ei:
+            ca.Aligner(ednafull, 16, 4, distance_fn=d, mode=ca.MODE_NW_GLOBAL)
+        assert ei.value.code == ca.SWG_E_UNSUPPORTED
+    with pytest.raises(ca.SwgError) as ei:
+        ca.Aligner(ednafull, 16, 4, distance_fn=7)
+    assert ei.value.code == ca.SWG_E_ARGUMENT
 
 
 def test_null_arguments():
