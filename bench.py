@@ -64,6 +64,8 @@ def main():
                     help="force one kernel variant for the timed steps (auto = autotuned)")
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2nw", "c3", "c4"],
                     help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity check")
+    ap.add_argument("--distance", default="pid", choices=["pid", "kimura2", "jc"],
+                    help="c3 distance function: PercentIdentity (the headline), Kimura2 or JukesCantor")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000")
@@ -113,7 +115,10 @@ def main():
 
     torch.cuda.init()
     dev_out = torch.empty(max_rows * max_cols, dtype=torch.int16, device="cuda")
-    al = ca.Aligner(M, o, e, alignment_type=ca.ALIGN_NUCLEIC, tdistance=ca.TDIST_INT16,
+    DIST = {"pid": (ca.DIST_PERCENT_IDENTITY, "PercentIdentity"), "kimura2": (ca.DIST_KIMURA2, "Kimura2"),
+            "jc": (ca.DIST_JUKES_CANTOR, "JukesCantor")}
+    dist_fn, dist_name = DIST[args.distance]
+    al = ca.Aligner(M, o, e, alignment_type=ca.ALIGN_NUCLEIC, distance_fn=dist_fn, tdistance=ca.TDIST_INT16,
                     schedule=ca.SCHED_PERSISTENT)
     al.set_sequences(seqs)
     SCHED = {"auto": ca.SCHED_PERSISTENT, "i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
@@ -220,7 +225,7 @@ def main():
         while dt < args.cpu_seconds:
             a, b = sample(chunk)
             tc = time.perf_counter()
-            d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, threads=threads)
+            d, _ = O.sw_pairs_mt(seqs.residues, seqs.offsets, a, b, M, o, e, dist_fn=dist_fn, threads=threads)
             dt += time.perf_counter() - tc
             cells += float((L[a].astype(np.float64) * L[b]).sum())
             As.append(a); Bs.append(b); Ds.append(d)
@@ -265,8 +270,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "uneven-length DNA all-pairs: 50,000 sequences (60%% N(450,50), 40%% N(1450,100)), "
-                        "EDNAFULL 16/4, PercentIdentity Int16; step = one %dx%d reference block at P=%d"
-                        % (max_rows, max_cols, args.world),
+                        "EDNAFULL 16/4, %s Int16; step = one %dx%d reference block at P=%d"
+                        % (dist_name, max_rows, max_cols, args.world),
             "sequences": n,
             "pairs_per_step": tot_pairs // max(args.steps, 1),
             "cells_per_step": tot_cells / max(args.steps, 1),
