@@ -33,6 +33,7 @@ DIST_PERCENT_IDENTITY, DIST_KIMURA2, DIST_JUKES_CANTOR, DIST_MINMAX_NORM = 0, 1,
 MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
 TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
+PROGRAM_SWMPI, PROGRAM_SWMS = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
 OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING = 1, 2
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
@@ -81,7 +82,7 @@ class swg_params(ctypes.Structure):
         ("mode", ctypes.c_int32), ("alignment_type", ctypes.c_int32), ("distance_fn", ctypes.c_int32),
         ("tdistance", ctypes.c_int32), ("diag_mode", ctypes.c_int32), ("schedule", ctypes.c_int32),
         ("gap_open", ctypes.c_float), ("gap_ext", ctypes.c_float),
-        ("scores", ctypes.POINTER(ctypes.c_float)), ("device", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("scores", ctypes.POINTER(ctypes.c_float)), ("device", ctypes.c_int32), ("program", ctypes.c_int32),
     ]
 
 
@@ -282,10 +283,11 @@ class Aligner:
 
     def __init__(self, scores, gap_open, gap_ext, alignment_type=ALIGN_NUCLEIC,
                  distance_fn=DIST_PERCENT_IDENTITY, tdistance=TDIST_INT16, diag_mode=DIAG_LOWER_MIRROR,
-                 schedule=SCHED_PERSISTENT, mode=MODE_SW_LOCAL, device=0):
+                 schedule=SCHED_PERSISTENT, mode=MODE_SW_LOCAL, device=0, program=PROGRAM_SWMPI):
         self._scores = np.ascontiguousarray(scores, dtype=np.float32).reshape(MATRIX_SIZE, MATRIX_SIZE)
         p = swg_params()
         p.mode, p.alignment_type, p.distance_fn = mode, alignment_type, distance_fn
+        p.program = program
         p.tdistance, p.diag_mode, p.schedule = tdistance, diag_mode, schedule
         p.gap_open, p.gap_ext = float(gap_open), float(gap_ext)
         p.scores = self._scores.ctypes.data_as(ctypes.POINTER(ctypes.c_float))

@@ -148,7 +148,8 @@ struct swg_ctx {
     int32_t iscores[SWG_MATRIX_SIZE * SWG_MATRIX_SIZE];
     int32_t gap_open = 0, gap_ext = 0;
     bool count_similarity = false;  // protein: PercentSimilarity
-    int32_t dist_kind = kDistFraction;  // kDistFraction, kDistJukesCantor or kDistKimura
+    int32_t dist_kind = kDistFraction;  // DistKind
+    bool bio_helpers = false;       // SWMS / NW ComputeDistance helpers (SWG_PROGRAM_SWMS or global mode)
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
     bool lpp_ok = false;            // non-negative penalties (pre-column cells of the 16-lane kernel)
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
@@ -164,6 +165,7 @@ struct swg_ctx {
     DevBuf<int64_t> off;
     DevBuf<int32_t> len;
     DevBuf<int2> prof;
+    DevBuf<int32_t> self_score;  // MinMaxNormScore: sum of M[c][c] per sequence
     int64_t pad_off = 0;
 
     Lane main;      // the context's own execution lane
@@ -355,11 +357,14 @@ SeqSet make_seqset(const swg_ctx* c, const Lane& ln, SeqSet* second)
     S.gap_open = c->gap_open; S.gap_ext = c->gap_ext;
     S.tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
     S.nw_region = c->nw && !c->count_similarity ? 1 : 0;
-    S.gap_inc = c->dist_kind == kDistFraction ? 0x10000 : 0;
+    S.gap_inc = c->dist_kind == kDistFraction || c->dist_kind == kDistMinMax ? 0x10000 : 0;
     S.dist_kind = c->dist_kind;
     S.aux_plane = 0;
     S.undefined = ln.undef.p;
     S.undefined_rows = ln.undef_rows ? ln.undef.p + 1 : nullptr;
+    S.bio_helpers = c->bio_helpers ? 1 : 0;
+    S.minmax_global = c->nw ? 1 : 0;
+    S.self_score = c->self_score.p;
     *second = S;
     if (c->dist_kind == kDistKimura) {
         second->aux_plane = 1;
@@ -541,7 +546,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
                 if (e == hipSuccess) e = after(SWG_KERNEL_RECT64);
             }
         }
-        if (words && !c->nw) {
+        if (words) {
             if (e == hipSuccess) e = launch_distance(S, B.desc.p, brows, bcols, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_DISTANCE);
         }
@@ -805,9 +810,9 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
                     const int sc = c->iscores[sx * SWG_MATRIX_SIZE + sy];
                     int cnt = (sx == sy) ? 1 : 0;  // identity: c1 == c2 (SmithWatermanGotoh.cs:284)
                     if (c->count_similarity && sx != sy && sc > 0) cnt = 1;  // similarity: scores[c1,c2] > 0 (:290)
-                    // global protein distance: non-gap columns scoring > 0 only
-                    // (Salsa.NeedlemanWunschTPL/Program.cs:583-604)
-                    if (c->nw && c->count_similarity) cnt = sc > 0 ? 1 : 0;
+                    // SWMS and global protein distance: non-gap columns scoring > 0 only
+                    // (Salsa.SmithWatermanMS/Program.cs:521-541, Salsa.NeedlemanWunschTPL/Program.cs:583-604)
+                    if (c->bio_helpers && c->count_similarity) cnt = (sc > 0 && sx != '-' && sy != '-') ? 1 : 0;
                     // Jukes-Cantor: differences; Kimura2: transitions (A<->G,
                     // C<->T), then transversions (Alignment.cs:268-283, 328-337)
                     const bool ts = (sx == 'A' && sy == 'G') || (sx == 'G' && sy == 'A') ||
@@ -850,6 +855,15 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     SWG_HIP(hipMemcpy(c->off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice));
     SWG_HIP(hipMemcpy(c->len.p, lens.data(), lens.size() * 4, hipMemcpyHostToDevice));
     SWG_HIP(hipMemcpy(c->prof.p, prof.data(), prof.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (c->dist_kind == kDistMinMax) {
+        // si.Sum(b => mat[b, b]) (Salsa.SmithWatermanMS/Program.cs:547-548)
+        std::vector<int32_t> self((size_t)std::max(ntot, 1), 0);
+        for (int32_t i = 0; i < n; ++i)
+            for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k)
+                self[i] += c->iscores[residues[k] * SWG_MATRIX_SIZE + residues[k]];
+        SWG_HIP(c->self_score.reserve(self.size()));
+        SWG_HIP(hipMemcpy(c->self_score.p, self.data(), self.size() * 4, hipMemcpyHostToDevice));
+    }
     c->pad_off = pad_off;
     c->nseq = ntot;
     c->h_len.assign(lens.begin(), lens.begin() + ntot);
@@ -865,7 +879,7 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
 // Exception("Jukes and Cantor Distance Undefined - Log(Zero)") for them.
 int take_undefined(swg_ctx* c)
 {
-    if (c->dist_kind != kDistJukesCantor || !c->main.undef.p) return SWG_OK;
+    if (c->dist_kind != kDistJukesCantor || c->bio_helpers || !c->main.undef.p) return SWG_OK;
     int32_t flag = 0;
     SWG_HIP(hipMemcpy(&flag, c->main.undef.p, sizeof(flag), hipMemcpyDeviceToHost));
     if (!flag) return SWG_OK;
@@ -891,13 +905,15 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (p->diag_mode != SWG_DIAG_LOWER_MIRROR && p->diag_mode != SWG_DIAG_FULL) return fail(SWG_E_ARGUMENT, "bad diag_mode");
     if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPP16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
-    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn != SWG_DIST_PERCENT_IDENTITY &&
-        p->distance_fn != SWG_DIST_KIMURA2 && p->distance_fn != SWG_DIST_JUKES_CANTOR)
-        return fail(p->distance_fn == SWG_DIST_MINMAX_NORM ? SWG_E_UNSUPPORTED : SWG_E_ARGUMENT,
-                    "nucleic distances: PercentIdentity, Kimura2 or JukesCantor");
+    if (p->program != SWG_PROGRAM_SWMPI && p->program != SWG_PROGRAM_SWMS) return fail(SWG_E_ARGUMENT, "bad program");
+    if (p->distance_fn < SWG_DIST_PERCENT_IDENTITY || p->distance_fn > SWG_DIST_MINMAX_NORM)
+        return fail(SWG_E_ARGUMENT, "bad distance_fn");
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_MINMAX_NORM &&
+        p->mode == SWG_MODE_SW_LOCAL && p->program != SWG_PROGRAM_SWMS)
+        return fail(SWG_E_UNSUPPORTED, "MinMaxNormScore is a distance of the SWMS and global programs");
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->mode == SWG_MODE_NW_GLOBAL &&
-        p->distance_fn != SWG_DIST_PERCENT_IDENTITY)
-        return fail(SWG_E_UNSUPPORTED, "the global alignment implements PercentIdentity only for nucleic sequences");
+        (p->distance_fn == SWG_DIST_KIMURA2 || p->distance_fn == SWG_DIST_JUKES_CANTOR))
+        return fail(SWG_E_UNSUPPORTED, "Kimura2/JukesCantor are not implemented for the global alignment"); 
     if (!is_integral(p->gap_open) || !is_integral(p->gap_ext))
         return fail(SWG_E_NONINTEGRAL, "gap penalties must be integers below 32768 for the integer kernels");
     for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k)
@@ -917,6 +933,8 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_KIMURA2) c->dist_kind = kDistKimura;
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_JUKES_CANTOR)
         c->dist_kind = kDistJukesCantor;
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_MINMAX_NORM) c->dist_kind = kDistMinMax;
+    c->bio_helpers = p->mode == SWG_MODE_NW_GLOBAL || p->program == SWG_PROGRAM_SWMS;
     c->nw = p->mode == SWG_MODE_NW_GLOBAL;
     c->schedule = p->schedule;
     {
@@ -1183,6 +1201,7 @@ void swg_destroy(swg_ctx* c)
         B.release();
     }
     c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
+    c->self_score.release();
     c->main.undef.release();
     c->out_stage.release(); c->stats_stage.release();
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
@@ -1440,6 +1459,8 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
         hbm_budget_bytes <= 0)
         return fail(SWG_E_ARGUMENT, "bad server arguments");
     *out = nullptr;
+    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_MINMAX_NORM)
+        return fail(SWG_E_UNSUPPORTED, "the server does not implement MinMaxNormScore");
     swg_params q = *p;
     q.schedule = SWG_SCHED_PERSISTENT_I32;
     swg_ctx* c = nullptr;

@@ -36,6 +36,7 @@ enum DistKind : int32_t {
     kDistFraction = 0,     // hi = columns, lo = identity / similarity
     kDistJukesCantor = 1,  // hi = diagonal columns, lo = differences (plane 0)
     kDistKimura = 2,       // hi = diagonal columns, lo = transitions (plane 0) or transversions (plane 1)
+    kDistMinMax = 3,       // the alignment score (plane 0); T as for kDistFraction
 };
 
 // Read-only data of a context (sequence set + scoring).
@@ -54,6 +55,9 @@ struct SeqSet {
     int32_t aux_plane;     // word-mode distances: the aux plane this pass stores
     int32_t* undefined;    // set to 1 where Jukes-Cantor is undefined (d <= double.Epsilon)
     int32_t* undefined_rows;  // optional: also set [a - r0] for the row sequence a of such a pair
+    int32_t bio_helpers;   // 1: the SWMS / NW helpers (Kimura2 clamped to 10, Jukes-Cantor 10 where undefined)
+    int32_t minmax_global; // MinMaxNormScore's minimum: 0 (local) or 2*open + (La+Lb)*ext (global)
+    const int32_t* self_score;  // per sequence: sum of M[c][c] over its residues (MinMaxNormScore)
 };
 
 // Task lists: task t aligns s1 = task_row[t] against the 64 sequences of

@@ -81,7 +81,8 @@ __device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc&
         return;
     }
     // word mode: distance_kernel computes the distance
-    D.aux[((int64_t)S.aux_plane * D.rows + (a - D.r0)) * D.aux_ld + (b - D.c0)] = (uint32_t)Tb;
+    D.aux[((int64_t)S.aux_plane * D.rows + (a - D.r0)) * D.aux_ld + (b - D.c0)] =
+        S.dist_kind == kDistMinMax ? (uint32_t)score : (uint32_t)Tb;
     if (D.stats) {
         const int4 st = make_int4(score, (int)hi, (int)lo, 0);
         reinterpret_cast<int4*>(D.stats)[(int64_t)(a - D.r0) * D.ld + (b - D.c0)] = st;
@@ -541,8 +542,19 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
             if (region < 0) region = 0;
             // nucleic: identity over the aligned region (Program.cs:547-571);
             // protein: positive-score columns over all columns (:583-604)
-            write_fraction(S, *T.desc, a, b, cnt, S.nw_region ? (uint32_t)region : cols,
-                           make_int4(capV, (int)cols, (int)cnt, region));
+            if (S.dist_kind == kDistMinMax) {
+                // the score for distance_kernel, the statistics as usual
+                const BlockDesc& D = *T.desc;
+                D.aux[(int64_t)(a - D.r0) * D.aux_ld + (b - D.c0)] = (uint32_t)capV;
+                if (D.stats) {
+                    const int4 st = make_int4(capV, (int)cols, (int)cnt, region);
+                    reinterpret_cast<int4*>(D.stats)[(int64_t)(a - D.r0) * D.ld + (b - D.c0)] = st;
+                    if (D.mirror) reinterpret_cast<int4*>(D.stats)[(int64_t)(b - D.r0) * D.ld + (a - D.c0)] = st;
+                }
+            } else {
+                write_fraction(S, *T.desc, a, b, cnt, S.nw_region ? (uint32_t)region : cols,
+                               make_int4(capV, (int)cols, (int)cnt, region));
+            }
         }
     }
 }
@@ -1629,12 +1641,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
 }
 
-// Jukes-Cantor and Kimura2 distances of a block from the count words the
-// alignment kernels stored (hi = length - gapCount of the aligned pair):
+// Jukes-Cantor, Kimura2 and MinMaxNormScore distances of a block from the
+// words the alignment kernels stored (count words with hi = length - gapCount
+// of the aligned pair, or the score):
 //   PairwiseAlignment.ComputeJukesCantorDistance (Alignment.cs:314-348)
 //   PairwiseAlignment.ComputeKimuraDistance (Alignment.cs:254-302)
+//   with the SWMS / NW helpers: Kimura2 above 10 is 10, Jukes-Cantor where
+//   undefined is 10 (Salsa.SmithWatermanMS/Program.cs:555-668,
+//   Salsa.NeedlemanWunschTPL/Program.cs:606-700), and ComputeMinMaxNormScore
+//   (Salsa.SmithWatermanMS/Program.cs:544-553: minimum 0;
+//   Salsa.NeedlemanWunschTPL/Program.cs:573-581: 2*open + (|si|+|sj|)*ext)
 // in binary64 (built with -ffp-contract=off), then Int16 as (short)(d * 32767)
-// or Double as is.  One workgroup per 256 columns of a row.
+// or Double as is (the Double builds have no MinMaxNormScore case: 0).
+// One workgroup per 256 columns of a row.
 constexpr int kDistThreads = 256;
 
 __global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const BlockDesc* desc)
@@ -1650,12 +1669,24 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const 
     const uint32_t w = D.aux[k];
     const uint32_t hi = w >> 16, lo = w & 0xFFFFu;
     double v;
-    if (S.dist_kind == kDistJukesCantor) {
+    if (S.dist_kind == kDistMinMax) {
+        const int64_t s = (int32_t)w;
+        const int64_t La = S.len[a], Lb = S.len[b];
+        const int64_t maxS = max(S.self_score[a], S.self_score[b]);
+        const int64_t minS = S.minmax_global ? 2 * (int64_t)S.gap_open + (La + Lb) * (int64_t)S.gap_ext : 0;
+        v = 1.0 - (double)(s - minS) * 1.0 / (double)(maxS - minS);
+        // the Double builds have no MinMaxNormScore case; a local pair with an
+        // empty sequence has no score (-Infinity): both give 0
+        if (S.tdist || s == INT_MIN) v = S.tdist ? 0.0 : __builtin_nan("");
+    } else if (S.dist_kind == kDistJukesCantor) {
         const double d = 1.0 - ((4.0 / 3.0) * ((double)lo / (double)hi));
-        if (d <= kDoubleEpsilon) {  // the reference throws here
-            v = 0.0;
-            *S.undefined = 1;
-            if (S.undefined_rows) S.undefined_rows[row] = 1;
+        if (d <= kDoubleEpsilon) {
+            v = 10.0;  // SWMS / NW: the artificial distance
+            if (!S.bio_helpers) {  // SWMPI: the reference throws here
+                v = 0.0;
+                *S.undefined = 1;
+                if (S.undefined_rows) S.undefined_rows[row] = 1;
+            }
         } else {
             v = -0.75 * log(d);
         }
@@ -1666,6 +1697,7 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const 
         if (1.0 - (2.0 * P + Q) <= kDoubleEpsilon) v = 10.0;
         else if (1.0 - (2.0 * Q) <= kDoubleEpsilon) v = 10.0;
         else v = (-0.5 * log(1.0 - 2.0 * P - Q) - 0.25 * log(1.0 - 2.0 * Q));
+        if (S.bio_helpers && v > 10.0) v = 10.0;
         if (D.stats) {
             // the second pass left (score, hi, transversions, 0)
             const int64_t pos = (int64_t)row * D.ld + col;

@@ -57,6 +57,14 @@ extern "C" {
  * every (i,j) of the block (Salsa.SmithWatermanMS/Program.cs:341-375) */
 #define SWG_DIAG_LOWER_MIRROR 0
 #define SWG_DIAG_FULL         1
+/* which program's ComputeDistance helpers a local alignment follows: SWMPI
+ * (PairwiseAlignment, Salsa.CoreTPL/Bio/Algorithms/Alignment.cs:187-348) or
+ * SWMS (Salsa.SmithWatermanMS/Program.cs:417-668: PercentSimilarity counts
+ * non-gap columns scoring > 0 only, Kimura2 clamped to 10, JukesCantor 10
+ * instead of an exception, MinMaxNormScore).  The global mode always follows
+ * Salsa.NeedlemanWunschTPL/Program.cs:269-300,547-700. */
+#define SWG_PROGRAM_SWMPI 0
+#define SWG_PROGRAM_SWMS  1
 /* schedules */
 #define SWG_SCHED_PERSISTENT     0  /* length-bucketed tiles, device work queue, hipGraph per block,
                                        variant (int32 one-pass / int16 two-pass) autotuned */
@@ -75,10 +83,11 @@ typedef struct swg_params {
     int32_t mode;            /* SWG_MODE_* */
     int32_t alignment_type;  /* SWG_ALIGN_*  (SmithWatermanSection.AlignmentType) */
     int32_t distance_fn;     /* SWG_DIST_*   (SmithWatermanSection.DistanceFunctionType); nucleic
-                                local alignments: PercentIdentity, Kimura2 or JukesCantor
-                                (ComputeDistance, Salsa.SmithWatermanMPI/Program.cs:259-314);
-                                protein: PercentSimilarity whatever is set; MinMaxNormScore and
-                                Kimura2/JukesCantor on the global alignment: SWG_E_UNSUPPORTED */
+                                alignments: PercentIdentity, Kimura2 or JukesCantor
+                                (ComputeDistance, Salsa.SmithWatermanMPI/Program.cs:259-314),
+                                MinMaxNormScore with SWG_PROGRAM_SWMS or the global mode;
+                                protein: PercentSimilarity whatever is set; Kimura2/JukesCantor
+                                on the global alignment: SWG_E_UNSUPPORTED */
     int32_t tdistance;       /* SWG_TDIST_* */
     int32_t diag_mode;       /* SWG_DIAG_* */
     int32_t schedule;        /* SWG_SCHED_* */
@@ -86,7 +95,7 @@ typedef struct swg_params {
     float   gap_ext;         /* _gapExtension (Program.cs:59) */
     const float* scores;     /* ScoringMatrix.Scores: 127*127 row-major, row = s1 symbol (ScoringMatrix.cs:155,188) */
     int32_t device;          /* HIP device ordinal */
-    int32_t reserved;
+    int32_t program;         /* SWG_PROGRAM_* (local mode) */
 } swg_params;
 
 /* Per-pair statistics (optional output, same row-major position as the

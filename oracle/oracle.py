@@ -20,7 +20,7 @@ class OracleResult(ctypes.Structure):
         ("identity", ctypes.c_int32), ("similarity", ctypes.c_int32), ("gaps", ctypes.c_int32),
         ("transitions", ctypes.c_int32), ("transversions", ctypes.c_int32),
         ("nw_region_len", ctypes.c_int32), ("nw_region_identity", ctypes.c_int32), ("nw_similarity", ctypes.c_int32),
-        ("kimura_gaps", ctypes.c_int32),
+        ("kimura_gaps", ctypes.c_int32), ("pos_similarity", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -53,6 +53,7 @@ def lib():
         L.oracle_f64_to_i16.argtypes = [ctypes.c_double]
         L.oracle_f64_to_i16.restype = ctypes.c_int16
         L.oracle_sw_pairs_mt.argtypes = [vp, vp, vp, vp, vp, i64, vp, f32, f32, i32, i32, i32, vp, vp, i32]
+        L.oracle_pairs_mt.argtypes = [vp, vp, vp, vp, vp, i64, vp, f32, f32, i32, i32, i32, i32, i32, vp, vp, i32]
         _lib = L
     return _lib
 
@@ -136,9 +137,11 @@ def jukes_cantor(columns, gaps, differences):
 
 
 def sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True, dist_fn=0, tdist_double=False,
-                threads=1, want_stats=False):
+                threads=1, want_stats=False, program=0, global_=None):
     """Multi-threaded literal aligner over pairs (a[k], b[k]); returns
-    (distances, stats[k, (score, columns, identity, similarity)] or None)."""
+    (distances, stats[k, (score, columns, identity, similarity)] or None).
+    program 1 = the SWMS ComputeDistance helpers; dist_fn -1 (or global_)
+    = the global aligner and the NeedlemanWunschTPL distances."""
     residues = np.ascontiguousarray(residues, dtype=np.uint8)
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     lengths = np.diff(offsets).astype(np.int32)
@@ -149,9 +152,11 @@ def sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True
     out = np.zeros(n, dtype=np.float64 if tdist_double else np.int16)
     st = np.zeros((n, 4), dtype=np.int32) if want_stats else None
     res_ptr = residues.ctypes.data if residues.size else np.zeros(1, np.uint8).ctypes.data
-    rc = lib().oracle_sw_pairs_mt(res_ptr, offsets.ctypes.data, lengths.ctypes.data, a.ctypes.data, b.ctypes.data,
-                                  n, sc.ctypes.data, gap_open, gap_ext, int(nucleic), dist_fn, int(tdist_double),
-                                  out.ctypes.data, st.ctypes.data if want_stats else None, threads)
+    glob = (dist_fn < 0) if global_ is None else bool(global_)
+    rc = lib().oracle_pairs_mt(res_ptr, offsets.ctypes.data, lengths.ctypes.data, a.ctypes.data, b.ctypes.data,
+                               n, sc.ctypes.data, gap_open, gap_ext, int(glob), int(program), int(nucleic),
+                               max(dist_fn, 0), int(tdist_double), out.ctypes.data,
+                               st.ctypes.data if want_stats else None, threads)
     if rc == -3:
         raise ArithmeticError("Jukes and Cantor Distance Undefined - Log(Zero)")
     if rc != 0:
@@ -160,9 +165,9 @@ def sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True
 
 
 def nw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=True, tdist_double=False, threads=1,
-                want_stats=False):
+                want_stats=False, dist_fn=0):
     """Global-alignment counterpart of sw_pairs_mt (oracle_nw_literal and the
     distance of Salsa.NeedlemanWunschTPL/Program.cs:269-300); stats rows are
-    (score, columns, count, region length)."""
-    return sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=nucleic, dist_fn=-1,
-                       tdist_double=tdist_double, threads=threads, want_stats=want_stats)
+    (score, columns, count, region length).  dist_fn 3 = MinMaxNormScore."""
+    return sw_pairs_mt(residues, offsets, a, b, scores, gap_open, gap_ext, nucleic=nucleic, dist_fn=dist_fn,
+                       tdist_double=tdist_double, threads=threads, want_stats=want_stats, global_=True)

@@ -135,7 +135,7 @@ int oracle_sw_literal(const uint8_t* s1, int32_t n1, const uint8_t* s2, int32_t 
     }
 
     /* Traceback, :229-332 */
-    int32_t len1 = 0, identity = 0, similarity = 0, gaps = 0;
+    int32_t len1 = 0, identity = 0, similarity = 0, gaps = 0, possim = 0;
     int32_t i = cell_row, j = cell_col;
     size_t k = (size_t)i * n;
     int going = 1;
@@ -156,6 +156,8 @@ int oracle_sw_literal(const uint8_t* s1, int32_t n1, const uint8_t* s2, int32_t 
             r1[len1] = c1; r2[len1] = c2; ++len1;
             if (c1 == c2) { ++identity; ++similarity; }
             else if (scores[(size_t)(uint8_t)c1 * ORACLE_MATRIX_SIZE + (uint8_t)c2] > 0) ++similarity;
+            if (c1 != GAP_CHAR && c2 != GAP_CHAR && scores[(size_t)(uint8_t)c1 * ORACLE_MATRIX_SIZE + (uint8_t)c2] > 0)
+                ++possim;
             break;
         }
         case DIR_LEFT: {
@@ -181,6 +183,7 @@ int oracle_sw_literal(const uint8_t* s1, int32_t n1, const uint8_t* s2, int32_t 
     res->identity = identity;
     res->similarity = similarity;
     res->gaps = gaps;
+    res->pos_similarity = possim;
     kimura_counts(r1, r2, len1, res);
     if (aln1 && aln2) {  /* Reverse(), :345-355 */
         for (int32_t q = 0; q < len1; ++q) { aln1[q] = r1[len1 - 1 - q]; aln2[q] = r2[len1 - 1 - q]; }
@@ -455,7 +458,26 @@ typedef struct {
     void* out; int32_t* out_stats;
     int64_t next; pthread_mutex_t mu;
     int32_t err;
+    int32_t program;
 } mt_job;
+
+/* sum of M[c][c] over a sequence: si.Sum(b => mat[b, b]) */
+static int64_t self_score(const uint8_t* s, int32_t n, const float* M)
+{
+    int64_t t = 0;
+    for (int32_t k = 0; k < n; ++k) t += (int64_t)M[(size_t)s[k] * ORACLE_MATRIX_SIZE + s[k]];
+    return t;
+}
+
+/* ComputeMinMaxNormScore (Salsa.SmithWatermanMS/Program.cs:544-553,
+ * Salsa.NeedlemanWunschTPL/Program.cs:573-581) and the Int16 distance
+ * (TDistance)((1.0 - norm) * TDistance.MaxValue) */
+static int16_t minmax_i16(int64_t s, int64_t self_i, int64_t self_j, int64_t min_s)
+{
+    const int64_t max_s = self_i > self_j ? self_i : self_j;
+    const double norm = (double)(s - min_s) * 1.0 / (double)(max_s - min_s);
+    return oracle_f64_to_i16((1.0 - norm) * 32767);
+}
 
 static void* mt_worker(void* arg)
 {
@@ -477,6 +499,8 @@ static void* mt_worker(void* arg)
                                     J->lengths[ib], J->scores, J->o, J->e, &r, NULL, NULL);
             if (rc != 0) { J->err = -1; continue; }
             double d = 0.0; int16_t d16 = 0;
+            const uint8_t* sa = J->residues + J->offsets[ia];
+            const uint8_t* sb = J->residues + J->offsets[ib];
             if (J->global) {
                 /* Salsa.NeedlemanWunschTPL/Program.cs:269-300 with
                  * ComputePercentIdentity (:547-571) / ComputePercentSimilarity (:583-604) */
@@ -484,6 +508,12 @@ static void* mt_worker(void* arg)
                 int32_t den = J->nucleic ? r.nw_region_len : r.columns;
                 d16 = oracle_dist_i16_fraction(cnt, den);
                 d = oracle_dist_f64_fraction(cnt, den);
+                if (J->nucleic && J->dist_fn == 3) {
+                    const int64_t min_s = 2 * (int64_t)J->o + ((int64_t)J->lengths[ia] + J->lengths[ib]) * (int64_t)J->e;
+                    d16 = minmax_i16((int64_t)r.score, self_score(sa, J->lengths[ia], J->scores),
+                                     self_score(sb, J->lengths[ib], J->scores), min_s);
+                    d = 0.0;
+                }
                 if (J->tdist_double) ((double*)J->out)[p] = d; else ((int16_t*)J->out)[p] = d16;
                 if (J->out_stats) {
                     J->out_stats[4 * p + 0] = (int32_t)r.score;
@@ -494,18 +524,26 @@ static void* mt_worker(void* arg)
                 continue;
             }
             if (!J->nucleic) {
-                d16 = oracle_dist_i16_fraction(r.similarity, r.columns);
-                d = oracle_dist_f64_fraction(r.similarity, r.columns);
+                const int32_t sim = J->program ? r.pos_similarity : r.similarity;
+                d16 = oracle_dist_i16_fraction(sim, r.columns);
+                d = oracle_dist_f64_fraction(sim, r.columns);
+            } else if (J->dist_fn == 3) {
+                d16 = isinf(r.score) ? 0
+                    : minmax_i16((int64_t)r.score, self_score(sa, J->lengths[ia], J->scores),
+                                 self_score(sb, J->lengths[ib], J->scores), 0);
+                d = 0.0;
             } else if (J->dist_fn == 0) {
                 d16 = oracle_dist_i16_fraction(r.identity, r.columns);
                 d = oracle_dist_f64_fraction(r.identity, r.columns);
             } else if (J->dist_fn == 1) {
                 d = oracle_kimura2(r.columns, r.kimura_gaps, r.transitions, r.transversions);
+                if (J->program && d > 10.0) d = 10.0;  /* Salsa.SmithWatermanMS/Program.cs:600-604 */
                 d16 = oracle_f64_to_i16(d * 32767);
             } else {
                 int32_t und = 0;
                 d = oracle_jukes_cantor(r.columns, r.kimura_gaps, r.transitions + r.transversions, &und);
-                if (und) J->err = -3;
+                if (und && J->program) d = 10.0;       /* Salsa.SmithWatermanMS/Program.cs:659-664 */
+                else if (und) J->err = -3;
                 d16 = oracle_f64_to_i16(d * 32767);
             }
             if (J->tdist_double) ((double*)J->out)[p] = d; else ((int16_t*)J->out)[p] = d16;
@@ -515,7 +553,8 @@ static void* mt_worker(void* arg)
                 J->out_stats[4 * p + 1] = kim ? r.columns - r.kimura_gaps : r.columns;
                 J->out_stats[4 * p + 2] = !kim ? r.identity
                                         : J->dist_fn == 1 ? r.transitions : r.transitions + r.transversions;
-                J->out_stats[4 * p + 3] = !kim ? r.similarity : J->dist_fn == 1 ? r.transversions : 0;
+                J->out_stats[4 * p + 3] = !kim ? (J->program ? r.pos_similarity : r.similarity)
+                                        : J->dist_fn == 1 ? r.transversions : 0;
             }
         }
     }
@@ -528,10 +567,21 @@ int oracle_sw_pairs_mt(const uint8_t* residues, const int64_t* offsets, const in
                        int32_t nucleic, int32_t dist_fn, int32_t tdist_double,
                        void* out, int32_t* out_stats, int32_t threads)
 {
+    return oracle_pairs_mt(residues, offsets, lengths, a, b, npairs, scores, o, e, dist_fn < 0, 0, nucleic,
+                           dist_fn < 0 ? 0 : dist_fn, tdist_double, out, out_stats, threads);
+}
+
+int oracle_pairs_mt(const uint8_t* residues, const int64_t* offsets, const int32_t* lengths,
+                    const int32_t* a, const int32_t* b, int64_t npairs,
+                    const float* scores, float o, float e,
+                    int32_t global, int32_t program, int32_t nucleic, int32_t dist_fn, int32_t tdist_double,
+                    void* out, int32_t* out_stats, int32_t threads)
+{
     mt_job J;
     J.residues = residues; J.offsets = offsets; J.lengths = lengths;
     J.a = a; J.b = b; J.npairs = npairs; J.scores = scores; J.o = o; J.e = e;
-    J.nucleic = nucleic; J.dist_fn = dist_fn; J.tdist_double = tdist_double; J.global = dist_fn < 0;
+    J.nucleic = nucleic; J.dist_fn = dist_fn; J.tdist_double = tdist_double; J.global = global;
+    J.program = program;
     J.out = out; J.out_stats = out_stats; J.next = 0; J.err = 0;
     pthread_mutex_init(&J.mu, NULL);
     if (threads < 1) threads = 1;

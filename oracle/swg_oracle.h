@@ -38,6 +38,8 @@ typedef struct oracle_result {
     int32_t nw_similarity;      /* non-gap columns with M[a][b] > 0 */
     int32_t kimura_gaps;    /* gapCount of Alignment.cs:270-273, 330-333: columns whose
                                two characters differ and one is the gap '-' */
+    int32_t pos_similarity; /* local: non-gap columns with M[a][b] > 0
+                               (Salsa.SmithWatermanMS/Program.cs:521-541) */
 } oracle_result;
 
 /* Literal restatement of SmithWatermanGotoh.Align
@@ -102,6 +104,19 @@ int oracle_sw_pairs_mt(const uint8_t* residues, const int64_t* offsets, const in
                        const float* scores, float gap_open, float gap_ext,
                        int32_t nucleic, int32_t dist_fn, int32_t tdist_double,
                        void* out, int32_t* out_stats, int32_t threads);
+
+/* Same driver with the program's ComputeDistance helpers: `global` selects
+ * oracle_nw_literal and Salsa.NeedlemanWunschTPL/Program.cs:269-300,547-700;
+ * otherwise program 0 = SWMPI (as oracle_sw_pairs_mt), 1 = SWMS
+ * (Salsa.SmithWatermanMS/Program.cs:417-668: PercentSimilarity over non-gap
+ * columns scoring > 0, Kimura2 clamped to 10, JukesCantor 10 where
+ * undefined).  dist_fn 3 = MinMaxNormScore (SWMS: minimum 0; global:
+ * 2*open + (|si|+|sj|)*ext), Int16 only (the Double builds return 0). */
+int oracle_pairs_mt(const uint8_t* residues, const int64_t* offsets, const int32_t* lengths,
+                    const int32_t* a, const int32_t* b, int64_t npairs,
+                    const float* scores, float gap_open, float gap_ext,
+                    int32_t global, int32_t program, int32_t nucleic, int32_t dist_fn, int32_t tdist_double,
+                    void* out, int32_t* out_stats, int32_t threads);
 
 #ifdef __cplusplus
 }
