@@ -510,6 +510,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         if (c->nw) {
             e = launch_nw(S, T, W, s1, persist, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_GLOBAL);
+            if (e == hipSuccess && kimura) e = launch_nw(S2, T2, W, s1, persist, n1, st);
+            if (e == hipSuccess && kimura) e = after(SWG_KERNEL_GLOBAL);
         } else if (v == kVarI32 || v == kVarPlain) {
             e = launch_sw(S, T, W, s1, persist, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_ONE_PASS);
@@ -772,7 +774,7 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
         for (int x = 0; x < SWG_MATRIX_SIZE; ++x)
             for (int y = 0; y < SWG_MATRIX_SIZE; ++y)
                 if (c->iscores[x * SWG_MATRIX_SIZE + y] != 0) seen[x] = seen[y] = true;
-    if (c->dist_kind != kDistFraction) {
+    if (c->dist_kind == kDistJukesCantor || c->dist_kind == kDistKimura) {
         // Kimura2 / Jukes-Cantor count a column holding the gap character as a
         // gap whatever the pointer: residues '-' would need per-residue gap
         // increments, which the kernels do not carry
@@ -911,9 +913,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_MINMAX_NORM &&
         p->mode == SWG_MODE_SW_LOCAL && p->program != SWG_PROGRAM_SWMS)
         return fail(SWG_E_UNSUPPORTED, "MinMaxNormScore is a distance of the SWMS and global programs");
-    if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->mode == SWG_MODE_NW_GLOBAL &&
-        (p->distance_fn == SWG_DIST_KIMURA2 || p->distance_fn == SWG_DIST_JUKES_CANTOR))
-        return fail(SWG_E_UNSUPPORTED, "Kimura2/JukesCantor are not implemented for the global alignment"); 
+
     if (!is_integral(p->gap_open) || !is_integral(p->gap_ext))
         return fail(SWG_E_NONINTEGRAL, "gap penalties must be integers below 32768 for the integer kernels");
     for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k)

@@ -423,6 +423,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
     int2* __restrict__ bnd2 = reinterpret_cast<int2*>(bnd + W.bnd_stride / 2);
     const int ntasks = T.desc->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
+    const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
 
     for (int iter = 0;; ++iter) {
@@ -448,16 +449,16 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
 
         // row 0: a horizontal gap run of j columns from the origin
         for (int j = 1; j <= ncols; ++j) {
-            bnd[(int64_t)j * kWave + lane] = make_int4(nw_border(o, e, j) - o, kNeg, j << 16, 0);
+            bnd[(int64_t)j * kWave + lane] = make_int4(nw_border(o, e, j) - o, kNeg, j * ginc, 0);
             bnd2[(int64_t)j * kWave + lane] = make_int2((j << 16) | j, 0);
         }
 
         // result cell (La, Lb): border cells when a sequence is empty
         int capV = 0;
         uint32_t cap1 = 0, cap2 = 0;
-        if (La == 0 && Lb > 0) { capV = nw_border(o, e, Lb); cap1 = (uint32_t)Lb << 16; cap2 = ((uint32_t)Lb << 16) | Lb; }
+        if (La == 0 && Lb > 0) { capV = nw_border(o, e, Lb); cap1 = (uint32_t)(Lb * ginc); cap2 = ((uint32_t)Lb << 16) | Lb; }
         if (Lb == 0 && La > 0) {
-            capV = nw_border(o, e, La); cap1 = (uint32_t)La << 16; cap2 = ((uint32_t)La << 16) | kUpBit | La;
+            capV = nw_border(o, e, La); cap1 = (uint32_t)(La * ginc); cap2 = ((uint32_t)La << 16) | kUpBit | La;
         }
 
         for (int i0 = 0; i0 < La; i0 += R) {
@@ -470,12 +471,12 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = i0 + r + 1;  // column-0 border: a vertical run of i rows
-                tr[r] = nw_border(o, e, i) - o; Tr1[r] = i << 16; Tr2[r] = (i << 16) | (int)kUpBit | i;
+                tr[r] = nw_border(o, e, i) - o; Tr1[r] = i * ginc; Tr2[r] = (i << 16) | (int)kUpBit | i;
                 Hr[r] = kNeg; TH1[r] = 0; TH2[r] = 0;
             }
             int dtb, dTb1, dTb2;  // V[i0][0] for the first row's diagonal
             if (i0 == 0) { dtb = -o; dTb1 = 0; dTb2 = 0; }
-            else { dtb = nw_border(o, e, i0) - o; dTb1 = i0 << 16; dTb2 = (i0 << 16) | (int)kUpBit | i0; }
+            else { dtb = nw_border(o, e, i0) - o; dTb1 = i0 * ginc; dTb2 = (i0 << 16) | (int)kUpBit | i0; }
 
             for (int j = 1; j <= ncols; ++j) {
                 const uint32_t w = s2w[(j - 1) >> 2];
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
                         const int g1 = Gu - e;
                         const bool cg = g1 > tu;
                         const int G = cg ? g1 : tu;
-                        const int TG1 = (cg ? TGu1 : Tu1) + 0x10000;
+                        const int TG1 = (cg ? TGu1 : Tu1) + ginc;
                         const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
                         const int TG2 = (cg ? TGu2 : baseG) + 1;
                         // horizontal gap
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
                         const bool ch = h1 > tr[r];
                         Hr[r] = ch ? h1 : tr[r];
                         const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
-                        TH1[r] = (ch ? TH1[r] : Tr1[r]) + 0x10000;
+                        TH1[r] = (ch ? TH1[r] : Tr1[r]) + ginc;
                         TH2[r] = (ch ? TH2[r] : baseH) + 1;
                         const int f = dt + sc;
                         const int V = max(max(f, G), Hr[r]);
@@ -542,12 +543,15 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
             if (region < 0) region = 0;
             // nucleic: identity over the aligned region (Program.cs:547-571);
             // protein: positive-score columns over all columns (:583-604)
-            if (S.dist_kind == kDistMinMax) {
-                // the score for distance_kernel, the statistics as usual
+            if (S.dist_kind != kDistFraction) {
+                // word mode: the score (MinMaxNormScore) or the count word
+                // for distance_kernel, and the statistics
                 const BlockDesc& D = *T.desc;
-                D.aux[(int64_t)(a - D.r0) * D.aux_ld + (b - D.c0)] = (uint32_t)capV;
+                D.aux[((int64_t)S.aux_plane * D.rows + (a - D.r0)) * D.aux_ld + (b - D.c0)] =
+                    S.dist_kind == kDistMinMax ? (uint32_t)capV : cap1;
                 if (D.stats) {
-                    const int4 st = make_int4(capV, (int)cols, (int)cnt, region);
+                    const int4 st = S.dist_kind == kDistMinMax ? make_int4(capV, (int)cols, (int)cnt, region)
+                                                               : make_int4(capV, (int)cols, (int)cnt, 0);
                     reinterpret_cast<int4*>(D.stats)[(int64_t)(a - D.r0) * D.ld + (b - D.c0)] = st;
                     if (D.mirror) reinterpret_cast<int4*>(D.stats)[(int64_t)(b - D.r0) * D.ld + (a - D.c0)] = st;
                 }

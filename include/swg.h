@@ -86,8 +86,7 @@ typedef struct swg_params {
                                 alignments: PercentIdentity, Kimura2 or JukesCantor
                                 (ComputeDistance, Salsa.SmithWatermanMPI/Program.cs:259-314),
                                 MinMaxNormScore with SWG_PROGRAM_SWMS or the global mode;
-                                protein: PercentSimilarity whatever is set; Kimura2/JukesCantor
-                                on the global alignment: SWG_E_UNSUPPORTED */
+                                protein: PercentSimilarity whatever is set */
     int32_t tdistance;       /* SWG_TDIST_* */
     int32_t diag_mode;       /* SWG_DIAG_* */
     int32_t schedule;        /* SWG_SCHED_* */

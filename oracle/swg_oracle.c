@@ -514,12 +514,27 @@ static void* mt_worker(void* arg)
                                      self_score(sb, J->lengths[ib], J->scores), min_s);
                     d = 0.0;
                 }
+                int kim = J->nucleic && (J->dist_fn == 1 || J->dist_fn == 2);
+                if (kim) {
+                    /* ComputeKimuraDistance / ComputeJukesCantorDistance of the
+                     * global program (:606-700): clamp at 10, 10 where undefined */
+                    if (J->dist_fn == 1) {
+                        d = oracle_kimura2(r.columns, r.kimura_gaps, r.transitions, r.transversions);
+                        if (d > 10.0) d = 10.0;
+                    } else {
+                        int32_t und = 0;
+                        d = oracle_jukes_cantor(r.columns, r.kimura_gaps, r.transitions + r.transversions, &und);
+                        if (und) d = 10.0;
+                    }
+                    d16 = oracle_f64_to_i16(d * 32767);
+                }
                 if (J->tdist_double) ((double*)J->out)[p] = d; else ((int16_t*)J->out)[p] = d16;
                 if (J->out_stats) {
                     J->out_stats[4 * p + 0] = (int32_t)r.score;
-                    J->out_stats[4 * p + 1] = r.columns;
-                    J->out_stats[4 * p + 2] = cnt;
-                    J->out_stats[4 * p + 3] = r.nw_region_len;
+                    J->out_stats[4 * p + 1] = kim ? r.columns - r.kimura_gaps : r.columns;
+                    J->out_stats[4 * p + 2] = !kim ? cnt : J->dist_fn == 1 ? r.transitions
+                                                                         : r.transitions + r.transversions;
+                    J->out_stats[4 * p + 3] = !kim ? r.nw_region_len : J->dist_fn == 1 ? r.transversions : 0;
                 }
                 continue;
             }

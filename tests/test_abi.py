@@ -41,17 +41,16 @@ def test_nonintegral_scores_rejected(ednafull):
 
 
 def test_unsupported_distance_rejected(ednafull):
-    # checked before any device call: MinMaxNormScore (a global-aligner
-    # distance) and Kimura2 / Jukes-Cantor on the global alignment
+    # checked before any device call: MinMaxNormScore exists in the SWMS and
+    # global programs only; unknown distance functions and programs
     with pytest.raises(ca.SwgError) as ei:
         ca.Aligner(ednafull, 16, 4, distance_fn=ca.DIST_MINMAX_NORM)
     assert ei.value.code == ca.SWG_E_UNSUPPORTED
-    for d in (ca.DIST_KIMURA2, ca.DIST_JUKES_CANTOR):
-        with pytest.raises(ca.SwgError) as ei:
-            ca.Aligner(ednafull, 16, 4, distance_fn=d, mode=ca.MODE_NW_GLOBAL)
-        assert ei.value.code == ca.SWG_E_UNSUPPORTED
     with pytest.raises(ca.SwgError) as ei:
         ca.Aligner(ednafull, 16, 4, distance_fn=7)
+    assert ei.value.code == ca.SWG_E_ARGUMENT
+    with pytest.raises(ca.SwgError) as ei:
+        ca.Aligner(ednafull, 16, 4, program=5)
     assert ei.value.code == ca.SWG_E_ARGUMENT
 
 

@@ -149,6 +149,33 @@ def test_global_minmax(dna_set, ednafull):
     al.close()
 
 
+@pytest.mark.parametrize("tdouble", [False, True])
+@pytest.mark.parametrize("dist", [ca.DIST_KIMURA2, ca.DIST_JUKES_CANTOR])
+def test_global_kimura_jukes_cantor(dna_set, ednafull, dist, tdouble):
+    """The global kernel carries the Kimura2 / Jukes-Cantor counts like the
+    local ones (Kimura2: two passes); the global program's helpers clamp
+    Kimura2 at 10 and give Jukes-Cantor 10 where undefined.  Int16 and the
+    statistics bit-exact, Double within 4 ulp (the logarithm)."""
+    say("global dist=%d double=%d" % (dist, tdouble))
+    al = ca.Aligner(ednafull, 16, 4, distance_fn=dist, mode=ca.MODE_NW_GLOBAL,
+                    tdistance=ca.TDIST_DOUBLE if tdouble else ca.TDIST_INT16)
+    al.set_sequences(dna_set)
+    n = len(dna_set)
+    got, gst, _ = al.compute_block(0, 50, 51, n - 1, False, stats=True)
+    a, b = np.meshgrid(np.arange(0, 51), np.arange(51, n), indexing="ij")
+    want, wst = O.nw_pairs_mt(dna_set.residues, dna_set.offsets, a.ravel(), b.ravel(), ednafull, 16, 4,
+                              dist_fn=dist, tdist_double=tdouble, threads=THREADS, want_stats=True)
+    assert np.array_equal(gst.reshape(-1, 4), wst)
+    g = got.ravel()
+    if tdouble:
+        nan = np.isnan(want)
+        assert np.array_equal(np.isnan(g), nan)
+        assert np.all(np.abs(g[~nan] - want[~nan]) <= np.abs(want[~nan]) * 2.0 ** -50)
+    else:
+        assert np.array_equal(g, want)
+    al.close()
+
+
 def test_minmax_needs_swms_or_global(ednafull):
     with pytest.raises(ca.SwgError) as ei:
         ca.Aligner(ednafull, 16, 4, distance_fn=ca.DIST_MINMAX_NORM)

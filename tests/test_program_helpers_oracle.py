@@ -117,3 +117,22 @@ def test_minmax_global(ednafull):
         assert d[k] == want
     assert int(st[1][0]) == 20 and d[1] == 0          # identical: norm 1
     assert int(st[0][0]) == 20 - 16                  # one end gap
+
+
+def test_global_kimura_jukes_cantor(ednafull):
+    # global alignments: ACGTACGT / ACGTACGA is 8 diagonal columns with one
+    # T/A transversion; ACGTT / ACGT has one end gap and no difference
+    res, off = pairs(b"ACGTACGT", b"ACGTACGA", b"ACGTT", b"ACGT")
+    for fn in (1, 2):
+        d, st = O.nw_pairs_mt(res, off, [0, 2], [1, 3], ednafull, 16, 4, dist_fn=fn, want_stats=True,
+                              tdist_double=True)
+        k2 = -0.5 * np.log(1.0 - 2.0 * 0 - 1 / 8) - 0.25 * np.log(1.0 - 2.0 * (1 / 8))
+        jc = -0.75 * np.log(1.0 - (4.0 / 3.0) * (1 / 8))
+        assert d[0] == (k2 if fn == 1 else jc)
+        assert st[0].tolist()[1:] == ([8, 0, 1] if fn == 1 else [8, 1, 0])
+        assert d[1] == 0.0 and st[1].tolist()[1:] == [4, 0, 0]
+    # undefined Jukes-Cantor: the global program's artificial distance 10
+    M = positive_mismatch_matrix()
+    res, off = pairs(b"AAAA", b"CCCC")
+    d, _ = O.nw_pairs_mt(res, off, [0], [1], M, 16, 4, dist_fn=2)
+    assert d[0] == -10
