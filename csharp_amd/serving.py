@@ -2,7 +2,9 @@
 
 Latency of a query = its submit time -> the time its last distance is in
 host memory (the server's own steady-clock stamps); GCUPS = DP cells of the
-completed queries / (last completion - first submit).
+completed queries / (last completion - first submit); admitted concurrency =
+queries admitted to a batch and not yet done, averaged over that span and at
+its peak; queue wait = submit -> admission.
 """
 import threading
 import time
@@ -42,16 +44,22 @@ def run_poisson(server, queries, rate, seed=0, timeout_s=600.0):
             if time.time() > deadline:
                 raise TimeoutError("serving run did not drain")
     n1, cells1, _ = server.stats()
-    sub, done = [], []
+    sub, adm, done = [], [], []
     for t in tickets:
-        s, _, d = server.times(t)
+        s, a, d = server.times(t)
         sub.append(s)
+        adm.append(a)
         done.append(d)
         server.release(t)
     sub = np.array(sub)
+    adm = np.array(adm)
     done = np.array(done)
     lat = done - sub
     span_s = (done.max() - sub.min()) / 1e3
+    # in flight between admission and completion: +1 at admit, -1 at done
+    ev = np.concatenate([np.stack([adm, np.ones_like(adm)], 1), np.stack([done, -np.ones_like(done)], 1)])
+    ev = ev[np.lexsort((ev[:, 1], ev[:, 0]))]
+    peak = int(np.cumsum(ev[:, 1]).max())
     return {
         "rate_qps": rate,
         "queries": len(queries),
@@ -60,4 +68,7 @@ def run_poisson(server, queries, rate, seed=0, timeout_s=600.0):
         "mean_ms": round(float(lat.mean()), 2),
         "gcups": round((cells1 - cells0) / span_s / 1e9, 1),
         "achieved_qps": round(len(queries) / span_s, 1),
+        "queue_wait_p50_ms": round(float(np.percentile(adm - sub, 50)), 2),
+        "admitted_concurrency_mean": round(float((done - adm).sum() / 1e3 / span_s), 2),
+        "admitted_concurrency_max": peak,
     }
