@@ -89,9 +89,19 @@ struct BlockBufs {
     std::vector<int32_t> h_task_row, h_task_tile, h_tile_col, h_tile_maxlen;
     hipEvent_t done = nullptr;
     bool pending = false;
-    // one captured graph per variant, keyed by every baked launch argument
-    hipGraphExec_t graph[kNumVariants] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<unsigned char> graph_key[kNumVariants];
+    // one captured graph per variant and grid size class (the whole device or
+    // a serving lane's share), keyed by every baked launch argument
+    hipGraphExec_t graph[kNumVariants][2] = {};
+    std::vector<unsigned char> graph_key[kNumVariants][2];
+
+    void destroy_graphs()
+    {
+        for (auto& per_variant : graph)
+            for (auto& gexec : per_variant) {
+                if (gexec) (void)hipGraphExecDestroy(gexec);
+                gexec = nullptr;
+            }
+    }
 
     // device memory and the completion event (graphs are destroyed by the owner)
     void release()
@@ -567,8 +577,10 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         append_key(key, RV); append_key(key, n3); append_key(key, RB); append_key(key, n4);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
-        if (!B.graph[v] || key != B.graph_key[v]) {
-            if (B.graph[v]) { (void)hipGraphExecDestroy(B.graph[v]); B.graph[v] = nullptr; }
+        hipGraphExec_t& gexec = B.graph[v][ln.share == 1 ? 0 : 1];
+        std::vector<unsigned char>& gkey = B.graph_key[v][ln.share == 1 ? 0 : 1];
+        if (!gexec || key != gkey) {
+            if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
             hipGraph_t gr = nullptr;
             SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const hipError_t e = launch_all(false);
@@ -578,10 +590,10 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
                 return fail(SWG_E_DEVICE, std::string("graph capture failed: ") +
                                               hipGetErrorString(e != hipSuccess ? e : e2));
             }
-            hipError_t e3 = hipGraphInstantiate(&B.graph[v], gr, nullptr, nullptr, 0);
+            hipError_t e3 = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
-            if (e3 != hipSuccess) { B.graph[v] = nullptr; return fail(SWG_E_DEVICE, "hipGraphInstantiate failed"); }
-            B.graph_key[v] = key;
+            if (e3 != hipSuccess) { gexec = nullptr; return fail(SWG_E_DEVICE, "hipGraphInstantiate failed"); }
+            gkey = key;
         }
     }
     if (timing) {
@@ -593,7 +605,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
         if (v == kVarPlain || timing) SWG_HIP(launch_all(timing));
-        else SWG_HIP(hipGraphLaunch(B.graph[v], st));
+        else SWG_HIP(hipGraphLaunch(B.graph[v][ln.share == 1 ? 0 : 1], st));
     }
     SWG_HIP(hipEventRecord(ln.ev1, st));
     SWG_HIP(hipEventRecord(B.done, st));
@@ -1196,8 +1208,7 @@ void swg_destroy(swg_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->main.stream) (void)hipStreamSynchronize(c->main.stream);
     for (auto& B : c->main.bufs) {
-        for (auto& gexec : B.graph)
-            if (gexec) (void)hipGraphExecDestroy(gexec);
+        B.destroy_graphs();
         B.release();
     }
     c->res.release(); c->off.release(); c->len.release(); c->prof.release(); c->main.bnd.release();
@@ -1331,6 +1342,14 @@ int launch_batch(swg_server* s, ServeLane& L)
     // batches too small to fill the GPU one lane per pair use 16 lanes per
     // pair; larger ones the int16 two-pass (or int32) variant -- no timing
     // trials on the serving path
+    // a partial batch that starts while another lane's batch is on the
+    // device gets a 1/lanes share of the resident slots, so that neither
+    // lane's later kernels wait for the other's persistent grid to drain
+    // (latency under moderate load); alone, or full (a deep queue: the
+    // device saturated), the whole device (throughput)
+    bool others = false;
+    for (auto& Lp : s->lanes) others = others || (Lp.get() != &L && Lp->active);
+    L.lane.share = others && nb < s->max_batch ? s->nlanes : 1;
     BlockGeom g;
     g.r0 = L.slot_base; g.r1 = L.slot_base + nb - 1; g.c0 = 0; g.c1 = s->npanel - 1; g.diag_lower = false;
     c->schedule = c->nw                                  ? SWG_SCHED_PERSISTENT_I32
@@ -1602,7 +1621,7 @@ void swg_serve_destroy(swg_server* s)
         ServeLane& L = *Lp;
         if (L.lane.stream) (void)hipStreamSynchronize(L.lane.stream);
         for (auto& B : L.lane.bufs) {
-            for (auto& g : B.graph) if (g) (void)hipGraphExecDestroy(g);
+            B.destroy_graphs();
             B.release();
         }
         L.lane.bnd.release();
