@@ -142,9 +142,9 @@ def main():
 
     torch.cuda.synchronize()
     tot_cells, tot_kms, tot_pairs = 0.0, 0.0, 0
-    # per-kernel times of every timed block: the timed blocks run in the
-    # library's kernel-timing mode (direct launches, an event after each kernel
-    # on the stream the kernels run on; the same kernels as the block's graph)
+    # per-kernel times of every timed block: in the library's kernel-timing
+    # mode the block's graph also runs a one-thread stamp kernel (device wall
+    # clock) before its kernels and after each, on the stream they run on
     ksum = []
     al.set_kernel_timing(True)
     t_start = time.perf_counter()

@@ -325,7 +325,8 @@ class Aligner:
         _check(lib().swg_set_option(self._h, OPT_STREAM_CHUNK_BYTES, int(nbytes)))
 
     def set_kernel_timing(self, on):
-        """Direct launches with an event after each kernel (see kernel_times)."""
+        """A wall-clock stamp kernel before the block's kernels and after each,
+        inside the block's graph (see kernel_times)."""
         _check(lib().swg_set_option(self._h, OPT_KERNEL_TIMING, 1 if on else 0))
 
     def kernel_times(self):

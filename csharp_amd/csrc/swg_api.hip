@@ -90,9 +90,13 @@ struct BlockBufs {
     hipEvent_t done = nullptr;
     bool pending = false;
     // one captured graph per variant and grid size class (the whole device or
-    // a serving lane's share), keyed by every baked launch argument
+    // a serving lane's share), keyed by every baked launch argument; with
+    // kernel timing, the graph's stamp kernels and the kernel ids they follow
+    static constexpr int kMaxGraphKernels = 12;
     hipGraphExec_t graph[kNumVariants][2] = {};
     std::vector<unsigned char> graph_key[kNumVariants][2];
+    int32_t graph_kids[kNumVariants][2][kMaxGraphKernels] = {};
+    int graph_kn[kNumVariants][2] = {};
 
     void destroy_graphs()
     {
@@ -201,10 +205,12 @@ struct swg_ctx {
     int32_t last_strip = 0;
     int32_t last_variant = 0;
 
-    // kernel-timing mode: direct launches with an event after each kernel
+    // kernel-timing mode: a stamp kernel before the block's kernels and after
+    // each of them (inside the block's graph), wall-clock ticks at wall_khz
     bool kernel_timing = false;
-    static constexpr int kMaxKernels = 12;
-    hipEvent_t kt_ev[kMaxKernels] = {};
+    static constexpr int kMaxKernels = BlockBufs::kMaxGraphKernels;
+    DevBuf<unsigned long long> kt_stamp;   // kMaxKernels + 1 stamps
+    int64_t wall_khz = 0;
     int32_t kt_ids[kMaxKernels] = {};
     int kt_n = 0;
     Plan kt_plan;
@@ -504,17 +510,21 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     RC2.queue = B.queue.p + kQueueSecond + kQueueCapture;
     RB2.queue = B.queue.p + kQueueSecond + kQueueRect;
 
-    // the block's kernels in stream order; with `marks`, an event is recorded
-    // after each kernel (kernel-timing mode, direct launches)
+    // the block's kernels in stream order; with `mark`, a stamp kernel runs
+    // first and after each kernel (kernel-timing mode)
     const bool timing = c->kernel_timing && &ln == &c->main;
+    int32_t ids[BlockBufs::kMaxGraphKernels] = {};
+    int nids = 0;
     auto launch_all = [&](bool mark) -> hipError_t {
         int nk = 0;
         auto after = [&](int kernel_id) -> hipError_t {
             if (!mark) return hipSuccess;
-            c->kt_ids[nk] = kernel_id;
-            return hipEventRecord(c->kt_ev[nk++], st);
+            if (nk >= BlockBufs::kMaxGraphKernels) return hipErrorInvalidValue;
+            ids[nk++] = kernel_id;
+            return launch_stamp(c->kt_stamp.p, nk, st);
         };
-        hipError_t e = launch_fill_i32(B.queue.p, 0, kQueueWords, st);
+        hipError_t e = mark ? launch_stamp(c->kt_stamp.p, 0, st) : hipSuccess;
+        if (e == hipSuccess) e = launch_fill_i32(B.queue.p, 0, kQueueWords, st);
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
@@ -562,13 +572,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             if (e == hipSuccess) e = launch_distance(S, B.desc.p, brows, bcols, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_DISTANCE);
         }
-        if (mark) c->kt_n = nk;
+        nids = nk;
         return e;
     };
 
     // capture (or re-capture) before the timing events so that host-side
     // capture time is not counted as device time
-    if (plan.ntasks > 0 && v != kVarPlain && !timing) {
+    const int gs = ln.share == 1 ? 0 : 1;
+    if (plan.ntasks > 0 && v != kVarPlain) {
         // every pointer and scalar baked into the graph is part of its key:
         // re-capture whenever one changed (new sequences, grown buffers)
         std::vector<unsigned char> key;
@@ -577,13 +588,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         append_key(key, RV); append_key(key, n3); append_key(key, RB); append_key(key, n4);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
-        hipGraphExec_t& gexec = B.graph[v][ln.share == 1 ? 0 : 1];
-        std::vector<unsigned char>& gkey = B.graph_key[v][ln.share == 1 ? 0 : 1];
+        append_key(key, timing); append_key(key, c->kt_stamp.p);
+        hipGraphExec_t& gexec = B.graph[v][gs];
+        std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
             if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
             hipGraph_t gr = nullptr;
             SWG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const hipError_t e = launch_all(false);
+            const hipError_t e = launch_all(timing);
             hipError_t e2 = hipStreamEndCapture(st, &gr);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (gr) (void)hipGraphDestroy(gr);
@@ -594,6 +606,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             (void)hipGraphDestroy(gr);
             if (e3 != hipSuccess) { gexec = nullptr; return fail(SWG_E_DEVICE, "hipGraphInstantiate failed"); }
             gkey = key;
+            B.graph_kn[v][gs] = nids;
+            std::copy(ids, ids + nids, B.graph_kids[v][gs]);
         }
     }
     if (timing) {
@@ -604,8 +618,17 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     }
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks > 0) {
-        if (v == kVarPlain || timing) SWG_HIP(launch_all(timing));
-        else SWG_HIP(hipGraphLaunch(B.graph[v][ln.share == 1 ? 0 : 1], st));
+        if (v == kVarPlain) {
+            SWG_HIP(launch_all(timing));
+        } else {
+            SWG_HIP(hipGraphLaunch(B.graph[v][gs], st));
+            nids = B.graph_kn[v][gs];
+            std::copy(B.graph_kids[v][gs], B.graph_kids[v][gs] + nids, ids);
+        }
+        if (timing) {
+            c->kt_n = nids;
+            std::copy(ids, ids + nids, c->kt_ids);
+        }
     }
     SWG_HIP(hipEventRecord(ln.ev1, st));
     SWG_HIP(hipEventRecord(B.done, st));
@@ -974,8 +997,13 @@ int swg_create(swg_ctx** out, const swg_params* p)
     if (e != hipSuccess) return bail(e, "hipStreamCreate");
     if ((e = hipEventCreate(&c->main.ev0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&c->main.ev1)) != hipSuccess) return bail(e, "hipEventCreate");
-    for (auto& ev : c->kt_ev)
-        if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = c->kt_stamp.reserve(swg_ctx::kMaxKernels + 1)) != hipSuccess) return bail(e, "hipMalloc");
+    {
+        int khz = 0;
+        if ((e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device)) != hipSuccess)
+            return bail(e, "hipDeviceGetAttribute");
+        c->wall_khz = khz;
+    }
     if ((e = c->main.undef.reserve(1)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = hipMemset(c->main.undef.p, 0, sizeof(int32_t))) != hipSuccess) return bail(e, "hipMemset");
     for (auto& B : c->main.bufs)
@@ -1147,13 +1175,14 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
     if (c->kt_meta) SWG_HIP(hipMemcpy(meta64, c->kt_meta, sizeof(meta64), hipMemcpyDeviceToHost));
     if (c->kt_meta2) SWG_HIP(hipMemcpy(meta64b, c->kt_meta2, sizeof(meta64b), hipMemcpyDeviceToHost));
     const int cnt = std::min<int>(c->kt_n, max);
+    unsigned long long stamp[swg_ctx::kMaxKernels + 1] = {};
+    if (cnt > 0) SWG_HIP(hipMemcpy(stamp, c->kt_stamp.p, (size_t)(c->kt_n + 1) * sizeof(stamp[0]), hipMemcpyDeviceToHost));
+    if (cnt > 0 && c->wall_khz <= 0) return fail(SWG_E_DEVICE, "unknown wall-clock rate");
     for (int k = 0; k < cnt; ++k) {
-        float ms = 0.0f;
-        SWG_HIP(hipEventElapsedTime(&ms, k == 0 ? c->main.ev0 : c->kt_ev[k - 1], c->kt_ev[k]));
         swg_kernel_time& t = out[k];
         t.kernel = c->kt_ids[k];
         t.reserved = 0;
-        t.ms = ms;
+        t.ms = (double)(int64_t)(stamp[k + 1] - stamp[k]) / (double)c->wall_khz;
         t.cells = 0.0;
         t.cells_run = 0.0;
         switch (t.kernel) {
@@ -1225,8 +1254,7 @@ void swg_destroy(swg_ctx* c)
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->main.ev0) (void)hipEventDestroy(c->main.ev0);
     if (c->main.ev1) (void)hipEventDestroy(c->main.ev1);
-    for (auto& ev : c->kt_ev)
-        if (ev) (void)hipEventDestroy(ev);
+    c->kt_stamp.release();
     if (c->main.stream) (void)hipStreamDestroy(c->main.stream);
     delete c;
 }

@@ -203,6 +203,10 @@ int nw_blocks_per_cu(int strip_rows);
 // Distances of a block from the count words in aux (Jukes-Cantor, Kimura2):
 // desc[0] is the output desc (rows, aux_ld = columns).
 hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int cols, hipStream_t stream);
+// Kernel-timing stamp: p[k] = the device's constant-rate wall clock, taken
+// when every earlier kernel on the stream has finished (a node of the block's
+// graph like any other kernel).
+hipError_t launch_stamp(unsigned long long* p, int k, hipStream_t stream);
 // Fill kernels (graph-capturable replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_fill_bytes_zero(void* p, int64_t nbytes, hipStream_t stream);

@@ -1723,6 +1723,11 @@ __global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const 
     }
 }
 
+__global__ void stamp_kernel(unsigned long long* p, int k)
+{
+    if (threadIdx.x == 0) p[k] = wall_clock64();
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1955,6 +1960,12 @@ hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int
     if (blocks <= 0) return hipSuccess;
     if (blocks > (int64_t(1) << 31) / kDistThreads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(distance_kernel, dim3((unsigned)blocks), dim3(kDistThreads), 0, st, S, desc);
+    return hipGetLastError();
+}
+
+hipError_t launch_stamp(unsigned long long* p, int k, hipStream_t st)
+{
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(kWave), 0, st, p, k);
     return hipGetLastError();
 }
 

@@ -159,9 +159,10 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * by chunk while the previous chunk is copied to pinned host memory;
  * default 64 MiB). */
 #define SWG_OPT_STREAM_CHUNK_BYTES 1
-/* SWG_OPT_KERNEL_TIMING = 1: the following blocks launch their kernels
- * directly (no captured graph) with an event after each kernel, for
- * swg_last_kernel_times; 0 (default) restores graph launches. */
+/* SWG_OPT_KERNEL_TIMING = 1: the following blocks run a one-thread stamp
+ * kernel (the device's constant-rate wall clock) before their kernels and
+ * after each of them, inside the block's captured graph, for
+ * swg_last_kernel_times; 0 (default) leaves the stamps out. */
 #define SWG_OPT_KERNEL_TIMING 2
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
@@ -188,7 +189,7 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
 typedef struct swg_kernel_time {
     int32_t kernel;     /* SWG_KERNEL_* */
     int32_t reserved;
-    double  ms;         /* event time from the previous kernel's end (block start for the first) */
+    double  ms;         /* wall-clock time from the previous kernel's end (block start for the first) */
     double  cells;      /* DP cells the kernel is charged with: the block's cells (one-pass, score
                            pass); the rectangles up to each pair's first-maximum block (reverse
                            pass); the rectangles from each pair's earliest optimal start to that
