@@ -175,7 +175,7 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
                      bool persistent, int nslots, hipStream_t stream);
 hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
                      bool persistent, int nslots, hipStream_t stream);
-// Score-only int16x2 pass; strip_rows in {16, 32}.
+// Score-only int16x2 pass; strip_rows in {16, 24, 32}.
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
                           int nslots, hipStream_t stream);
 // Sort of the score pass's first-maximum cells (count, scan, scatter).

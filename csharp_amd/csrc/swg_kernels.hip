@@ -1829,6 +1829,10 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
         if (joint) hipLaunchKernelGGL((score16_kernel<16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
         else hipLaunchKernelGGL((score16_kernel<16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
         break;
+    case 24:
+        if (joint) hipLaunchKernelGGL((score16_kernel<24, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        else hipLaunchKernelGGL((score16_kernel<24, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        break;
     case 32:
         if (joint) hipLaunchKernelGGL((score16_kernel<32, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
         else hipLaunchKernelGGL((score16_kernel<32, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
@@ -1931,6 +1935,10 @@ int score16_blocks_per_cu(int strip_rows, int ncode)
     case 16:
         err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, true>, kWave, 0)
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, false>, kWave, 0);
+        break;
+    case 24:
+        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<24, true>, kWave, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<24, false>, kWave, 0);
         break;
     case 32:
         err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32, true>, kWave, 0)

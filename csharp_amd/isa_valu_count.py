@@ -18,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 # kernel symbol fragment -> DP cells evaluated by one pass of the loop body
 KERNELS = {
-    "score16_kernelILi16ELb1E": ("score16", 4 * 16 * 2),       # 4 columns x 16 rows x 2 pairs per lane
-    "score16_kernelILi16ELb0E": ("score16 (large alphabet)", 4 * 16 * 2),
+    "score16_kernelILi24ELb1E": ("score16", 4 * 24 * 2),       # 4 columns x 24 rows x 2 pairs per lane
+    "score16_kernelILi24ELb0E": ("score16 (large alphabet)", 4 * 24 * 2),
     "rerun_kernelILi16ELb0E": ("rerun_capture", 4 * 16),
     "reverse_kernelILi16E": ("reverse", 4 * 16 * 2),           # 2 pairs per lane
     "sw_kernelILi16ELb1E": ("one_pass", 4 * 16),
