@@ -91,7 +91,7 @@ struct ScoreTaskSet {
 
 // Columns per first-maximum block of the score pass: best_cell names the
 // block, the re-run finds the exact column inside it.
-constexpr int kFirstMaxBlock = 4;
+constexpr int kFirstMaxBlock = 16;
 
 // best_cell flag: the pair's score may have saturated int16, or a sequence
 // is empty; the re-run then tracks the first maximum over the whole matrix.

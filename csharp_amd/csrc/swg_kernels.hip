@@ -676,7 +676,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
             uint32_t tr[R], Hr[R], rb[R], rj[R], bm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = NEG2; rb[r] = M1; rj[r] = 0u; }
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = NEG2; rb[r] = M1; rj[r] = 0u; bm[r] = NEG2; }
             uint32_t dtb = T0;
 
             uint32_t nwa = wA[0], nwb = wB[0];
@@ -753,7 +753,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                             const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G),
                                                                       __builtin_elementwise_max(H, Z2));
                             const s16x2 tn = V - O2;
-                            bm[r] = q == 0 ? u2(V) : u2(__builtin_elementwise_max(v2(bm[r]), V));
+                            bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
                             dt = tr[r];
                             tr[r] = u2(tn);
                             Hr[r] = u2(H);
@@ -765,18 +765,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
                     for (int r4 = 0; r4 < R / 4; ++r4) cur[r4] = nx[r4];
                 }
-                // block maximum bm > row maximum rb (no wrap: rb >= -1,
+                // after the last 4-column step of a 16-column block (or of the
+                // row): block maximum bm > row maximum rb (no wrap: rb >= -1,
                 // bm >= 0): the sign of (rb - bm) per half, spread to a 16-bit
                 // mask, selects the block with one bitfield insert
-                static_assert(kFirstMaxBlock == 4, "a block is one 4-column step");
-                const uint32_t JB = splat16(j);
+                static_assert(kFirstMaxBlock == 16, "a block is four 4-column steps");
+                if (((j - 1) & 15) == 12 || j + 4 > ncols) {
+                    const uint32_t JB = splat16(((j - 1) & ~15) + 1);
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint32_t d = u2(v2(rb[r]) - v2(bm[r]));
-                    uint32_t mk;
-                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
-                    rj[r] = (mk & JB) | (~mk & rj[r]);
-                    rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), v2(bm[r])));
+                    for (int r = 0; r < R; ++r) {
+                        const uint32_t d = u2(v2(rb[r]) - v2(bm[r]));
+                        uint32_t mk;
+                        asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
+                        rj[r] = (mk & JB) | (~mk & rj[r]);
+                        rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), v2(bm[r])));
+                        bm[r] = NEG2;
+                    }
                 }
             }
 #pragma unroll

@@ -4,8 +4,10 @@ Compiles csrc/swg_kernels.hip for gfx950 to assembly (this build's own
 code), takes each kernel's largest basic block (the unrolled 4-column x
 R-row loop body, with the block that closes the loop when the compiler
 splits it in two) and divides its VALU instruction count by the cells that
-body evaluates.  bench.py's roofline uses these figures as the kernels'
-operations per cell.
+body evaluates.  The score pass folds its block maxima into the row maxima
+once per 16-column block, i.e. after every fourth body: the instructions the
+body skips with a conditional branch count a quarter.  bench.py's roofline
+uses these figures as the kernels' operations per cell.
 
 usage: python -m csharp_amd.isa_valu_count
 """
@@ -17,9 +19,10 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 # kernel symbol fragment -> DP cells evaluated by one pass of the loop body
+# (and, optionally, the passes per execution of its conditional tail)
 KERNELS = {
-    "score16_kernelILi24ELb1E": ("score16", 4 * 24 * 2),       # 4 columns x 24 rows x 2 pairs per lane
-    "score16_kernelILi24ELb0E": ("score16 (large alphabet)", 4 * 24 * 2),
+    "score16_kernelILi24ELb1E": ("score16", 4 * 24 * 2, 4),    # 4 columns x 24 rows x 2 pairs per lane
+    "score16_kernelILi24ELb0E": ("score16 (large alphabet)", 4 * 24 * 2, 4),
     "rerun_kernelILi16ELb0E": ("rerun_capture", 4 * 16),
     "reverse_kernelILi16E": ("reverse", 4 * 16 * 2),           # 2 pairs per lane
     "sw_kernelILi16ELb1E": ("one_pass", 4 * 16),
@@ -29,7 +32,9 @@ KERNELS = {
 
 def count(asm_text):
     out = {}
-    for frag, (name, cells) in KERNELS.items():
+    for frag, spec in KERNELS.items():
+        name, cells = spec[0], spec[1]
+        every = spec[2] if len(spec) > 2 else 1
         m = re.search(r"\n(_Z\S*%s\S*):[^\n]*\n" % re.escape(frag), asm_text)
         if not m:
             continue
@@ -49,6 +54,13 @@ def count(asm_text):
 
         label = max(blocks, key=lambda lb: valu(blocks[lb]))
         best = valu(blocks[label])
+        if every > 1:
+            # the part after the body's last conditional forward branch runs
+            # once per `every` passes
+            m3 = list(re.finditer(r"\n\s*s_cbranch_\w+\s+\.LBB", blocks[label]))
+            if m3:
+                tail = valu(blocks[label][m3[-1].end():])
+                best = best - tail + tail / every
         # a loop body the compiler closed with a second block that jumps back
         nxt = branch_target(blocks[label])
         if nxt in blocks and nxt != label and re.search(r"s_c?branch\S*\s+%s\b" % re.escape(label), blocks[nxt]):
