@@ -574,6 +574,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
 // the pair is sent to the exact int32 re-run over its whole matrix.  Every
 // other operation only lowers values, and all values stay >= -(gap_open+16384).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s16x2 v2(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
 __device__ __forceinline__ uint32_t u2(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const int ntasks = D->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
     const int ncode = S.ncode, pad = ncode - 1;
-    const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e)), Z2 = v2(0u);
+    const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
     const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
 
     for (;;) {
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
             uint32_t tr[R], Hr[R], rb[R], rj[R], bm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = NEG2; rb[r] = M1; rj[r] = 0u; bm[r] = NEG2; }
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; rb[r] = M1; rj[r] = 0u; bm[r] = NEG2; }
             uint32_t dtb = T0;
 
             uint32_t nwa = wA[0], nwb = wB[0];
@@ -748,10 +749,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                         for (int k = 0; k < 4; ++k) {
                             const int r = 4 * r4 + k;
                             const s16x2 G = __builtin_elementwise_max(v2(Gu) - E2, v2(tu));
-                            const s16x2 H = __builtin_elementwise_max(v2(Hr[r]) - E2, v2(tr[r]));
+                            // the horizontal state is kept as H' = max(H, 0), which
+                            // carries the 0 floor of V: an unsigned saturating
+                            // subtract gives max(H' - e, 0) = max(H - e, 0), so
+                            // H' = max(max(H' - e, 0), t) = max(H, 0) and
+                            // V = max(f, G, H, 0) = max(f, G, H')
+                            const s16x2 H = __builtin_elementwise_max(
+                                __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                                    __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))),
+                                v2(tr[r]));
                             const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp4[k]));
-                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G),
-                                                                      __builtin_elementwise_max(H, Z2));
+                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G), H);
                             const s16x2 tn = V - O2;
                             bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
                             dt = tr[r];
