@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
             int tr[R], Tr[R], Hr[R], THr[R], rb[R], rT[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
+                tr[r] = -o; Tr[r] = 0; Hr[r] = 0; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
             }
             int dtb = -o, dTb = 0;  // V[i0][0]-o and T[i0][0] (border: 0, STOP)
 
@@ -188,14 +188,15 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
                             const int G = cg ? g1 : tu;
                             const int TG = (cg ? TGu : Tu) + ginc;
                             // horizontal gap (:174-187)
-                            const int h1 = Hr[r] - e;
+                            // H is kept as max(H, 0) (an unsigned saturating subtract): V = max(f, G, H, 0)
+                            // = max(f, G, H'); the gap run it carries matters only where H >= 0, where H' = H
+                            const int h1 = (int)__builtin_elementwise_sub_sat((unsigned)Hr[r], (unsigned)e);
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
                             THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             // diagonal and V (:154-157, :190)
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
-                            V = max(V, 0);
                             // pointer order STOP > DIAGONAL > UP > LEFT (:193-208)
                             const int Td = dT + inc;
                             int x = (V == G) ? TG : THr[r];
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
             int tr[R], Tr[R], Hr[R], THr[R], rb[R], rT[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
+                tr[r] = -o; Tr[r] = 0; Hr[r] = 0; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
             }
             int ot = -o, oG = kNeg, oT = 0, oTG = 0;  // this lane's last-row output of the previous step
             int pt = -o, pT = 0;                      // up-values received at the previous step (diagonal)
@@ -354,13 +355,14 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
                             const bool cg = g1 > tu;
                             const int G = cg ? g1 : tu;
                             const int TG = (cg ? TGu : Tu) + ginc;
-                            const int h1 = Hr[r] - e;
+                            // H is kept as max(H, 0) (an unsigned saturating subtract): V = max(f, G, H, 0)
+                            // = max(f, G, H'); the gap run it carries matters only where H >= 0, where H' = H
+                            const int h1 = (int)__builtin_elementwise_sub_sat((unsigned)Hr[r], (unsigned)e);
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
                             THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
-                            V = max(V, 0);
                             const int Td = dT + inc;
                             int x = (V == G) ? TG : THr[r];
                             x = (V == f) ? Td : x;
@@ -1128,7 +1130,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
             int rb[TRACK ? R : 1], rT[TRACK ? R : 1];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0;
+                tr[r] = -o; Tr[r] = 0; Hr[r] = 0; THr[r] = 0;
                 if (TRACK) { rb[r] = INT_MIN; rT[r] = 0; }
             }
             int dtb = -o, dTb = 0;
@@ -1175,13 +1177,14 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
                         const bool cg = g1 > tu;
                         const int G = cg ? g1 : tu;
                         const int TG = (cg ? TGu : Tu) + ginc;
-                        const int h1 = Hr[r] - e;
+                        // H is kept as max(H, 0) (an unsigned saturating subtract): V = max(f, G, H, 0)
+                        // = max(f, G, H'); the gap run it carries matters only where H >= 0, where H' = H
+                        const int h1 = (int)__builtin_elementwise_sub_sat((unsigned)Hr[r], (unsigned)e);
                         const bool ch = h1 > tr[r];
                         Hr[r] = ch ? h1 : tr[r];
                         THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                         const int f = dt + sc;
                         int V = max(max(f, G), Hr[r]);
-                        V = max(V, 0);
                         const int Td = dT + inc;
                         int x = (V == G) ? TG : THr[r];
                         x = (V == f) ? Td : x;
@@ -1318,7 +1321,7 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
             int tr[R], Tr[R], Hr[R], THr[R], rb[R], rT[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                tr[r] = -o; Tr[r] = 0; Hr[r] = kNeg; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
+                tr[r] = -o; Tr[r] = 0; Hr[r] = 0; THr[r] = 0; rb[r] = INT_MIN; rT[r] = 0;
             }
             int ot = -o, oG = kNeg, oT = 0, oTG = 0;  // this lane's last-row output of the previous step
             int pt = -o, pT = 0;                      // up-values received at the previous step (diagonal)
@@ -1356,13 +1359,14 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
                             const bool cg = g1 > tu;
                             const int G = cg ? g1 : tu;
                             const int TG = (cg ? TGu : Tu) + ginc;
-                            const int h1 = Hr[r] - e;
+                            // H is kept as max(H, 0) (an unsigned saturating subtract): V = max(f, G, H, 0)
+                            // = max(f, G, H'); the gap run it carries matters only where H >= 0, where H' = H
+                            const int h1 = (int)__builtin_elementwise_sub_sat((unsigned)Hr[r], (unsigned)e);
                             const bool ch = h1 > tr[r];
                             Hr[r] = ch ? h1 : tr[r];
                             THr[r] = (ch ? THr[r] : Tr[r]) + ginc;
                             const int f = dt + sc;
                             int V = max(max(f, G), Hr[r]);
-                            V = max(V, 0);
                             const int Td = dT + inc;
                             int x = (V == G) ? TG : THr[r];
                             x = (V == f) ? Td : x;
