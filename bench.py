@@ -40,7 +40,7 @@ ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture"
 # on an optimal path back from the maximum), so its work is the cells it
 # evaluates, not the prefix rectangles it decides.
 EVALUATED_IS_WORK = {"reverse"}
-VALU_PER_CELL = {"score16": 4.885, "reverse": 7.469, "rerun_capture": 21.797, "rerun_track": 23.906, "one_pass": 23.906,
+VALU_PER_CELL = {"score16": 4.745, "reverse": 7.469, "rerun_capture": 21.797, "rerun_track": 23.906, "one_pass": 23.906,
                  "global": 38.875}
 
 

@@ -862,18 +862,19 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
                 prof[((size_t)tb * c->ncode + x) * c->ncode + y] = v;
             }
 
-    // residue codes, each sequence 4-byte aligned, then the slots, then a
-    // tail of pad codes
+    // residue codes, each sequence 4-byte aligned and followed by pad codes
+    // up to the end of its next word (seq_span), then the slots, then a tail
+    // of pad codes
     const int32_t ntot = n + nslots;
     std::vector<int64_t> off(std::max(ntot, 1));
     int64_t total = 0;
     for (int32_t i = 0; i < n; ++i) {
         off[i] = total;
-        total += ((offsets[i + 1] - offsets[i]) + 3) & ~int64_t(3);
+        total += seq_span(offsets[i + 1] - offsets[i]);
     }
     for (int32_t q = 0; q < nslots; ++q) {
         off[n + q] = total;
-        total += ((int64_t)slot_cap + 3) & ~int64_t(3);
+        total += seq_span(slot_cap);
     }
     const int64_t pad_off = total;
     total += maxlen + 64;
@@ -1349,21 +1350,23 @@ int launch_batch(swg_server* s, ServeLane& L)
     const int nb = (int)L.batch.size();
     const hipStream_t st = L.lane.stream;
     // query residues into the lane's slots, lengths into the length table
+    // (each with the pad codes of its span: a slot keeps an earlier query's
+    // codes past that)
+    const int64_t stride = seq_span(s->max_qlen);
     int64_t pos = 0;
     std::vector<int32_t> lens(nb);
     for (int k = 0; k < nb; ++k) {
         const auto& q = L.batch[k];
-        std::memcpy(L.h_codes + pos, q->codes.data(), q->codes.size());
         lens[k] = (int32_t)q->codes.size();
+        std::memcpy(L.h_codes + pos, q->codes.data(), q->codes.size());
+        std::memset(L.h_codes + pos + lens[k], c->ncode - 1, (size_t)(seq_span(lens[k]) - lens[k]));
         c->h_len[L.slot_base + k] = lens[k];
-        pos += ((int64_t)s->max_qlen + 3) & ~int64_t(3);
+        pos += stride;
     }
     std::memcpy(L.h_codes + pos, lens.data(), nb * sizeof(int32_t));
-    for (int k = 0; k < nb; ++k) {
-        const int64_t stride = ((int64_t)s->max_qlen + 3) & ~int64_t(3);
-        SWG_HIP(hipMemcpyAsync(c->res.p + c->h_off[L.slot_base + k], L.h_codes + k * stride, lens[k],
-                               hipMemcpyHostToDevice, st));
-    }
+    for (int k = 0; k < nb; ++k)
+        SWG_HIP(hipMemcpyAsync(c->res.p + c->h_off[L.slot_base + k], L.h_codes + k * stride,
+                               (size_t)seq_span(lens[k]), hipMemcpyHostToDevice, st));
     SWG_HIP(hipMemcpyAsync(c->len.p + L.slot_base, L.h_codes + pos, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
     if (c->dist_kind == kDistJukesCantor)
         SWG_HIP(launch_fill_i32(L.lane.undef.p, 0, 1 + nb, st));
@@ -1543,7 +1546,7 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
         if (s->want_stats &&
             (e = hipHostMalloc(reinterpret_cast<void**>(&L->h_stats), nout * 16, 0)) != hipSuccess)
             return bail("hipHostMalloc", e);
-        const size_t codes_bytes = (size_t)max_batch * (((size_t)max_query_len + 3) & ~size_t(3)) + max_batch * 4;
+        const size_t codes_bytes = (size_t)max_batch * (size_t)seq_span(max_query_len) + max_batch * 4;
         if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_codes), codes_bytes, 0)) != hipSuccess)
             return bail("hipHostMalloc", e);
         if ((e = hipHostMalloc(reinterpret_cast<void**>(&L->h_undef), (size_t)(1 + max_batch) * 4, 0)) != hipSuccess)

@@ -40,8 +40,13 @@ enum DistKind : int32_t {
 };
 
 // Read-only data of a context (sequence set + scoring).
+// Bytes a sequence of L residues occupies in the residue store: its codes,
+// then pad codes to the end of the word after its last one, so that every
+// word from 0 to (L + 3) / 4 can be read and holds pad codes past L.
+inline int64_t seq_span(int64_t L) { return ((L + 3) & ~int64_t(3)) + 4; }
+
 struct SeqSet {
-    const uint8_t* res;    // residue codes; each sequence starts 4-byte aligned
+    const uint8_t* res;    // residue codes; each sequence starts 4-byte aligned, seq_span bytes
     const int64_t* off;    // start of each sequence in res
     const int32_t* len;
     int64_t pad_off;       // offset of an all-pad run of length >= maxlen + 16

@@ -602,11 +602,13 @@ __device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, co
 
 // Joint profile for small alphabets: entry (ca, cc) of row r holds the
 // packed pair (score of code ca | score of code cc << 16), i.e. both halves'
-// scores of one lane in one word.  Layout [r][pair] with a row stride of 64
-// words: a lane reads two rows with one ds_read2st64_b32, and lanes reading
-// different pairs of one row hit different banks (up to 32 pairs).
+// scores of one lane in one word.  Layout [r][ca][cc] with 8 words per ca and
+// a row stride of 64 words: the byte offset of a pair in its row is
+// ca * 32 + cc * 4 < 256, so the offsets of four columns are the bytes of
+// (word of A's codes << 5) | (word of B's codes << 2), and a lane reads two
+// rows with one ds_read2st64_b32.
 constexpr int kJointMaxCodes = 8;
-constexpr int kJointRow = 64;  // words per row (>= kJointMaxCodes^2)
+constexpr int kJointRow = 64;  // words per row (= kJointMaxCodes^2)
 
 template <int R>
 __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, const uint8_t* s1, int La, int i0,
@@ -621,7 +623,7 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
         int va = S.prof[c1 * ncode + ca].x, vc = S.prof[c1 * ncode + cc].x;
         va = va < kPad16 ? kPad16 : va;
         vc = vc < kPad16 ? kPad16 : vc;
-        jp[r * kJointRow + pr] = ((uint32_t)va & 0xFFFFu) | ((uint32_t)vc << 16);
+        jp[r * kJointRow + ca * kJointMaxCodes + cc] = ((uint32_t)va & 0xFFFFu) | ((uint32_t)vc << 16);
     }
 }
 
@@ -644,7 +646,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const BlockDesc* D = T.desc;
     const int ntasks = D->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
-    const int ncode = S.ncode, pad = ncode - 1;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
     const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
 
@@ -665,6 +666,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         const uint8_t* s1 = S.res + S.off[a];
         const uint32_t* wA = reinterpret_cast<const uint32_t*>(S.res + (bA >= 0 ? S.off[bA] : S.pad_off));
         const uint32_t* wB = reinterpret_cast<const uint32_t*>(S.res + (bB >= 0 ? S.off[bB] : S.pad_off));
+        // word (L + 3) / 4 of a column sequence holds only pad codes (and the
+        // words before it pad codes past L): reading word min(k, lastA)
+        // gives the pad code for every column past the sequence's end
+        const int lastA = (LbA + 3) >> 2, lastB = (LbB + 3) >> 2;
         const int ncols = (maxLb + 3) & ~3;
 
         for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2((int)T0, (int)NEG2);
@@ -688,9 +693,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
             // a column's profile words (R/4 int4) are loaded one column ahead:
             // joint entries (ca, cc) of the R rows, or the entries of ca and
             // cc interleaved
+            // (JOINT: ca = the pair's byte offset in a row, cc unused)
             auto load_col = [&](int ca, int cc, int4* out) {
                 if (JOINT) {
-                    const uint32_t* pj = reinterpret_cast<const uint32_t*>(lprof) + (ca * ncode + cc);
+                    const uint32_t* pj =
+                        reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lprof) + ca);
 #pragma unroll
                     for (int r4 = 0; r4 < R / 4; ++r4)
                         out[r4] = make_int4((int)pj[(4 * r4) * kJointRow], (int)pj[(4 * r4 + 1) * kJointRow],
@@ -702,30 +709,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                     for (int r8 = 0; r8 < R / 8; ++r8) { out[2 * r8] = pa[r8]; out[2 * r8 + 1] = pc[r8]; }
                 }
             };
+            // JOINT: the byte offsets of the word's four columns
+            auto joint_offsets = [](uint32_t wa, uint32_t wb) { return (wa << 5) | (wb << 2); };
             int4 cur[R / 4];
-            load_col(LbA >= 1 ? (int)(nwa & 0xFFu) : pad, LbB >= 1 ? (int)(nwb & 0xFFu) : pad, cur);
+            uint32_t nxo = JOINT ? joint_offsets(nwa, nwb) : 0u;
+            if (JOINT) load_col((int)(nxo & 0xFFu), 0, cur);
+            else load_col((int)(nwa & 0xFFu), (int)(nwb & 0xFFu), cur);
             // residue words: current block, next block (needed by the
             // look-ahead load of the last column) and the one after
-            uint32_t n2a = wA[1], n2b = wB[1];
+            uint32_t n2a = wA[min(1, lastA)], n2b = wB[min(1, lastB)];
             for (int j = 1; j <= ncols; j += 4) {
                 const uint32_t cwa = nwa, cwb = nwb;
+                const uint32_t cxo = nxo;
                 int2 cb[4] = {nb0, nb1, nb2, nb3};
                 nwa = n2a;
                 nwb = n2b;
-                n2a = wA[(j + 7) >> 2];
-                n2b = wB[(j + 7) >> 2];
+                if (JOINT) nxo = joint_offsets(nwa, nwb);
+                n2a = wA[min((j + 7) >> 2, lastA)];
+                n2b = wB[min((j + 7) >> 2, lastB)];
                 const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
                 nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
+                // keep these loads at the top of the step: the whole step hides
+                // their latency (the boundary row comes from HBM)
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
                     const int sh = 8 * ((q + 1) & 3);
-                    int ca = (int)(((q < 3 ? cwa : nwa) >> sh) & 0xFFu);
-                    int cc = (int)(((q < 3 ? cwb : nwb) >> sh) & 0xFFu);
-                    ca = jj + 1 > LbA ? pad : ca;
-                    cc = jj + 1 > LbB ? pad : cc;
                     int4 nx[R / 4];
-                    load_col(ca, cc, nx);
+                    if (JOINT) {
+                        load_col((int)(((q < 3 ? cxo : nxo) >> sh) & 0xFFu), 0, nx);
+                    } else {
+                        load_col((int)(((q < 3 ? cwa : nwa) >> sh) & 0xFFu),
+                                 (int)(((q < 3 ? cwb : nwb) >> sh) & 0xFFu), nx);
+                    }
                     uint32_t tu = (uint32_t)cb[q].x, Gu = (uint32_t)cb[q].y;
                     uint32_t dt = dtb;
                     dtb = tu;
