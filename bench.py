@@ -138,7 +138,7 @@ def main():
 
     for k in range(args.warmup):
         p, c, ms, strip, var = run_block(k)
-        log("warmup block %d: %.3e cells, kernel %.1f ms (strip %d, %s)" % (k, c, ms, strip, VNAME[var]))
+        log("warmup block %d: %.3e cells, kernel %.1f ms (int32 strip %d, %s)" % (k, c, ms, strip, VNAME[var]))
 
     torch.cuda.synchronize()
     tot_cells, tot_kms, tot_pairs = 0.0, 0.0, 0
@@ -156,7 +156,7 @@ def main():
             ksum = [[name, 0.0, 0.0, 0.0, 0] for name, _, _, _ in kt]
         for acc, (name, kms, kc, krun) in zip(ksum, kt):
             acc[1] += kms; acc[2] += kc; acc[3] += krun; acc[4] += 1
-        log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (strip %d, %s)" % (
+        log("step %d: %d pairs %.3e cells kernel %.1f ms -> %.1f GCUPS (int32 strip %d, %s)" % (
             k, p, c, ms, c / ms / 1e6, strip, VNAME[var]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
@@ -275,7 +275,7 @@ def main():
             "sequences": n,
             "pairs_per_step": tot_pairs // max(args.steps, 1),
             "cells_per_step": tot_cells / max(args.steps, 1),
-            "strip_rows": strip,
+            "int32_strip_rows": strip,  # the int32 (re-run) kernel's; the int16 score pass uses 24
         },
         "kernel_gcups": round(kernel_gcups, 2),
         "variant": VNAME[var],
