@@ -85,7 +85,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(pairs shard by the reference's block checkerboard across GPUs, see DESIGN.md)" % args.gpus,
+              "(across GPUs, each rank would compute its blocks of the reference's P x P block "
+              "checkerboard on its own GPU, one process per GPU)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
