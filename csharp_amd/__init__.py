@@ -176,6 +176,24 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
+def _out_ld(out, dtype, rows, cols):
+    """Row pitch (elements) of a caller-supplied 2-D output array: it must
+    hold TDistance values with unit column stride and at least rows x cols
+    elements (a row or column slice of a larger matrix is fine)."""
+    if not isinstance(out, np.ndarray) or out.ndim != 2:
+        raise ValueError("out must be a 2-D numpy array")
+    if out.dtype != np.dtype(dtype):
+        raise ValueError("out has dtype %s, the context's TDistance is %s" % (out.dtype, np.dtype(dtype)))
+    if out.shape[0] < rows or out.shape[1] < cols:
+        raise ValueError("out of shape %s is smaller than the block (%d x %d)" % (out.shape, rows, cols))
+    if rows > 0 and cols > 0:
+        if out.strides[1] != out.itemsize or out.strides[0] % out.itemsize or out.strides[0] < cols * out.itemsize:
+            raise ValueError("out must have unit column stride and a row pitch of at least the block width")
+        if not out.flags.writeable:
+            raise ValueError("out is read-only")
+    return out.strides[0] // out.itemsize if out.shape[0] > 0 else cols
+
+
 # ---------------------------------------------------------------- scoring
 def load_scoring_matrix(name_or_path):
     """ScoringMatrix.Load (Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:145-196):
@@ -343,11 +361,11 @@ class Aligner:
         rows, cols = row_end - row_start + 1, col_end - col_start + 1
         if out is None:
             out = np.zeros((max(rows, 0), max(cols, 0)), dtype=self.dtype)
+        ld = _out_ld(out, self.dtype, rows, cols)
         st = np.zeros((max(rows, 0), max(cols, 0), 4), dtype=np.int32) if stats else None
         done = ctypes.c_int64(0)
         _check(lib().swg_compute_block(self._h, row_start, row_end, col_start, col_end, int(bool(is_diagonal)),
-                                       _ptr(out), out.shape[1] if out.ndim == 2 else cols,
-                                       _ptr(st) if stats else None, ctypes.byref(done)))
+                                       _ptr(out), ld, _ptr(st) if stats else None, ctypes.byref(done)))
         return out, st, done.value
 
     def compute_block_device(self, row_start, row_end, col_start, col_end, is_diagonal, dev_out_ptr, ld,

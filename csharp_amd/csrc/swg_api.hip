@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -165,7 +166,6 @@ struct swg_ctx {
     int32_t dist_kind = kDistFraction;  // DistKind
     bool bio_helpers = false;       // SWMS / NW ComputeDistance helpers (SWG_PROGRAM_SWMS or global mode)
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
-    bool lpp_ok = false;            // non-negative penalties (pre-column cells of the 16-lane kernel)
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
 
     // sequences
@@ -455,11 +455,11 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     const bool kimura = c->dist_kind == kDistKimura;
     const bool words = c->dist_kind != kDistFraction;  // distance_kernel after the alignment kernels
     const int brows = g.r1 - g.r0 + 1, bcols = g.c1 - g.c0 + 1;
+    desc[0].rows = brows;
     if (words) {
         SWG_HIP(B.aux.reserve(std::max<size_t>((size_t)(kimura ? 2 : 1) * brows * bcols, 1)));
         desc[0].aux = B.aux.p;
         desc[0].aux_ld = bcols;
-        desc[0].rows = brows;
     }
     SWG_HIP(B.desc.reserve(2));
     SWG_HIP(B.queue.reserve(kQueueWords));
@@ -478,6 +478,10 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     ReverseSet RV{};
     RerunSet RT{}, RC{}, RB{};
     const int sort_grid = std::max(1, c->num_cu * 2 / ln.share);
+    // the distance and diagonal kernels stride over the block from its
+    // descriptor: a fixed grid, so that a captured graph fits every block size
+    const int pair_grid = std::max(1, c->num_cu * 4 / ln.share);
+    const int tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
     if (v == kVarI16) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
@@ -525,6 +529,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         };
         hipError_t e = mark ? launch_stamp(c->kt_stamp.p, 0, st) : hipSuccess;
         if (e == hipSuccess) e = launch_fill_i32(B.queue.p, 0, kQueueWords, st);
+        if (e == hipSuccess) e = launch_zero_diagonal(tdist, B.desc.p, pair_grid, st);
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
@@ -569,7 +574,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
             }
         }
         if (words) {
-            if (e == hipSuccess) e = launch_distance(S, B.desc.p, brows, bcols, st);
+            if (e == hipSuccess) e = launch_distance(S, B.desc.p, pair_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_DISTANCE);
         }
         nids = nk;
@@ -588,7 +593,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         append_key(key, RV); append_key(key, n3); append_key(key, RB); append_key(key, n4);
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
-        append_key(key, timing); append_key(key, c->kt_stamp.p);
+        append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -617,6 +622,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
         c->kt_meta2 = v == kVarI16 ? P2.meta : nullptr;
     }
     SWG_HIP(hipEventRecord(ln.ev0, st));
+    if (plan.ntasks == 0 && g.diag_lower) SWG_HIP(launch_zero_diagonal(tdist, B.desc.p, pair_grid, st));
     if (plan.ntasks > 0) {
         if (v == kVarPlain) {
             SWG_HIP(launch_all(timing));
@@ -695,14 +701,10 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant,
         *variant = kVarI32;
         return SWG_OK;
     }
-    if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) {
-        if (!c->lpp_ok) return fail(SWG_E_UNSUPPORTED, "the 16-lanes-per-pair kernel needs non-negative gap penalties");
-        *variant = kVarLpp;
-        return SWG_OK;
-    }
+    if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) { *variant = kVarLpp; return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *variant = kVarI32; return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *variant = kVarI16; return SWG_OK; }
-    if (c->lpp_ok && thin_block(c, ln, g)) { *variant = kVarLpp; return SWG_OK; }
+    if (thin_block(c, ln, g)) { *variant = kVarLpp; return SWG_OK; }
     if (!c->int16_ok) { *variant = kVarI32; return SWG_OK; }
     const int64_t key = tune_key(c, g);
     TuneEntry* t = nullptr;
@@ -952,6 +954,12 @@ int swg_create(swg_ctx** out, const swg_params* p)
 
     if (!is_integral(p->gap_open) || !is_integral(p->gap_ext))
         return fail(SWG_E_NONINTEGRAL, "gap penalties must be integers below 32768 for the integer kernels");
+    // penalties are costs subtracted from the score (SmithWatermanGotoh.cs:159-187);
+    // the kernels keep the horizontal gap state as max(H, 0) with an unsigned
+    // saturating subtract of gap_ext and run pad rows/columns as DP borders,
+    // both of which need gap_open >= 0 and gap_ext >= 0
+    if (p->gap_open < 0.0f || p->gap_ext < 0.0f)
+        return fail(SWG_E_UNSUPPORTED, "negative gap penalties are not supported (gap_open and gap_ext must be >= 0)");
     for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k)
         if (!is_integral(p->scores[k]))
             return fail(SWG_E_NONINTEGRAL, "scoring matrix entries must be integers below 32768 for the integer kernels");
@@ -977,9 +985,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
         // the int16 score pass holds score + gap_open and the pad score in int16
         int maxabs = 0;
         for (int k = 0; k < SWG_MATRIX_SIZE * SWG_MATRIX_SIZE; ++k) maxabs = std::max(maxabs, std::abs(c->iscores[k]));
-        c->int16_ok = maxabs + std::abs(c->gap_open) <= 8192 && std::abs(c->gap_ext) <= 8192 && c->gap_open >= 0 &&
-                      c->gap_ext >= 0;
-        c->lpp_ok = c->gap_open >= 0 && c->gap_ext >= 0;
+        c->int16_ok = maxabs + c->gap_open <= 8192 && c->gap_ext <= 8192;
     }
     c->device = p->device;
 
@@ -1278,7 +1284,6 @@ void swg_destroy(swg_ctx* c)
 
 #include <condition_variable>
 #include <memory>
-#include <mutex>
 #include <thread>
 #include <unordered_map>
 
@@ -1337,8 +1342,13 @@ namespace {
 
 int64_t query_footprint(const swg_server* s, int32_t qlen)
 {
-    // distances + stats + best cells + bounded-pass tiles per panel pair, + residues
-    const int64_t per_pair = elem_size(s->c) + (s->want_stats ? 16 : 0) + 4 + 9;
+    // device bytes per panel pair: the distance and statistics, the two-pass
+    // schedule's best cell (4), best score (2), start cell (4) and two sort
+    // slots (2 x 4), the count words of the word-mode distances (Jukes-Cantor
+    // one plane, Kimura2 two, 4 bytes each) and ~1 byte of task tiles; plus
+    // the query's residues
+    const int planes = s->c->dist_kind == kDistKimura ? 2 : s->c->dist_kind == kDistFraction ? 0 : 1;
+    const int64_t per_pair = elem_size(s->c) + (s->want_stats ? 16 : 0) + 18 + 4 * planes + 1;
     return per_pair * s->npanel + qlen;
 }
 
@@ -1384,7 +1394,7 @@ int launch_batch(swg_server* s, ServeLane& L)
     BlockGeom g;
     g.r0 = L.slot_base; g.r1 = L.slot_base + nb - 1; g.c0 = 0; g.c1 = s->npanel - 1; g.diag_lower = false;
     c->schedule = c->nw                                  ? SWG_SCHED_PERSISTENT_I32
-                : (c->lpp_ok && thin_block(c, L.lane, g)) ? SWG_SCHED_PERSISTENT_LPP16
+                : thin_block(c, L.lane, g)                ? SWG_SCHED_PERSISTENT_LPP16
                 : c->int16_ok                            ? SWG_SCHED_PERSISTENT_I16
                                                          : SWG_SCHED_PERSISTENT_I32;
     int64_t pairs = 0;
@@ -1444,6 +1454,8 @@ void serve_loop(swg_server* s)
                 s->error = SWG_E_DEVICE;
                 s->error_msg = std::string("batch failed: ") + hipGetErrorString(q);
                 for (auto& r : L.batch) r->state = SWG_E_DEVICE;
+                s->used -= L.footprint;  // the failed batch's HBM share is free again
+                L.footprint = 0;
                 L.batch.clear();
                 L.active = false;
                 s->cv.notify_all();

@@ -205,9 +205,13 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode);
 int sw_blocks_per_cu(int strip_rows, bool nw);
 int score16_blocks_per_cu(int strip_rows, int ncode);
 int nw_blocks_per_cu(int strip_rows);
-// Distances of a block from the count words in aux (Jukes-Cantor, Kimura2):
-// desc[0] is the output desc (rows, aux_ld = columns).
-hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int cols, hipStream_t stream);
+// Distances of a block from the count words in aux (Jukes-Cantor, Kimura2,
+// MinMaxNormScore): desc[0] is the output desc (rows, aux_ld = columns); the
+// grid (256-thread workgroups) strides over the block, whatever its size.
+hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int grid, hipStream_t stream);
+// (i, i) := 0 (distance and statistics) when desc[0] is a lower-mirror
+// diagonal block (desc[0].rows entries); nothing otherwise.
+hipError_t launch_zero_diagonal(int tdist, const BlockDesc* desc, int grid, hipStream_t stream);
 // Kernel-timing stamp: p[k] = the device's constant-rate wall clock, taken
 // when every earlier kernel on the stream has finished (a node of the block's
 // graph like any other kernel).

@@ -1690,16 +1690,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 //   Salsa.NeedlemanWunschTPL/Program.cs:573-581: 2*open + (|si|+|sj|)*ext)
 // in binary64 (built with -ffp-contract=off), then Int16 as (short)(d * 32767)
 // or Double as is (the Double builds have no MinMaxNormScore case: 0).
-// One workgroup per 256 columns of a row.
+// A fixed grid strides over the block's rows x columns, both read from the
+// descriptor, so that one captured launch serves blocks of every size.
 constexpr int kDistThreads = 256;
+
+__device__ __forceinline__ void distance_of_pair(const SeqSet& S, const BlockDesc& D, int row, int col);
 
 __global__ __launch_bounds__(kDistThreads) void distance_kernel(SeqSet S, const BlockDesc* desc)
 {
     const BlockDesc D = desc[0];
-    const int ntile = (D.aux_ld + kDistThreads - 1) / kDistThreads;
-    const int row = blockIdx.x / ntile;
-    const int col = (blockIdx.x % ntile) * kDistThreads + threadIdx.x;
-    if (row >= D.rows || col >= D.aux_ld) return;
+    const int64_t n = (int64_t)D.rows * D.aux_ld;
+    for (int64_t k = (int64_t)blockIdx.x * kDistThreads + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * kDistThreads)
+        distance_of_pair(S, D, (int)(k / D.aux_ld), (int)(k % D.aux_ld));
+}
+
+// (i, i) of a lower-mirror diagonal block: the reference leaves it at the
+// matrix's initial 0 (Salsa.SmithWatermanMPI/Program.cs:209-231 aligns j < i
+// only); written here so that a reused device buffer holds no stale value.
+__global__ __launch_bounds__(kDistThreads) void zero_diagonal_kernel(int tdist, const BlockDesc* desc)
+{
+    const BlockDesc D = desc[0];
+    if (!D.mirror) return;
+    for (int64_t i = (int64_t)blockIdx.x * kDistThreads + threadIdx.x; i < D.rows;
+         i += (int64_t)gridDim.x * kDistThreads) {
+        const int64_t pos = i * D.ld + i;
+        if (tdist) static_cast<double*>(D.out)[pos] = 0.0;
+        else static_cast<short*>(D.out)[pos] = 0;
+        if (D.stats) reinterpret_cast<int4*>(D.stats)[pos] = make_int4(0, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void distance_of_pair(const SeqSet& S, const BlockDesc& D, int row, int col)
+{
     const int a = (int)D.r0 + row, b = (int)D.c0 + col;
     if (D.mirror && b >= a) return;  // lower-mirror block: pairs j < i only
     const int64_t k = (int64_t)row * D.aux_ld + col;
@@ -1995,12 +2018,17 @@ int sw_blocks_per_cu(int strip_rows, bool /*nw*/)
     return err == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int rows, int cols, hipStream_t st)
+hipError_t launch_distance(const SeqSet& S, const BlockDesc* desc, int grid, hipStream_t st)
 {
-    const int64_t blocks = (int64_t)rows * ((cols + kDistThreads - 1) / kDistThreads);
-    if (blocks <= 0) return hipSuccess;
-    if (blocks > (int64_t(1) << 31) / kDistThreads) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(distance_kernel, dim3((unsigned)blocks), dim3(kDistThreads), 0, st, S, desc);
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(distance_kernel, dim3((unsigned)grid), dim3(kDistThreads), 0, st, S, desc);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_diagonal(int tdist, const BlockDesc* desc, int grid, hipStream_t st)
+{
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zero_diagonal_kernel, dim3((unsigned)grid), dim3(kDistThreads), 0, st, tdist, desc);
     return hipGetLastError();
 }
 

@@ -111,7 +111,8 @@ typedef struct swg_pair_stats {
 } swg_pair_stats;
 
 /* Create a context: validates the scoring matrix and gap penalties
- * (integral, as the integer kernels require) and selects the device.
+ * (integral, as the integer kernels require: SWG_E_NONINTEGRAL; negative
+ * penalties: SWG_E_UNSUPPORTED) and selects the device.
  * Replaces `new SmithWatermanGotoh(type)` + `ScoringMatrix.Load(name)`
  * (Salsa.SmithWatermanMPI/Program.cs:206-207). */
 int  swg_create(swg_ctx** ctx, const swg_params* p);
@@ -216,9 +217,10 @@ typedef struct swg_server swg_server;
  * the panel's symbols and every symbol the scoring matrix scores.  Up to
  * `lanes` batches of at most `max_batch` queries (each <= max_query_len
  * residues) are in flight at once; a batch is admitted only while the summed
- * per-query device footprint (distances, statistics, best cells and task
- * tiles: ~(sizeof(TDistance)+16+13) bytes per panel pair) stays within
- * hbm_budget_bytes. */
+ * per-query device footprint stays within hbm_budget_bytes: per panel pair
+ * sizeof(TDistance), 16 with statistics, 18 for the two-pass schedule's best
+ * cells and sort slots, 4 per count-word plane (Jukes-Cantor 1, Kimura2 2)
+ * and ~1 for the task tiles. */
 int  swg_serve_create(swg_server** server, const swg_params* p, const uint8_t* panel_residues,
                       const int64_t* panel_offsets, int32_t npanel, int32_t max_query_len,
                       int32_t max_batch, int32_t lanes, int64_t hbm_budget_bytes, int32_t want_stats);

@@ -27,6 +27,34 @@ class OracleResult(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+def cpu_info():
+    """Host CPUs this process may run on: nproc (os.cpu_count), the affinity
+    mask, the cgroup CPU quota (cpu.max, v2; cfs_quota_us, v1) if one is set,
+    and `usable` = the affinity count capped by the quota."""
+    nproc = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        try:
+            q = int(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
+            period = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / period
+        except (OSError, ValueError):
+            quota = None
+    usable = affinity if quota is None else max(1, min(affinity, int(quota + 0.5)))
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota_cpus": quota, "usable": usable}
+
+
+def cpu_threads():
+    """Oracle worker threads: one per usable host CPU (cpu_info)."""
+    return cpu_info()["usable"]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 

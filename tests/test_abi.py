@@ -60,3 +60,15 @@ def test_null_arguments():
     assert L.swg_last_error()
     assert L.swg_synchronize(None) == ca.SWG_E_ARGUMENT
     L.swg_destroy(None)
+
+
+def test_negative_gap_penalties_rejected(ednafull):
+    # checked before any device call: penalties are costs the reference
+    # subtracts (SmithWatermanGotoh.cs:159-187); the kernels need them >= 0
+    for o, e in ((-1, 4), (16, -1), (-16, -4)):
+        with pytest.raises(ca.SwgError) as ei:
+            ca.Aligner(ednafull, o, e)
+        assert ei.value.code == ca.SWG_E_UNSUPPORTED
+        with pytest.raises(ca.SwgError) as ei:
+            ca.Aligner(ednafull, o, e, mode=ca.MODE_NW_GLOBAL)
+        assert ei.value.code == ca.SWG_E_UNSUPPORTED

@@ -3,7 +3,6 @@ uses a joint profile of both packed pairs' scores up to 8 codes (pad code
 included) and a per-code profile beyond, here up to the 16 IUPAC nucleotide
 symbols of EDNAFULL; every size must give the oracle's distances and
 statistics bit for bit."""
-import os
 
 import numpy as np
 import pytest
@@ -13,7 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(16, os.cpu_count() or 1)
+THREADS = O.cpu_threads()
 IUPAC = b"ACGTRYKMSWBDHVNU"
 
 

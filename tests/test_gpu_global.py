@@ -7,7 +7,6 @@ nucleic = identity over the aligned region [first non-gap, last non-gap]
 global aligner ships only as a compiled library, so its tie order is this
 oracle's own definition, pinned against brute-force enumeration).
 """
-import os
 import sys
 
 import numpy as np
@@ -18,7 +17,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(16, os.cpu_count() or 1)
+THREADS = O.cpu_threads()
 
 
 def say(msg):

@@ -11,7 +11,6 @@ uses the device library's log, the oracle the host C library's, and neither
 is correctly rounded, so a Double distance may differ from the oracle's by a
 few units in the last place: tolerance |got - want| <= 2^-50 |want| (4 ulp).
 """
-import os
 import sys
 
 import numpy as np
@@ -22,7 +21,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(16, os.cpu_count() or 1)
+THREADS = O.cpu_threads()
 DISTS = {"kimura2": ca.DIST_KIMURA2, "jukes_cantor": ca.DIST_JUKES_CANTOR}
 
 

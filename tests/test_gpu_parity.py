@@ -4,7 +4,6 @@ Bit-exact comparison of Int16/Double distances and of the per-pair
 statistics (score, aligned length, identity/similarity count) on seeded
 inputs, over every block kind and schedule, plus edge cases.
 """
-import os
 import sys
 
 import numpy as np
@@ -15,7 +14,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(16, os.cpu_count() or 1)
+THREADS = O.cpu_threads()
 
 
 def say(msg):

@@ -10,7 +10,6 @@ Int16 distances and the statistics.  (Both programs align with the closed
 Bio.dll; the alignment here is this build's, so the reference-level parity
 of these programs stays unpinned.)
 """
-import os
 import sys
 
 import numpy as np
@@ -21,7 +20,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(16, os.cpu_count() or 1)
+THREADS = O.cpu_threads()
 
 
 def say(msg):

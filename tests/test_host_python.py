@@ -53,3 +53,24 @@ def test_serving_accounting():
     assert r["admitted_concurrency_mean"] == round(10 * 25.0 / span_ms, 2)
     assert r["admitted_concurrency_max"] == 3
     assert r["gcups"] == round(10 * 1e9 / (span_ms / 1e3) / 1e9, 1)
+
+
+def test_out_array_validation():
+    # compute_block(out=...) takes the row pitch from the array itself and
+    # refuses arrays that would place or overrun the block wrongly
+    from csharp_amd import _out_ld
+    full = np.zeros((50, 80), dtype=np.int16)
+    assert _out_ld(full, np.int16, 50, 80) == 80
+    assert _out_ld(full[10:20, 30:70], np.int16, 10, 40) == 80  # a window of a larger matrix
+    with pytest.raises(ValueError):
+        _out_ld(full, np.float64, 10, 10)          # wrong TDistance
+    with pytest.raises(ValueError):
+        _out_ld(full[:5, :5], np.int16, 10, 10)    # too small
+    with pytest.raises(ValueError):
+        _out_ld(full[:, ::2], np.int16, 10, 10)    # no unit column stride
+    with pytest.raises(ValueError):
+        _out_ld(np.zeros(100, np.int16), np.int16, 10, 10)  # not 2-D
+    ro = np.zeros((10, 10), np.int16)
+    ro.flags.writeable = False
+    with pytest.raises(ValueError):
+        _out_ld(ro, np.int16, 10, 10)
