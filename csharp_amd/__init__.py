@@ -45,7 +45,9 @@ ABI_SYMBOLS = (
     "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
     "swg_serve_create", "swg_serve_submit", "swg_serve_wait", "swg_serve_times", "swg_serve_release",
     "swg_serve_stats", "swg_serve_destroy", "swg_set_option", "swg_last_kernel_times",
+    "swg_compute_block_device_noalloc",
 )
+NOALLOC_SCRATCH_BYTES = 64 << 20  # SWG_NOALLOC_SCRATCH_BYTES
 HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
     "salsa_write_matrix", "salsa_write_swms_block", "salsa_load_scoring_matrix",
@@ -107,6 +109,8 @@ def lib():
         L.swg_set_sequences.argtypes = [vp, vp, vp, i32]
         L.swg_compute_block.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp, ctypes.POINTER(i64)]
         L.swg_compute_block_device.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp, ctypes.POINTER(i64)]
+        L.swg_compute_block_device_noalloc.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp,
+                                                       ctypes.POINTER(i64)]
         L.swg_synchronize.argtypes = [vp]
         L.swg_set_schedule.argtypes = [vp, i32, i32]
         L.swg_last_kernel_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -375,6 +379,18 @@ class Aligner:
                                               int(bool(is_diagonal)), ctypes.c_void_p(dev_out_ptr), ld,
                                               ctypes.c_void_p(dev_stats_ptr) if dev_stats_ptr else None,
                                               ctypes.byref(done)))
+        return done.value
+
+    def compute_block_device_noalloc(self, row_start, row_end, col_start, col_end, is_diagonal, dev_out_ptr, ld,
+                                     dev_stats_ptr=None):
+        """swg_compute_block_device_noalloc: as compute_block_device, with
+        every device buffer beyond the output taken from the library's static
+        device scratch (no allocation)."""
+        done = ctypes.c_int64(0)
+        _check(lib().swg_compute_block_device_noalloc(self._h, row_start, row_end, col_start, col_end,
+                                                      int(bool(is_diagonal)), ctypes.c_void_p(dev_out_ptr), ld,
+                                                      ctypes.c_void_p(dev_stats_ptr) if dev_stats_ptr else None,
+                                                      ctypes.byref(done)))
         return done.value
 
     def synchronize(self):

@@ -51,6 +51,38 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Device scratch of the allocation-free entry point
+// (swg_compute_block_device_noalloc): one file-scope array per GPU, reserved
+// when the code object is loaded and held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host the name is only a
+// stand-in): swg_create resolves the array's device address on each device
+// once (hipGetSymbolAddress, with the device current) into g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[SWG_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolve this device's scratch address (the device must be current).
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(SWG_E_ARGUMENT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SWG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SWG_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return SWG_OK;
+}
+
+unsigned char* scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 // Kernel variants a block can run with.
 enum Variant : int {
     kVarPlain = 0,     // sw_kernel<16, false>: natural order, static assignment
@@ -996,6 +1028,10 @@ int swg_create(swg_ctx** out, const swg_params* p)
     };
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return bail(e, "hipSetDevice");
+    {
+        const int rc = scratch_setup(c->device);
+        if (rc) { std::string m = g_last_error; swg_destroy(c); return fail(rc, m); }
+    }
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, c->device);
     if (e != hipSuccess) return bail(e, "hipGetDeviceProperties");
@@ -1041,6 +1077,152 @@ int swg_synchronize(swg_ctx* c)
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, c->main.ev0, c->main.ev1) == hipSuccess) c->last_kernel_ms = ms;
     return take_undefined(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Carve `bytes` (256-byte aligned) from the scratch array; null when it does
+// not fit.
+unsigned char* carve(unsigned char* base, size_t* used, size_t bytes)
+{
+    const size_t at = *used;
+    const size_t want = (bytes + 255) & ~size_t(255);
+    if (want > SWG_NOALLOC_SCRATCH_BYTES - at) return nullptr;
+    *used = at + want;
+    return base + at;
+}
+
+// swg_compute_block_device_noalloc: the 16-lanes-per-pair int32 kernel
+// (statistics carried, one pass) launched directly on the context stream,
+// with the block's task lists, descriptor, work counters, count words and
+// row-boundary scratch all carved from the device's scratch array.
+int enqueue_block_noalloc(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
+                          void* dev_out, int64_t ld, int32_t* dev_stats, int64_t* pairs_done)
+{
+    if (!c->res.p) return fail(SWG_E_ARGUMENT, "swg_set_sequences has not been called");
+    if (c->nw) return fail(SWG_E_UNSUPPORTED, "the allocation-free entry point runs the local alignment only");
+    if (r0 < 0 || c0 < 0 || r1 < r0 - 1 || c1 < c0 - 1 || r1 >= c->nseq || c1 >= c->nseq)
+        return fail(SWG_E_ARGUMENT, "block range outside the sequence set");
+    BlockGeom g;
+    g.r0 = r0; g.r1 = r1; g.c0 = c0; g.c1 = c1;
+    g.diag_lower = is_diag && c->p.diag_mode == SWG_DIAG_LOWER_MIRROR;
+    if (g.diag_lower && (r0 != c0 || r1 != c1))
+        return fail(SWG_E_ARGUMENT, "a diagonal block must have equal row and column ranges");
+    if (ld < (int64_t)(c1 - c0 + 1)) return fail(SWG_E_ARGUMENT, "out_ld smaller than the block width");
+    if (r1 < r0 || c1 < c0) {
+        if (pairs_done) *pairs_done = 0;
+        return SWG_OK;
+    }
+    unsigned char* base = scratch_ptr(c->device);
+    if (!base) return fail(SWG_E_DEVICE, "scratch address of this device was not resolved");
+    int rc;
+    Lane& ln = c->main;
+    BlockBufs& B = next_bufs(ln, &rc);  // waits until this set's host task lists are free again
+    if (rc) return rc;
+    const int v = kVarLpp;
+    int s1, s2;
+    variant_strips(c, v, &s1, &s2);
+    Plan plan;
+    plan_tiles(c, g, true, kLppTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
+
+    // scratch layout: descriptor, work counters, task lists, count words,
+    // then as many workgroups' boundary rows as fit (at most one per
+    // resident workgroup)
+    const bool kimura = c->dist_kind == kDistKimura;
+    const bool words = c->dist_kind != kDistFraction;
+    const int brows = g.r1 - g.r0 + 1, bcols = g.c1 - g.c0 + 1;
+    size_t used = 0;
+    auto* d_desc = reinterpret_cast<BlockDesc*>(carve(base, &used, sizeof(BlockDesc)));
+    auto* d_queue = reinterpret_cast<int32_t*>(carve(base, &used, kQueueWords * sizeof(int32_t)));
+    auto* d_task_row = reinterpret_cast<int32_t*>(carve(base, &used, B.h_task_row.size() * sizeof(int32_t)));
+    auto* d_task_tile = reinterpret_cast<int32_t*>(carve(base, &used, B.h_task_tile.size() * sizeof(int32_t)));
+    auto* d_tile_col = reinterpret_cast<int32_t*>(carve(base, &used, B.h_tile_col.size() * sizeof(int32_t)));
+    auto* d_tile_max = reinterpret_cast<int32_t*>(carve(base, &used, B.h_tile_maxlen.size() * sizeof(int32_t)));
+    uint32_t* d_aux = nullptr;
+    if (words)
+        d_aux = reinterpret_cast<uint32_t*>(
+            carve(base, &used, (size_t)(kimura ? 2 : 1) * (size_t)brows * (size_t)bcols * sizeof(uint32_t)));
+    // lpp16_kernel touches boundary columns 1 .. maxlen + 26, one int4 per
+    // pair of its wave (kLppTile pairs)
+    const int64_t stride = (int64_t)(c->maxlen + 32) * kLppTile;
+    const size_t slot_bytes = (size_t)stride * sizeof(int4);
+    const bool fits = d_desc && d_queue && d_task_row && d_task_tile && d_tile_col && d_tile_max && (!words || d_aux);
+    const int64_t room = fits ? (int64_t)((SWG_NOALLOC_SCRATCH_BYTES - used) / slot_bytes) : 0;
+    if (room < 1)
+        return fail(SWG_E_CAPACITY, "block too large for the allocation-free entry point: its task lists" +
+                                        std::string(words ? ", count words" : "") +
+                                        " and one workgroup's boundary rows exceed the " +
+                                        std::to_string(SWG_NOALLOC_SCRATCH_BYTES >> 20) + " MiB device scratch");
+    int4* d_bnd = reinterpret_cast<int4*>(base + used);
+    const int per_cu = lpp16_blocks_per_cu(s1, c->ncode);
+    if (per_cu <= 0) return fail(SWG_E_DEVICE, "occupancy query failed");
+    const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(room, (int64_t)per_cu * c->num_cu));
+
+    BlockDesc desc = {};
+    desc.ntasks = plan.ntasks;
+    desc.mirror = g.diag_lower ? 1 : 0;
+    desc.r0 = g.r0; desc.c0 = g.c0; desc.ld = ld;
+    desc.out = dev_out; desc.stats = dev_stats;
+    desc.rows = brows;
+    if (words) { desc.aux = d_aux; desc.aux_ld = bcols; }
+
+    const hipStream_t st = ln.stream;
+    SWG_HIP(hipMemcpyAsync(d_desc, &desc, sizeof(desc), hipMemcpyHostToDevice, st));
+    auto put = [&](int32_t* dst, const std::vector<int32_t>& h) -> hipError_t {
+        return h.empty() ? hipSuccess
+                         : hipMemcpyAsync(dst, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    };
+    SWG_HIP(put(d_task_row, B.h_task_row));
+    SWG_HIP(put(d_task_tile, B.h_task_tile));
+    SWG_HIP(put(d_tile_col, B.h_tile_col));
+    SWG_HIP(put(d_tile_max, B.h_tile_maxlen));
+
+    SeqSet S2;
+    const SeqSet S = make_seqset(c, ln, &S2);
+    Scratch W;
+    W.bnd = d_bnd;
+    W.bnd_stride = stride;
+    TaskSet T{};
+    T.task_row = d_task_row; T.task_tile = d_task_tile; T.tile_col = d_tile_col; T.tile_maxlen = d_tile_max;
+    T.desc = d_desc; T.queue = d_queue + kQueueFirst;
+    TaskSet T2 = T;
+    T2.queue = d_queue + kQueueSecond + kQueueFirst;
+    const int pair_grid = std::max(1, c->num_cu * 4);
+    const int tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
+
+    SWG_HIP(hipEventRecord(ln.ev0, st));
+    SWG_HIP(launch_zero_diagonal(tdist, d_desc, pair_grid, st));
+    if (plan.ntasks > 0) {
+        // the work counters start at zero on every call (a fill kernel, no memset)
+        SWG_HIP(launch_fill_i32(d_queue, 0, kQueueWords, st));
+        SWG_HIP(launch_lpp16(S, T, W, s1, true, nslots, st));
+        if (kimura) SWG_HIP(launch_lpp16(S2, T2, W, s1, true, nslots, st));
+        if (words) SWG_HIP(launch_distance(S, d_desc, pair_grid, st));
+    }
+    SWG_HIP(hipEventRecord(ln.ev1, st));
+    SWG_HIP(hipEventRecord(B.done, st));
+    B.pending = true;
+    c->last_strip = s1;
+    c->last_variant = v;
+    c->last_cells = plan.cells;
+    c->kt_n = 0;  // no per-kernel stamps on this path
+    if (pairs_done) *pairs_done = plan.pairs;
+    return SWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swg_compute_block_device_noalloc(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
+                                     void* dev_out, int64_t ld, swg_pair_stats* dev_stats, int64_t* pairs_done)
+{
+    if (!c || !dev_out) return fail(SWG_E_ARGUMENT, "null argument");
+    SWG_HIP(hipSetDevice(c->device));
+    return enqueue_block_noalloc(c, r0, r1, c0, c1, is_diag, dev_out, ld, reinterpret_cast<int32_t*>(dev_stats),
+                                 pairs_done);
 }
 
 }  // extern "C"

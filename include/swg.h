@@ -151,6 +151,33 @@ int  swg_compute_block_device(swg_ctx* ctx, int32_t row_start, int32_t row_end,
  * of any block since the last check had d <= double.Epsilon. */
 int  swg_synchronize(swg_ctx* ctx);
 
+/* Allocation-free variant of swg_compute_block_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and completion (swg_synchronize) as
+ * swg_compute_block_device, local alignment only (global mode:
+ * SWG_E_UNSUPPORTED); it runs the 16-lanes-per-pair int32 kernel launched
+ * directly on the context stream and calls no hipMalloc, hipFree or other
+ * allocating routine.  Every device buffer its kernels need besides the
+ * caller's inputs and outputs (descriptor, work counters, task lists, count
+ * words of the word-mode distances, row-boundary scratch) is carved from one
+ * library-static device array of SWG_NOALLOC_SCRATCH_BYTES per GPU, reserved
+ * when the library's code object is loaded on that GPU and held until the
+ * process ends.  A block whose task lists (~8 bytes per 4 pairs) and count
+ * words (4 bytes per pair and plane: Jukes-Cantor and MinMaxNormScore 1,
+ * Kimura2 2) leave no room for one workgroup's boundary rows
+ * ((longest sequence + 32) x 64 bytes) fails with SWG_E_CAPACITY; otherwise
+ * as many workgroups run as fit, up to the resident maximum.  Every call on a
+ * GPU shares that GPU's array and leaves its contents behind (the work
+ * counters are reset by a fill kernel at the start of each call): calls on
+ * one context are ordered by its stream, but calls that overlap on one GPU
+ * from different contexts or threads race. */
+#define SWG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int  swg_compute_block_device_noalloc(swg_ctx* ctx, int32_t row_start, int32_t row_end,
+                                      int32_t col_start, int32_t col_end, int32_t is_diagonal,
+                                      void* dev_out, int64_t out_ld, swg_pair_stats* dev_stats,
+                                      int64_t* pairs_done);
+
 /* Schedule control (SWG_SCHED_*) and strip height of the int32 alignment
  * kernel (8, 16 or 32; 0 = default 16; for SWG_SCHED_PERSISTENT_LPP16, 8
  * selects 8 rows per lane, anything else 4). */
