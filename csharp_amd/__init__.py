@@ -35,7 +35,7 @@ TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 PROGRAM_SWMPI, PROGRAM_SWMS = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
-OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING = 1, 2
+OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING, OPT_SCORE_STRIP_ROWS, OPT_TUNE_MIN_CELLS = 1, 2, 3, 4
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
                 7: "global", 8: "reverse", 9: "rect64", 10: "distance"}
 MATRIX_SIZE = 127
@@ -45,7 +45,7 @@ ABI_SYMBOLS = (
     "swg_synchronize", "swg_set_schedule", "swg_last_kernel_stats", "swg_last_error", "swg_destroy",
     "swg_serve_create", "swg_serve_submit", "swg_serve_wait", "swg_serve_times", "swg_serve_release",
     "swg_serve_stats", "swg_serve_destroy", "swg_set_option", "swg_last_kernel_times",
-    "swg_compute_block_device_noalloc",
+    "swg_compute_block_device_noalloc", "swg_last_block_config",
 )
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SWG_NOALLOC_SCRATCH_BYTES
 HOST_SYMBOLS = (
@@ -77,6 +77,14 @@ class JukesCantorUndefined(SwgError):
 class swg_kernel_time(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32), ("ms", ctypes.c_double),
                 ("cells", ctypes.c_double), ("cells_run", ctypes.c_double)]
+
+
+class swg_block_config(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("strip_rows", ctypes.c_int32), ("rerun_strip_rows", ctypes.c_int32),
+                ("pairs_per_wave", ctypes.c_int32), ("tuning", ctypes.c_int32), ("trials", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class swg_params(ctypes.Structure):
@@ -116,6 +124,7 @@ def lib():
         L.swg_last_kernel_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(i32), ctypes.POINTER(i32)]
         L.swg_last_kernel_times.argtypes = [vp, ctypes.POINTER(swg_kernel_time), i32, ctypes.POINTER(i32)]
+        L.swg_last_block_config.argtypes = [vp, ctypes.POINTER(swg_block_config)]
         L.swg_last_error.restype = ctypes.c_char_p
         L.swg_destroy.argtypes = [vp]
         L.swg_destroy.restype = None
@@ -345,6 +354,22 @@ class Aligner:
 
     def set_stream_chunk_bytes(self, nbytes):
         _check(lib().swg_set_option(self._h, OPT_STREAM_CHUNK_BYTES, int(nbytes)))
+
+    def set_score_strip(self, rows):
+        """Rows per strip of the int16 score pass (16, 24, 32; 0 = autotuned)."""
+        _check(lib().swg_set_option(self._h, OPT_SCORE_STRIP_ROWS, int(rows)))
+
+    def set_tune_min_cells(self, cells):
+        """DP cells a block needs for the autotuner to time it."""
+        _check(lib().swg_set_option(self._h, OPT_TUNE_MIN_CELLS, int(cells)))
+
+    def last_block_config(self):
+        """Launch configuration of the last block: variant (SWG_SCHED_*), strip
+        rows, re-run strip rows, pairs per wave, tuning state (0 fixed, 1
+        autotuner trial, 2 autotuner choice) and the key's timed trials."""
+        cfg = swg_block_config()
+        _check(lib().swg_last_block_config(self._h, ctypes.byref(cfg)))
+        return cfg.as_dict()
 
     def set_kernel_timing(self, on):
         """A wall-clock stamp kernel before the block's kernels and after each,

@@ -1,6 +1,7 @@
 // Host side of libswg.so: the C ABI declared in include/swg.h, the
 // block -> task scheduler (length buckets, work queue, hipGraph per block,
-// strip-height autotuner) and device-memory management.
+// autotuner of the kernel variant and its tile shape) and device-memory
+// management.
 #include "../../include/swg.h"
 #include "swg_internal.h"
 
@@ -159,12 +160,30 @@ struct Plan {
     int32_t ntasks = 0;      // tasks of the (first) kernel
 };
 
-// Autotuner record of one key (see choose_variant).
+// A block's launch configuration: the kernel variant and its tile shape --
+// rows per strip of the first kernel (score pass, one-pass, global kernel) or
+// rows per lane (16 lanes per pair), and rows per strip of the int32 re-run
+// (two-pass variant only).  The pairs per wave follow from the variant: 128
+// for the score pass (two per lane), 64 for the one-pass and global kernels,
+// 4 for 16 lanes per pair.
+struct VariantCfg {
+    int v = kVarI32;
+    int s1 = 16;
+    int s2 = 0;
+    bool operator==(const VariantCfg& o) const { return v == o.v && s1 == o.s1 && s2 == o.s2; }
+};
+
+// Autotuner record of one key (see choose_variant): the configurations
+// measured so far and their rates, the candidates of the current stage not
+// yet measured, and the choice once every stage is done.
 struct TuneEntry {
+    struct Trial { VariantCfg cfg; double rate; };
     int64_t key = 0;
-    int chosen = -1;
-    bool tried[kNumVariants] = {false, false, false, false};
-    double rate[kNumVariants] = {0.0, 0.0, 0.0, 0.0};
+    int stage = 0;
+    bool done = false;
+    VariantCfg best{};
+    std::vector<Trial> trials;
+    std::vector<VariantCfg> pending;
 };
 
 // An execution lane: a stream, two alternating buffer sets and a boundary
@@ -230,12 +249,17 @@ struct swg_ctx {
     size_t stream_chunk_bytes = size_t(64) << 20;
 
     int32_t schedule = SWG_SCHED_PERSISTENT;
-    int32_t strip_rows = 0;  // 0 = autotune
+    int32_t strip_rows = 0;   // int32 kernels' strip (re-run strip of the two-pass); 0 = autotuned
+    int32_t score_strip = 0;  // score pass strip; 0 = autotuned
+    double tune_min_cells = 5e10;  // smaller blocks run the key's current candidate untimed
     // autotuner cache (entries are never removed while the deque lives)
     std::deque<TuneEntry> tuned;
     double last_kernel_ms = 0.0, last_cells = 0.0;
     int32_t last_strip = 0;
     int32_t last_variant = 0;
+    VariantCfg last_cfg{};
+    int32_t last_tuning = 0;   // 0 fixed, 1 autotuner trial, 2 autotuner choice
+    int32_t last_trials = 0;   // trials measured for the last block's key
 
     // kernel-timing mode: a stamp kernel before the block's kernels and after
     // each of them (inside the block's graph), wall-clock ticks at wall_khz
@@ -256,24 +280,33 @@ bool is_integral(float v) { return std::isfinite(v) && std::floor(v) == v && std
 
 int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 : 2; }
 
-// strip heights per variant: (first kernel, second kernel)
-void variant_strips(const swg_ctx* c, int v, int* s1, int* s2)
+// The configuration of variant v with its default strips, or those the
+// caller fixed (swg_set_schedule's strip_rows, SWG_OPT_SCORE_STRIP_ROWS).
+VariantCfg default_cfg(const swg_ctx* c, int v)
 {
-    if (c->nw) { *s1 = c->strip_rows == 8 ? 8 : 16; *s2 = 0; return; }
-    if (v == kVarPlain) { *s1 = 16; *s2 = 0; return; }
-    if (v == kVarLpp) { *s1 = c->strip_rows == 8 ? 8 : 4; *s2 = 0; return; }
-    if (v == kVarI32) { *s1 = c->strip_rows ? c->strip_rows : 16; *s2 = 0; return; }
-    *s1 = 24;  // score pass: 24 rows per strip (measured 9% faster than 16; 32 spills)
-    *s2 = c->strip_rows == 32 ? 32 : 16;  // re-run
+    VariantCfg f;
+    f.v = v;
+    f.s2 = 0;
+    if (c->nw) f.s1 = c->strip_rows == 8 ? 8 : 16;
+    else if (v == kVarPlain) f.s1 = 16;
+    else if (v == kVarLpp) f.s1 = c->strip_rows == 8 ? 8 : 4;
+    else if (v == kVarI32) f.s1 = c->strip_rows ? c->strip_rows : 16;
+    else {
+        f.s1 = c->score_strip ? c->score_strip : 24;  // score pass (24 measured 9% faster than 16; 32 spills)
+        f.s2 = c->strip_rows == 32 ? 32 : 16;         // re-run
+    }
+    return f;
 }
 
-// Resident workgroups of the variant's kernels on the lane's share of the
-// CUs: n1 first kernel, n2 re-run, n3 reverse pass and n4 big-rectangle
+int pairs_per_wave(int v) { return v == kVarI16 ? kScoreTile : v == kVarLpp ? kLppTile : kWave; }
+
+// Resident workgroups of the configuration's kernels on the lane's share of
+// the CUs: n1 first kernel, n2 re-run, n3 reverse pass and n4 big-rectangle
 // re-run (two-pass variant).
-int slots_for(const swg_ctx* c, const Lane& ln, int v, int* n1, int* n2, int* n3 = nullptr, int* n4 = nullptr)
+int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, int* n2, int* n3 = nullptr,
+              int* n4 = nullptr)
 {
-    int s1, s2;
-    variant_strips(c, v, &s1, &s2);
+    const int v = f.v, s1 = f.s1, s2 = f.s2;
     int a = c->nw        ? nw_blocks_per_cu(s1)
           : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
@@ -428,13 +461,13 @@ void append_key(std::vector<unsigned char>& k, const T& v)
     k.insert(k.end(), p, p + sizeof(T));
 }
 
-// Plan, upload and launch one block (or the rows of a sample) with variant v.
-int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, void* dev_out, int64_t ld,
-                int32_t* dev_stats, Plan& plan)
+// Plan, upload and launch one block with configuration f.
+int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const VariantCfg& f, void* dev_out,
+                int64_t ld, int32_t* dev_stats, Plan& plan)
 {
-    int s1, s2, n1, n2, n3, n4;
-    variant_strips(c, v, &s1, &s2);
-    int rc = slots_for(c, ln, v, &n1, &n2, &n3, &n4);
+    const int v = f.v, s1 = f.s1, s2 = f.s2;
+    int n1, n2, n3, n4;
+    int rc = slots_for(c, ln, f, &n1, &n2, &n3, &n4);
     if (rc) return rc;
     int want = std::max(std::max(n1, n2), v == kVarI16 ? std::max(n3, n4) : 0);
     rc = ensure_scratch(c, ln, &want);
@@ -673,6 +706,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, int v, v
     B.pending = true;
     c->last_strip = v == kVarI16 ? s2 : s1;
     c->last_variant = v;
+    c->last_cfg = f;
     return SWG_OK;
 }
 
@@ -693,7 +727,21 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
 // the faster of the int32 one-pass and the int16 two-pass variants.  The
 // first blocks of a key run each candidate once, synchronously, on real work
 // (nothing is recomputed); the variant with the higher cells/ms wins.
-constexpr double kTuneMinCells = 5e10;  // smaller blocks run the default variant untimed
+// Autotuner.  Per key (residue alphabet, mean-length class, block-size
+// class) the first blocks of the key run candidate configurations, each once
+// and synchronously, on real work (nothing is recomputed); the one with the
+// highest cells/ms wins.  The search is staged, one dimension at a time:
+//   stage 0  the variant at its default tile shape: int16 two-pass (score
+//            strip 24, re-run strip 16), int32 one-pass (strip 16), 16 lanes
+//            per pair (4 rows per lane) -- lanes per pair 1 or 16, int16 or
+//            int32; the two-pass first for alphabets of <= 8 codes, the
+//            one-pass first for larger ones, so that a run of a single block
+//            gets the variant that measured faster on such data;
+//   stage 1  the winner's first strip: score strip 16 <-> 24, int32 strips
+//            8 and 32, rows per lane 8 <-> 4, the global kernel's 8 <-> 16;
+//   stage 2  the two-pass re-run strip 16 <-> 32.
+// Strips fixed by the caller (swg_set_schedule's strip_rows,
+// SWG_OPT_SCORE_STRIP_ROWS) end the search after stage 0.
 
 int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
 {
@@ -711,6 +759,55 @@ int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
     return ((int64_t)c->ncode << 40) | ((int64_t)len_class << 20) | (int64_t)size_class;
 }
 
+
+// Candidates of the entry's current stage (see above).
+std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
+{
+    std::vector<VariantCfg> out;
+    if (t.stage == 0) {
+        if (c->nw) { out.push_back(default_cfg(c, kVarI32)); return out; }
+        const bool small_alphabet = c->ncode <= 8;
+        if (c->int16_ok && small_alphabet) out.push_back(default_cfg(c, kVarI16));
+        out.push_back(default_cfg(c, kVarI32));
+        if (c->int16_ok && !small_alphabet) out.push_back(default_cfg(c, kVarI16));
+        out.push_back(default_cfg(c, kVarLpp));
+        return out;
+    }
+    if (c->strip_rows != 0 || c->score_strip != 0) return out;
+    const VariantCfg b = t.best;
+    VariantCfg f = b;
+    if (t.stage == 1) {
+        if (c->nw) { f.s1 = b.s1 == 16 ? 8 : 16; out.push_back(f); }
+        else if (b.v == kVarI16) { f.s1 = b.s1 == 24 ? 16 : 24; out.push_back(f); }
+        else if (b.v == kVarLpp) { f.s1 = b.s1 == 4 ? 8 : 4; out.push_back(f); }
+        else if (b.v == kVarI32)
+            for (int r : {8, 16, 32})
+                if (r != b.s1) { f.s1 = r; out.push_back(f); }
+    } else if (t.stage == 2 && b.v == kVarI16 && !c->nw) {
+        f.s2 = b.s2 == 16 ? 32 : 16;
+        out.push_back(f);
+    }
+    return out;
+}
+
+// Move the entry on once its stage's candidates are all measured: the best
+// so far, then the next stage's untried candidates, or done.
+void tune_advance(const swg_ctx* c, TuneEntry& t)
+{
+    while (t.pending.empty() && !t.done) {
+        double best_rate = -1.0;
+        for (const auto& x : t.trials)
+            if (x.rate > best_rate) { best_rate = x.rate; t.best = x.cfg; }
+        if (t.stage >= 3) { t.done = true; break; }
+        for (const VariantCfg& f : stage_candidates(c, t)) {
+            bool tried = false;
+            for (const auto& x : t.trials) tried = tried || x.cfg == f;
+            if (!tried) t.pending.push_back(f);
+        }
+        ++t.stage;
+    }
+}
+
 // Too few row sequences to fill the GPU with one-lane-per-pair waves: the
 // 16-lanes-per-pair kernel gives each pair 16 lanes instead.
 bool thin_block(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
@@ -718,26 +815,24 @@ bool thin_block(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
     const double rows = g.r1 - g.r0 + 1, cols = g.c1 - g.c0 + 1;
     const double waves = g.diag_lower ? rows * std::ceil(cols / 2.0 / kWave) : rows * std::ceil(cols / kWave);
     int n1 = 0, n2 = 0;
-    if (slots_for(c, ln, kVarI32, &n1, &n2) != SWG_OK) return false;
+    if (slots_for(c, ln, default_cfg(c, kVarI32), &n1, &n2) != SWG_OK) return false;
     return waves < 2.0 * n1;
 }
 
-int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant, TuneEntry** trial)
+// The configuration of a block: fixed by the schedule, the 16-lane kernel for
+// a thin block, or the autotuner's (*entry: its record; *trial: the
+// configuration is a candidate still to be measured).
+int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* cfg, TuneEntry** entry, bool* trial)
 {
-    *trial = nullptr;
-    if (c->schedule == SWG_SCHED_PLAIN) { *variant = kVarPlain; return SWG_OK; }
-    if (c->nw) {
-        // one global kernel (int32, one lane per pair, persistent)
-        if (c->schedule == SWG_SCHED_PERSISTENT_I16 || c->schedule == SWG_SCHED_PERSISTENT_LPP16)
-            return fail(SWG_E_UNSUPPORTED, "the global alignment has only the int32 one-pass kernel");
-        *variant = kVarI32;
-        return SWG_OK;
-    }
-    if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) { *variant = kVarLpp; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *variant = kVarI32; return SWG_OK; }
-    if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *variant = kVarI16; return SWG_OK; }
-    if (thin_block(c, ln, g)) { *variant = kVarLpp; return SWG_OK; }
-    if (!c->int16_ok) { *variant = kVarI32; return SWG_OK; }
+    *entry = nullptr;
+    *trial = false;
+    if (c->schedule == SWG_SCHED_PLAIN) { *cfg = default_cfg(c, kVarPlain); return SWG_OK; }
+    if (c->nw && (c->schedule == SWG_SCHED_PERSISTENT_I16 || c->schedule == SWG_SCHED_PERSISTENT_LPP16))
+        return fail(SWG_E_UNSUPPORTED, "the global alignment has only the int32 one-pass kernel");
+    if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) { *cfg = default_cfg(c, kVarLpp); return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *cfg = default_cfg(c, kVarI32); return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *cfg = default_cfg(c, kVarI16); return SWG_OK; }
+    if (!c->nw && thin_block(c, ln, g)) { *cfg = default_cfg(c, kVarLpp); return SWG_OK; }
     const int64_t key = tune_key(c, g);
     TuneEntry* t = nullptr;
     for (auto& e : c->tuned)
@@ -747,20 +842,11 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, int* variant,
         t = &c->tuned.back();
         t->key = key;
     }
-    if (t->chosen >= 0) { *variant = t->chosen; return SWG_OK; }
-    // trial order: the two-pass first for small alphabets (DNA), the one-pass
-    // first for large ones (protein, where cheap gap extension makes the
-    // re-run rectangles a large share of the cells): a run of a single block
-    // then gets the variant that measured faster on such data
-    const bool small_alphabet = c->ncode <= 8;
-    for (int v : {small_alphabet ? kVarI16 : kVarI32, small_alphabet ? kVarI32 : kVarI16}) {
-        if (!t->tried[v]) {
-            *variant = v;
-            *trial = t;
-            return SWG_OK;
-        }
-    }
-    *variant = kVarI16;
+    tune_advance(c, *t);
+    *entry = t;
+    if (t->done) { *cfg = t->best; return SWG_OK; }
+    *cfg = t->pending.front();
+    *trial = true;
     return SWG_OK;
 }
 
@@ -781,27 +867,29 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
         if (pairs_done) *pairs_done = 0;
         return SWG_OK;
     }
-    int variant = kVarI32;
-    TuneEntry* trial = nullptr;
-    int rc = choose_variant(c, ln, g, &variant, &trial);
+    VariantCfg cfg;
+    TuneEntry* entry = nullptr;
+    bool trial = false;
+    int rc = choose_variant(c, ln, g, &cfg, &entry, &trial);
     if (rc) return rc;
     BlockBufs& B = next_bufs(ln, &rc);
     if (rc) return rc;
     Plan plan;
-    rc = run_variant(c, ln, B, g, variant, dev_out, ld, dev_stats, plan);
+    rc = run_variant(c, ln, B, g, cfg, dev_out, ld, dev_stats, plan);
     if (rc) return rc;
     c->last_cells = plan.cells;
-    if (trial && plan.cells >= kTuneMinCells) {
+    if (trial && plan.cells >= c->tune_min_cells) {
         SWG_HIP(hipStreamSynchronize(ln.stream));
         B.pending = false;
         float ms = 0.0f;
         SWG_HIP(hipEventElapsedTime(&ms, ln.ev0, ln.ev1));
         c->last_kernel_ms = ms;
-        TuneEntry& t = *trial;
-        t.tried[variant] = true;
-        t.rate[variant] = plan.cells / std::max(1e-3, (double)ms);
-        if (t.tried[kVarI16] && t.tried[kVarI32]) t.chosen = t.rate[kVarI16] >= t.rate[kVarI32] ? kVarI16 : kVarI32;
+        entry->trials.push_back({cfg, plan.cells / std::max(1e-3, (double)ms)});
+        entry->pending.erase(entry->pending.begin());
+        tune_advance(c, *entry);
     }
+    c->last_tuning = trial ? 1 : entry ? 2 : 0;
+    c->last_trials = entry ? (int32_t)entry->trials.size() : 0;
     if (pairs_done) *pairs_done = plan.pairs;
     return SWG_OK;
 }
@@ -1121,9 +1209,8 @@ int enqueue_block_noalloc(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_
     Lane& ln = c->main;
     BlockBufs& B = next_bufs(ln, &rc);  // waits until this set's host task lists are free again
     if (rc) return rc;
-    const int v = kVarLpp;
-    int s1, s2;
-    variant_strips(c, v, &s1, &s2);
+    const VariantCfg f = default_cfg(c, kVarLpp);
+    const int v = f.v, s1 = f.s1;
     Plan plan;
     plan_tiles(c, g, true, kLppTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
 
@@ -1206,6 +1293,9 @@ int enqueue_block_noalloc(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_
     B.pending = true;
     c->last_strip = s1;
     c->last_variant = v;
+    c->last_cfg = f;
+    c->last_tuning = 0;
+    c->last_trials = 0;
     c->last_cells = plan.cells;
     c->kt_n = 0;  // no per-kernel stamps on this path
     if (pairs_done) *pairs_done = plan.pairs;
@@ -1319,8 +1409,14 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
     int rc = enqueue_block(c, c->main, r0, r1, c0, c1, is_diag, c->out_stage.p, cols,
                            stats ? c->stats_stage.p : nullptr, pairs_done);
     if (rc) return rc;
-    SWG_HIP(hipMemcpy2DAsync(out, (size_t)out_ld * es, c->out_stage.p, (size_t)cols * es, (size_t)cols * es,
-                             (size_t)rows, hipMemcpyDeviceToHost, c->main.stream));
+    // one flat copy when the caller's rows are contiguous (a strided copy
+    // moves the block row by row)
+    if (out_ld == cols)
+        SWG_HIP(hipMemcpyAsync(out, c->out_stage.p, (size_t)(rows * cols * es), hipMemcpyDeviceToHost,
+                               c->main.stream));
+    else
+        SWG_HIP(hipMemcpy2DAsync(out, (size_t)out_ld * es, c->out_stage.p, (size_t)cols * es, (size_t)cols * es,
+                                 (size_t)rows, hipMemcpyDeviceToHost, c->main.stream));
     if (stats)
         SWG_HIP(hipMemcpyAsync(stats, c->stats_stage.p, (size_t)rows * cols * 16, hipMemcpyDeviceToHost, c->main.stream));
     return swg_synchronize(c);
@@ -1334,6 +1430,7 @@ int swg_set_schedule(swg_ctx* c, int32_t schedule, int32_t strip_rows)
     if (strip_rows != 0 && strip_rows != 8 && strip_rows != 16 && strip_rows != 32)
         return fail(SWG_E_ARGUMENT, "strip_rows must be 0 (default), 8, 16 or 32");
     c->schedule = schedule;
+    if (strip_rows != c->strip_rows) c->tuned.clear();  // the search depends on the fixed strips
     c->strip_rows = strip_rows;
     return SWG_OK;
 }
@@ -1351,7 +1448,35 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         c->kernel_timing = value == 1;
         return SWG_OK;
     }
+    if (option == SWG_OPT_SCORE_STRIP_ROWS) {
+        if (value != 0 && value != 16 && value != 24 && value != 32)
+            return fail(SWG_E_ARGUMENT, "score-pass strip rows are 0 (autotuned), 16, 24 or 32");
+        if (value != c->score_strip) c->tuned.clear();
+        c->score_strip = (int32_t)value;
+        return SWG_OK;
+    }
+    if (option == SWG_OPT_TUNE_MIN_CELLS) {
+        if (value < 0) return fail(SWG_E_ARGUMENT, "the tuning threshold is a cell count >= 0");
+        c->tune_min_cells = (double)value;
+        return SWG_OK;
+    }
     return fail(SWG_E_ARGUMENT, "unknown option");
+}
+
+int swg_last_block_config(swg_ctx* c, swg_block_config* out)
+{
+    if (!c || !out) return fail(SWG_E_ARGUMENT, "null argument");
+    const VariantCfg& f = c->last_cfg;
+    out->variant = f.v == kVarPlain ? SWG_SCHED_PLAIN
+                 : f.v == kVarI32   ? SWG_SCHED_PERSISTENT_I32
+                 : f.v == kVarLpp   ? SWG_SCHED_PERSISTENT_LPP16
+                                    : SWG_SCHED_PERSISTENT_I16;
+    out->strip_rows = f.s1;
+    out->rerun_strip_rows = f.v == kVarI16 ? f.s2 : 0;
+    out->pairs_per_wave = pairs_per_wave(f.v);
+    out->tuning = c->last_tuning;
+    out->trials = c->last_trials;
+    return SWG_OK;
 }
 
 int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t* n)

@@ -179,8 +179,11 @@ int  swg_compute_block_device_noalloc(swg_ctx* ctx, int32_t row_start, int32_t r
                                       int64_t* pairs_done);
 
 /* Schedule control (SWG_SCHED_*) and strip height of the int32 alignment
- * kernel (8, 16 or 32; 0 = default 16; for SWG_SCHED_PERSISTENT_LPP16, 8
- * selects 8 rows per lane, anything else 4). */
+ * kernels (one-pass 8, 16 or 32; the two-pass re-run 16 or 32; the global
+ * kernel 8 or 16; for 16 lanes per pair, 8 selects 8 rows per lane, anything
+ * else 4); 0 = the defaults (16; 4 rows per lane), or autotuned with
+ * SWG_SCHED_PERSISTENT.  A non-zero value limits the autotuner to the choice
+ * of variant. */
 int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
 /* Tuning knobs: SWG_OPT_STREAM_CHUNK_BYTES = device output per streamed row
  * chunk of swg_compute_block (blocks larger than one chunk are computed chunk
@@ -192,6 +195,13 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * after each of them, inside the block's captured graph, for
  * swg_last_kernel_times; 0 (default) leaves the stamps out. */
 #define SWG_OPT_KERNEL_TIMING 2
+/* SWG_OPT_SCORE_STRIP_ROWS = rows per strip of the int16 score pass (16, 24
+ * or 32; 0 = autotuned, default 24).  Like swg_set_schedule's strip_rows, a
+ * fixed value limits the autotuner to the choice of variant. */
+#define SWG_OPT_SCORE_STRIP_ROWS 3
+/* SWG_OPT_TUNE_MIN_CELLS = DP cells a block needs for the autotuner to time
+ * it (default 5e10); smaller blocks run the key's current candidate untimed. */
+#define SWG_OPT_TUNE_MIN_CELLS 4
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the
@@ -200,6 +210,30 @@ int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
  * SWG_SCHED_* value). */
 int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32_t* strip_rows,
                            int32_t* variant);
+
+/* Launch configuration of the last block (no counterpart in the reference,
+ * which runs one fixed loop per block, Salsa.SmithWatermanMPI/Program.cs:
+ * 203-257,618-642): the kernel variant and its tile shape, and whether the
+ * autotuner chose it.  With SWG_SCHED_PERSISTENT the autotuner searches, per
+ * key (residue alphabet, mean-length class, block-size class), on the first
+ * blocks of the key (each run once, on its real work): the variant (int16
+ * two-pass, int32 one-pass, 16 lanes per pair) at its default strips, then
+ * the winner's first strip (score pass 16/24, one-pass 8/16/32, rows per lane
+ * 4/8, global kernel 8/16), then the two-pass re-run strip (16/32); the
+ * highest cells/ms wins.  Thin blocks (too few rows to fill the GPU one lane
+ * per pair) always take 16 lanes per pair. */
+typedef struct swg_block_config {
+    int32_t variant;          /* SWG_SCHED_PLAIN, _PERSISTENT_I32, _PERSISTENT_I16 or _PERSISTENT_LPP16 */
+    int32_t strip_rows;       /* rows per strip of the first kernel (score pass, one-pass or global
+                                 kernel) or rows per lane (16 lanes per pair) */
+    int32_t rerun_strip_rows; /* two-pass: rows per strip of the int32 re-run; 0 otherwise */
+    int32_t pairs_per_wave;   /* 128 score pass (two pairs per lane), 64 one-pass / global, 4 sixteen
+                                 lanes per pair */
+    int32_t tuning;           /* 0 fixed by the schedule (or a thin block), 1 an autotuner trial,
+                                 2 the autotuner's choice */
+    int32_t trials;           /* configurations the autotuner has timed for this block's key */
+} swg_block_config;
+int  swg_last_block_config(swg_ctx* ctx, swg_block_config* out);
 
 /* Kernels of a block (no counterpart in the reference: per-kernel
  * accounting for roofline measurements). */

@@ -33,7 +33,7 @@ def test_alphabet_sizes(ednafull, ncodes, schedule):
     rng = np.random.default_rng(100 + ncodes)
     alphabet = IUPAC[:ncodes]
     seqs = family(rng, alphabet, 40, 60, 400, 0.15) + \
-        [bytes(rng.choice(list(alphabet), size=int(rng.integers(1, 300)))) for _ in range(40)]
+        [rng.choice(list(alphabet), size=int(rng.integers(1, 300))).astype(np.uint8).tobytes() for _ in range(40)]
     ss = ca.SequenceSet.from_strings(seqs)
     al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
     al.set_sequences(ss)

@@ -1,0 +1,118 @@
+"""The autotuner's search space, on the GPU through the C ABI: every
+configuration it can choose (variant x tile shape) is bit-exact against the
+oracle when forced, and an autotuned run measures the variants, then the
+winner's strips, each trial on real work whose results are kept, and settles
+on one of the configurations it measured."""
+import sys
+
+import numpy as np
+import pytest
+
+import csharp_amd as ca
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = O.cpu_threads()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wide_set():
+    # 3,000 row sequences x 256 columns: 12,000 waves one lane per pair, not
+    # a thin block, so the autotuner (not the 16-lane shortcut) decides
+    return ca.synth_dna(61, 3256, 20, 0.10, 0.02, 0, [30, 90])
+
+
+@pytest.fixture(scope="module")
+def wide_expected(wide_set, ednafull):
+    a, b = np.meshgrid(np.arange(0, 3000), np.arange(3000, 3256), indexing="ij")
+    d, _ = O.sw_pairs_mt(wide_set.residues, wide_set.offsets, a.ravel(), b.ravel(), ednafull, 16, 4,
+                         threads=THREADS)
+    return d.reshape(a.shape)
+
+
+@pytest.mark.parametrize("score_strip", [16, 24, 32])
+@pytest.mark.parametrize("rerun_strip", [16, 32])
+def test_forced_two_pass_tile_shapes(wide_set, wide_expected, ednafull, score_strip, rerun_strip):
+    say("two-pass score strip %d re-run strip %d" % (score_strip, rerun_strip))
+    al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
+    al.set_sequences(wide_set)
+    al.set_schedule(ca.SCHED_PERSISTENT_I16, rerun_strip)
+    al.set_score_strip(score_strip)
+    got, _, _ = al.compute_block(0, 2999, 3000, 3255, False)
+    cfg = al.last_block_config()
+    assert (cfg["variant"], cfg["strip_rows"], cfg["rerun_strip_rows"], cfg["pairs_per_wave"], cfg["tuning"]) == (
+        ca.SCHED_PERSISTENT_I16, score_strip, rerun_strip, 128, 0)
+    assert np.array_equal(got, wide_expected)
+    al.close()
+
+
+def run_until_tuned(al, blk, expected, limit=10):
+    trials = []
+    for k in range(limit):
+        got, _, _ = al.compute_block(*blk, False)
+        cfg = al.last_block_config()
+        assert np.array_equal(got, expected), "block %d (%s) differs from the oracle" % (k, cfg)
+        say("block %d: %s" % (k, cfg))
+        if cfg["tuning"] == 2:
+            return trials, cfg
+        assert cfg["tuning"] == 1
+        trials.append(cfg)
+    raise AssertionError("the autotuner did not settle in %d blocks: %s" % (limit, trials))
+
+
+def shape(cfg):
+    return cfg["variant"], cfg["strip_rows"], cfg["rerun_strip_rows"]
+
+
+def test_autotuner_searches_variant_then_tile_shape(wide_set, wide_expected, ednafull):
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(wide_set)
+    al.set_tune_min_cells(0)  # time every block, however small
+    trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), wide_expected)
+    seen = [shape(t) for t in trials]
+    assert len(set(seen)) == len(seen), "a configuration was measured twice"
+    # stage 0: the three variants at their default tile shapes, the two-pass
+    # first for a 4-code alphabet
+    assert seen[:3] == [(ca.SCHED_PERSISTENT_I16, 24, 16), (ca.SCHED_PERSISTENT_I32, 16, 0),
+                        (ca.SCHED_PERSISTENT_LPP16, 4, 0)]
+    # later stages vary the winner's strips only
+    winner = [t for t in trials[3:]]
+    assert winner, "no tile-shape stage ran"
+    assert len({t["variant"] for t in winner}) == 1
+    assert shape(final) in seen and final["trials"] == len(trials)
+    assert [t["trials"] for t in trials] == list(range(1, len(trials) + 1))
+    # the choice sticks
+    got, _, _ = al.compute_block(0, 2999, 3000, 3255, False)
+    assert al.last_block_config() == final and np.array_equal(got, wide_expected)
+    al.close()
+
+
+def test_autotuner_with_fixed_strips_chooses_the_variant_only(wide_set, wide_expected, ednafull):
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(wide_set)
+    al.set_schedule(ca.SCHED_PERSISTENT, 32)
+    al.set_tune_min_cells(0)
+    trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), wide_expected)
+    assert [shape(t) for t in trials] == [(ca.SCHED_PERSISTENT_I16, 24, 32), (ca.SCHED_PERSISTENT_I32, 32, 0),
+                                          (ca.SCHED_PERSISTENT_LPP16, 4, 0)]
+    assert shape(final) in [shape(t) for t in trials]
+    al.close()
+
+
+def test_autotuner_global_strips(wide_set, ednafull):
+    al = ca.Aligner(ednafull, 16, 4, mode=ca.MODE_NW_GLOBAL)
+    al.set_sequences(wide_set)
+    al.set_tune_min_cells(0)
+    a, b = np.meshgrid(np.arange(0, 3000), np.arange(3000, 3256), indexing="ij")
+    d, _ = O.nw_pairs_mt(wide_set.residues, wide_set.offsets, a.ravel(), b.ravel(), ednafull, 16, 4,
+                         threads=THREADS)
+    trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), d.reshape(a.shape))
+    assert [shape(t) for t in trials] == [(ca.SCHED_PERSISTENT_I32, 16, 0), (ca.SCHED_PERSISTENT_I32, 8, 0)]
+    assert shape(final) in [shape(t) for t in trials]
+    al.close()

@@ -230,6 +230,41 @@ def test_long_sequences(ednafull, schedule):
     al.close()
 
 
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16,
+                                      ca.SCHED_PERSISTENT_LPP16])
+def test_very_long_sequences(ednafull, schedule):
+    """7,800-8,000-residue relatives (they score above the int16 range: the
+    score pass flags them for the whole-matrix int32 re-run), unrelated
+    sequences of the same lengths and a 5,000-residue one; then a
+    16,000-residue sequence against a mutated 9,000-residue copy of its start
+    and an unrelated 3,000-residue one: boundary scratch sized for the
+    longest sequence, bit-exact against the oracle."""
+    say("very long sequences schedule=%d" % schedule)
+    rng = np.random.default_rng(41)
+    fam = _mutated_family(rng, b"ACGT", 3, 7800, 8000, 0.02, 0.01)  # 0-2 related, 3-5 unrelated
+    five = rng.choice(list(b"ACGT"), size=5000).astype(np.uint8).tobytes()
+    big = rng.choice(list(b"ACGT"), size=16000).astype(np.uint8).tobytes()
+    copy = bytearray()
+    for ch in big[:9000]:
+        r = rng.random()
+        if r < 0.01:
+            continue
+        copy.append(rng.choice(list(b"ACGT")) if rng.random() < 0.05 else ch)
+    other = rng.choice(list(b"ACGT"), size=3000).astype(np.uint8).tobytes()
+    s = ca.SequenceSet.from_strings([fam.seq(i) for i in range(len(fam))] + [five, big, bytes(copy), other])
+    al = ca.Aligner(ednafull, 16, 4, schedule=schedule)
+    al.set_sequences(s)
+    got, gst, _ = al.compute_block(0, 1, 2, 6, False, stats=True)
+    want, wst = oracle_block(s, 0, 1, 2, 6, False, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "5,000-8,000 residues")
+    assert (wst[..., 0] > 32766).any(), "a related pair must exceed the int16 range"
+    got, gst, _ = al.compute_block(7, 7, 8, 9, False, stats=True)
+    want, wst = oracle_block(s, 7, 7, 8, 9, False, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "16,000 residues")
+    assert wst[0, 0, 0] > 32766
+    al.close()
+
+
 def _mutated_family(rng, alphabet, n, lo, hi, sub, indel):
     """n copies of one random ancestor with substitutions and indels, plus
     as many unrelated sequences: related pairs align over most of their
@@ -247,7 +282,7 @@ def _mutated_family(rng, alphabet, n, lo, hi, sub, indel):
             s.append(rng.choice(list(alphabet)) if rng.random() < sub else ch)
         out.append(bytes(s))
     for _ in range(n):
-        out.append(bytes(rng.choice(list(alphabet), size=int(rng.integers(lo, hi + 1)))))
+        out.append(rng.choice(list(alphabet), size=int(rng.integers(lo, hi + 1))).astype(np.uint8).tobytes())
     return ca.SequenceSet.from_strings(out)
 
 
