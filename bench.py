@@ -35,9 +35,10 @@ ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture"
            "rect64": (24, "int32"),
            "one_pass": (24, "int32"), "lpp16": (24, "int32"), "global": (30, "int32")}
 # VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
-# unrolled), counted from this build's gfx950 assembly by
-# `python -m csharp_amd.isa_valu_count`; a packed instruction counts once for
-# the two cells it evaluates.  Reported for information.
+# unrolled), counted by hand from this build's gfx950 assembly of
+# swg_kernels.hip (hipcc -S: the loop body's VALU instructions over the cells
+# it evaluates); a packed instruction counts once for the two cells it
+# evaluates.  Reported for information.
 # The reverse pass evaluates a data-dependent band (the DP cells that can lie
 # on an optimal path back from the maximum), so its work is the cells it
 # evaluates, not the prefix rectangles it decides.

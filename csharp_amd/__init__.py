@@ -52,7 +52,7 @@ HOST_SYMBOLS = (
     "salsa_range_partition", "salsa_block_is_computed", "salsa_compute_rank", "salsa_compute_full_matrix",
     "salsa_write_matrix", "salsa_write_swms_block", "salsa_load_scoring_matrix",
     "salsa_synth_dna", "salsa_synth_protein", "salsa_swms_compute_rank", "salsa_read_restart_file",
-    "salsa_read_matrix_rows", "salsa_read_partial_stripes", "salsa_distance_value",
+    "salsa_read_matrix_rows", "salsa_read_partial_stripes", "salsa_distance_value", "salsa_write_index_file",
 )
 
 
@@ -155,6 +155,7 @@ def host():
         H.salsa_compute_rank.argtypes = [vp, i32, i32, i32, vp, i32, ctypes.POINTER(i64)]
         H.salsa_compute_full_matrix.argtypes = [vp, i32, i32, vp, i32, ctypes.POINTER(i64)]
         H.salsa_write_matrix.argtypes = [ctypes.c_char_p, vp, i64]
+        H.salsa_write_index_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), vp, i32, i32]
         H.salsa_write_swms_block.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, ctypes.c_char_p, ctypes.c_char_p,
                                              i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_char_p), vp, i64, i32]
         H.salsa_load_scoring_matrix.argtypes = [ctypes.c_char_p, vp]
@@ -550,6 +551,16 @@ def write_matrix(path, data):
     partial stripe as PartialMatrixBinaryWriter writes it)."""
     data = np.ascontiguousarray(data)
     if host().salsa_write_matrix(path.encode(), _ptr(data), data.nbytes) != SWG_OK:
+        raise SwgError(SWG_E_ARGUMENT, "cannot write %s" % path)
+
+
+def write_index_file(path, seqs, labels=None, crlf=True):
+    """WriteIndexFile (Salsa.SmithWatermanMPI/Program.cs:316-331): one line
+    "<index>\t<label>\t<length>" per sequence."""
+    n = len(seqs)
+    labels = labels if labels is not None else (seqs.labels or [""] * n)
+    arr = (ctypes.c_char_p * max(n, 1))(*[x.encode("utf-8") for x in labels])
+    if host().salsa_write_index_file(path.encode(), arr, _ptr(seqs.offsets), n, int(bool(crlf))) != SWG_OK:
         raise SwgError(SWG_E_ARGUMENT, "cannot write %s" % path)
 
 

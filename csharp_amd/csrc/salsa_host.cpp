@@ -110,6 +110,24 @@ int salsa_write_matrix(const char* path, const void* data, int64_t nbytes)
     return w == (size_t)nbytes ? SWG_OK : SWG_E_ARGUMENT;
 }
 
+// WriteIndexFile (Salsa.SmithWatermanMPI/Program.cs:316-331): one line per
+// sequence, "<index>\t<label>\t<residue count>", as StreamWriter.WriteLine
+// writes it (UTF-8 without a byte-order mark; the line end is the platform's,
+// "\r\n" on the Windows builds of the reference).
+int salsa_write_index_file(const char* path, const char* const* labels, const int64_t* offsets, int32_t n,
+                           int32_t crlf)
+{
+    if (!path || n < 0 || (n > 0 && (!labels || !offsets))) return SWG_E_ARGUMENT;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return SWG_E_ARGUMENT;
+    bool ok = true;
+    for (int32_t i = 0; i < n && ok; ++i)
+        ok = std::fprintf(f, "%d\t%s\t%lld%s", i, labels[i] ? labels[i] : "",
+                          (long long)(offsets[i + 1] - offsets[i]), crlf ? "\r\n" : "\n") > 0;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? SWG_OK : SWG_E_ARGUMENT;
+}
+
 // MatrixBinaryReader.ReadInt16/ReadDouble(fileName, startRowIndex, rowCount)
 // (Salsa.CoreTPL/Blas/MatrixBinaryReader.cs:224-311, 290-309): seek to
 // startRow * n_cols elements, then row_count rows of n_cols TDistance.

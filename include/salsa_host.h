@@ -49,6 +49,13 @@ int salsa_compute_full_matrix(swg_ctx* ctx, int32_t n, int32_t world, void* full
 /* Raw row-major matrix file (MatrixBinaryWriter / WriteFullMatrixOnRank0). */
 int salsa_write_matrix(const char* path, const void* data, int64_t nbytes);
 
+/* The index file rank 0 writes beside the matrix (WriteIndexFile,
+ * Salsa.SmithWatermanMPI/Program.cs:194,316-331): line i is
+ * "<i>\t<labels[i]>\t<offsets[i+1]-offsets[i]>", UTF-8 labels, line end
+ * "\r\n" (crlf = 1, the reference's Windows StreamWriter) or "\n". */
+int salsa_write_index_file(const char* path, const char* const* labels, const int64_t* offsets, int32_t n,
+                           int32_t crlf);
+
 /* One SWMS block file "<rank>-<j>.bin" (header of Program.cs:1116-1160 then
  * the block's TDistance values row-major, little endian). names[4]: sequence
  * names at row start, row end, column start, column end. */

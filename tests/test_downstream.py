@@ -79,3 +79,16 @@ def test_int16_distance_round_trip_through_consumer():
         stored = int(np.float32(np.float32(1.0) - pct) * np.float32(32767.0))
         d = ca.distance_value(stored, ca.PWC)
         assert 0.0 <= (1.0 - float(pct)) - d < 1.0 / 32767.0 + 1e-12
+
+
+def test_index_file(tmp_path):
+    # WriteIndexFile (Salsa.SmithWatermanMPI/Program.cs:316-331): "{0}\t{1}\t{2}"
+    # of index, label and residue count, one StreamWriter line each
+    seqs = ca.SequenceSet.from_strings([b"ACGT", b"", b"GATTACA"], labels=["seq one", "s2", "éx"])
+    p = str(tmp_path / "index.txt")
+    ca.write_index_file(p, seqs)
+    assert open(p, "rb").read() == "0\tseq one\t4\r\n1\ts2\t0\r\n2\téx\t7\r\n".encode("utf-8")
+    ca.write_index_file(p, seqs, crlf=False)
+    assert open(p, "rb").read() == "0\tseq one\t4\n1\ts2\t0\n2\téx\t7\n".encode("utf-8")
+    ca.write_index_file(p, ca.SequenceSet.from_strings([]))
+    assert open(p, "rb").read() == b""
