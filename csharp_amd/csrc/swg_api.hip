@@ -251,6 +251,7 @@ struct swg_ctx {
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;   // int32 kernels' strip (re-run strip of the two-pass); 0 = autotuned
     int32_t score_strip = 0;  // score pass strip; 0 = autotuned
+    int32_t reverse_strip = 16;  // reverse pass strip (SWG_OPT_REVERSE_STRIP_ROWS)
     double tune_min_cells = 5e10;  // smaller blocks run the key's current candidate untimed
     // autotuner cache (entries are never removed while the deque lives)
     std::deque<TuneEntry> tuned;
@@ -312,7 +313,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
     int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
-    int r = v == kVarI16 ? reverse_blocks_per_cu(c->ncode) : 0;
+    int r = v == kVarI16 ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
     int q = v == kVarI16 ? rect64_blocks_per_cu(c->ncode) : 0;
     if (a <= 0 || (v == kVarI16 && (b <= 0 || r <= 0 || q <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
@@ -553,6 +554,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         ST.best_score = B.best_score.p;
         // sort 1: by first-maximum cell, for the reverse pass
         P.best_cell = B.best_cell.p; P.start_cell = nullptr; P.desc = B.desc.p; P.len = c->len.p;
+        P.best_score = B.best_score.p;
         P.meta = B.sortbuf.p; P.count = B.sortbuf.p + kMetaWords;
         P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
         RV.sorted = B.sorted.p; RV.best_cell = B.best_cell.p; RV.best_score = B.best_score.p;
@@ -619,7 +621,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
             if (e == hipSuccess) e = after(SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
-            if (e == hipSuccess) e = launch_reverse(S, RV, W, n3, st);
+            if (e == hipSuccess) e = launch_reverse(S, RV, W, c->reverse_strip, n3, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_REVERSE);
             if (e == hipSuccess) e = launch_sort(P2, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
@@ -659,6 +661,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
+        append_key(key, c->reverse_strip);
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -1453,6 +1456,12 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
             return fail(SWG_E_ARGUMENT, "score-pass strip rows are 0 (autotuned), 16, 24 or 32");
         if (value != c->score_strip) c->tuned.clear();
         c->score_strip = (int32_t)value;
+        return SWG_OK;
+    }
+    if (option == SWG_OPT_REVERSE_STRIP_ROWS) {
+        if (value != 0 && value != 8 && value != 16)
+            return fail(SWG_E_ARGUMENT, "reverse-pass strip rows are 0 (default 16), 8 or 16");
+        c->reverse_strip = value == 0 ? 16 : (int32_t)value;
         return SWG_OK;
     }
     if (option == SWG_OPT_TUNE_MIN_CELLS) {

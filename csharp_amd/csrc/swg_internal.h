@@ -128,8 +128,9 @@ constexpr int kMetaWords = 64;         // zeroed with the bucket counts before e
 
 struct SortSet {
     const uint32_t* best_cell;    // [rows * ld] of the score-pass desc
-    const uint32_t* start_cell;   // null: key = first-maximum cell (bi, jb); else key = the re-run
-                                  // rectangle [si..bi] x [sj..min(jb+kFirstMaxBlock-1, Lb)]
+    const uint32_t* start_cell;   // null: key = (first-maximum score, min(bi, wend)) for the reverse pass;
+                                  // else key = the re-run rectangle [si..bi] x [sj..min(jb+kFirstMaxBlock-1, Lb)]
+    const int16_t* best_score;    // the first-maximum score of each pair (reverse-pass key)
     const BlockDesc* desc;        // desc[0]: output (mirror), desc[1]: score pass (r0, c0, ld, rows, wr, wc)
     const int32_t* len;           // sequence lengths
     int32_t* meta;                // kMetaWords words, then the bucket counts
@@ -192,9 +193,10 @@ int rerun_blocks_per_cu(int strip_rows, int ncode);
 // Re-run of the big rectangles, one wave per pair.
 hipError_t launch_rect64(const SeqSet& S, const RerunSet& T, const Scratch& W, int nslots, hipStream_t stream);
 int rect64_blocks_per_cu(int ncode);
-// Reverse pass (strip_rows 16).
-hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t stream);
-int reverse_blocks_per_cu(int ncode);
+// Reverse pass; strip_rows in {8, 16}.
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, int nslots,
+                          hipStream_t stream);
+int reverse_blocks_per_cu(int strip_rows, int ncode);
 // Sixteen-lanes-per-pair kernel: tiles of kLppTile column sequences;
 // strip_rows (rows per lane) in {4, 8}.
 constexpr int kLppTile = 4;

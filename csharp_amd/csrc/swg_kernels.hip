@@ -864,9 +864,25 @@ __device__ __forceinline__ int side_bucket(int x, int w, bool fine)
     return min(32 + (x - 64) / wq, kBucketDim - 1);
 }
 
+// Bucket of a first-maximum score S (the reverse pass's key): one per score
+// below 96, then four per doubling, the last bucket holding the rest.
+__device__ __forceinline__ int score_bucket(int s)
+{
+    if (s < 96) return max(s, 0);
+    const int lg = 31 - __clz((unsigned)(s / 96));             // floor(log2(S / 96))
+    const int sub = (int)(((int64_t)(s - (96 << lg)) * 4) / (96 << lg));  // quarter of the octave
+    return min(96 + 4 * lg + sub, kBucketDim - 1);
+}
+
 // Bucket of the pair at `pos` and the DP cells of the rectangle its next pass
 // evaluates (the reverse pass: bi x wend; the re-run: (bi-si+1) x (wend-sj+1);
 // a flagged pair: La x Lb).
+// The reverse pass's cost per pair is its band: it starts at the
+// first-maximum block and runs back along the alignment for about as many
+// rows as the score S allows (capped by bi and wend), wherever the block
+// lies; its tiles are therefore keyed on (S, min(bi, wend)), so that the
+// pairs of a wave have bands of similar length and the queue starts with
+// the longest.  The re-run's cost is its rectangle, which is its key.
 __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1, int64_t pos, int Lb,
                                            uint64_t* cells, bool* flagged)
 {
@@ -886,9 +902,10 @@ __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1
         w = wend - (int)(st & 0xFFFFu) + 1;
     }
     *cells = (uint64_t)h * (uint64_t)w;
-    const bool fine = P.start_cell != nullptr;
-    if (fine && (int64_t)*cells > kBigRect) return kBigBucket;
-    return side_bucket(h, D1.wr, fine) * kBucketDim + side_bucket(w, D1.wc, fine);
+    if (!P.start_cell)
+        return score_bucket((int)P.best_score[pos]) * kBucketDim + side_bucket(min(bi, wend), max(D1.wr, D1.wc), false);
+    if ((int64_t)*cells > kBigRect) return kBigBucket;
+    return side_bucket(h, D1.wr, true) * kBucketDim + side_bucket(w, D1.wc, true);
 }
 
 // Slot order: rank 0 = the flagged list, rank 1 = the big rectangles, then
@@ -1444,15 +1461,44 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
 // the sorted list.  Saturation cannot create a hit: a path through a value
 // clamped at -32768 would need a segment scoring >= 32768 + S to climb back
 // to S, more than any local alignment of the pair scores (S < 32766).
-template <int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
+// Each lane's two pairs have their own row sequences, so the scores come from
+// a table indexed by residue codes: for alphabets of at most
+// kRevJointMaxCodes codes (JOINT) a joint table jt[ra][rb][ca][cb] holds both
+// halves' scores packed, one 32-bit read per row and column; otherwise the
+// [c1][c2] table, one 16-bit read per half and a pack.
+constexpr int kRevJointMaxCodes = 7;  // 7^4 words = 9.6 KB of LDS
+
+// LDS of the reverse pass: the [c1][c2] table (int16), padded to a word,
+// then the joint table
+size_t reverse_lds_bytes(int ncode)
 {
-    extern __shared__ short tab16[];  // [ncode][ncode]: score + gap_open, clamped to int16
+    const size_t n2 = (size_t)ncode * ncode;
+    return ((n2 + 1) & ~size_t(1)) * sizeof(short) + (ncode <= kRevJointMaxCodes ? n2 * n2 * sizeof(uint32_t) : 0);
+}
+
+// Waves per SIMD of the reverse pass: three for 8-row strips with the joint
+// table (158 VGPRs), else two.
+constexpr int reverse_waves(int R, bool joint) { return R == 8 && joint ? 3 : 2; }
+
+template <int R, bool JOINT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(reverse_waves(R, JOINT), reverse_waves(R, JOINT))))
+void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
+{
+    extern __shared__ short tab16[];  // [ncode][ncode]: score + gap_open, clamped to int16; then jt
     const int lane = threadIdx.x;
-    const int ncode = S.ncode, pad = ncode - 1;
-    for (int k = lane; k < ncode * ncode; k += kWave) {
+    const int ncode = S.ncode, pad = ncode - 1, ncode2 = ncode * ncode;
+    for (int k = lane; k < ncode2; k += kWave) {
         const int v = S.prof[k].x;
         tab16[k] = (short)(v < kPad16 ? kPad16 : v);
+    }
+    uint32_t* jt = reinterpret_cast<uint32_t*>(tab16 + ((ncode2 + 1) & ~1));
+    if (JOINT) {
+        __syncthreads();
+        for (int k = lane; k < ncode2 * ncode2; k += kWave) {
+            const int rr = k / ncode2, cc = k - rr * ncode2;   // rr = ra * ncode + rb, cc = ca * ncode + cb
+            const int ra = rr / ncode, rb = rr - ra * ncode, ca = cc / ncode, cb = cc - ca * ncode;
+            jt[k] = (uint32_t)(uint16_t)tab16[ra * ncode + ca] | ((uint32_t)(uint16_t)tab16[rb * ncode + cb] << 16);
+        }
     }
     __syncthreads();
 
@@ -1540,12 +1586,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         uint32_t done = 0;  // per half: 0xFFFF once a row without live cells (or past bi) was seen
         uint32_t hitr = 0, hitc = 0;
         for (int i0 = 0; i0 < Rmax; i0 += R) {
+            // per row: JOINT, the table row of the codes pair (ra, rb); else
+            // the [c1] rows of both halves
             int rba[R], rbb[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int ka = bi[0] - 1 - (i0 + r), kb = bi[1] - 1 - (i0 + r);
-                rba[r] = (ka >= 0 ? (int)s1a[ka] : pad) * ncode;
-                rbb[r] = (kb >= 0 ? (int)s1b[kb] : pad) * ncode;
+                const int ra = ka >= 0 ? (int)s1a[ka] : pad, rb = kb >= 0 ? (int)s1b[kb] : pad;
+                if (JOINT) {
+                    rba[r] = (ra * ncode + rb) * ncode2;
+                    rbb[r] = 0;
+                } else {
+                    rba[r] = ra * ncode;
+                    rbb[r] = rb * ncode;
+                }
             }
             uint32_t tr[R], Hr[R], rm[R];
 #pragma unroll
@@ -1566,9 +1620,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
             uint32_t wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
             // both pairs' scores of a column, packed, loaded one column ahead
             auto col_scores = [&](int ca, int cb, uint32_t* out) {
+                if (JOINT) {
+                    const int cp = ca * ncode + cb;
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    out[r] = (uint32_t)(uint16_t)tab16[rba[r] + ca] | ((uint32_t)(uint16_t)tab16[rbb[r] + cb] << 16);
+                    for (int r = 0; r < R; ++r) out[r] = jt[rba[r] + cp];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        out[r] = (uint32_t)(uint16_t)tab16[rba[r] + ca] | ((uint32_t)(uint16_t)tab16[rbb[r] + cb] << 16);
+                }
             };
             uint32_t cur[R];
             col_scores(wend[0] >= jlo ? (int)(wda >> 24) : pad, wend[1] >= jlo ? (int)(wdb >> 24) : pad, cur);
@@ -1954,18 +2014,42 @@ int rect64_blocks_per_cu(int ncode)
     return err == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int nslots, hipStream_t st)
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, int nslots,
+                          hipStream_t st)
 {
-    const size_t lds = (size_t)S.ncode * S.ncode * sizeof(short);
-    hipLaunchKernelGGL((reverse_kernel<16>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+    const size_t lds = reverse_lds_bytes(S.ncode);
+    const bool joint = S.ncode <= kRevJointMaxCodes;
+    switch (strip_rows) {
+    case 8:
+        if (joint) hipLaunchKernelGGL((reverse_kernel<8, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((reverse_kernel<8, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    case 16:
+        if (joint) hipLaunchKernelGGL((reverse_kernel<16, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        else hipLaunchKernelGGL((reverse_kernel<16, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-int reverse_blocks_per_cu(int ncode)
+int reverse_blocks_per_cu(int strip_rows, int ncode)
 {
-    const size_t lds = (size_t)ncode * ncode * sizeof(short);
+    const size_t lds = reverse_lds_bytes(ncode);
+    const bool joint = ncode <= kRevJointMaxCodes;
     int nb = 0;
-    const hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16>, kWave, lds);
+    hipError_t err = hipErrorInvalidValue;
+    switch (strip_rows) {
+    case 8:
+        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<8, true>, kWave, lds)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<8, false>, kWave, lds);
+        break;
+    case 16:
+        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16, true>, kWave, lds)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16, false>, kWave, lds);
+        break;
+    default: break;
+    }
     return err == hipSuccess ? nb : 0;
 }
 

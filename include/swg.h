@@ -202,6 +202,10 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
 /* SWG_OPT_TUNE_MIN_CELLS = DP cells a block needs for the autotuner to time
  * it (default 5e10); smaller blocks run the key's current candidate untimed. */
 #define SWG_OPT_TUNE_MIN_CELLS 4
+/* SWG_OPT_REVERSE_STRIP_ROWS = rows per strip of the two-pass schedule's
+ * reverse score pass (8 or 16; 0 = default 16: measured 54 against 84 ms on
+ * a 3125 x 3125 block of the 50,000-sequence DNA workload). */
+#define SWG_OPT_REVERSE_STRIP_ROWS 5
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the

@@ -116,3 +116,24 @@ def test_autotuner_global_strips(wide_set, ednafull):
     assert [shape(t) for t in trials] == [(ca.SCHED_PERSISTENT_I32, 16, 0), (ca.SCHED_PERSISTENT_I32, 8, 0)]
     assert shape(final) in [shape(t) for t in trials]
     al.close()
+
+
+@pytest.mark.parametrize("reverse_strip", [8, 16])
+def test_reverse_strip_heights(wide_set, wide_expected, ednafull, reverse_strip):
+    # the two-pass schedule's reverse pass at both strip heights: unrelated
+    # pairs (short bands) and related families (long bands, big rectangles)
+    say("reverse strip %d" % reverse_strip)
+    al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
+    al.set_reverse_strip(reverse_strip)
+    al.set_sequences(wide_set)
+    got, _, _ = al.compute_block(0, 2999, 3000, 3255, False)
+    assert np.array_equal(got, wide_expected)
+    fam = ca.synth_dna(62, 400, 4, 0.05, 0.02, 0, [300, 900])  # 4 ancestors: most pairs related
+    al.set_sequences(fam)
+    got, gst, _ = al.compute_block(0, 199, 200, 399, False, stats=True)
+    a, b = np.meshgrid(np.arange(0, 200), np.arange(200, 400), indexing="ij")
+    d, st = O.sw_pairs_mt(fam.residues, fam.offsets, a.ravel(), b.ravel(), ednafull, 16, 4, threads=THREADS,
+                          want_stats=True)
+    assert np.array_equal(got.ravel(), d)
+    assert np.array_equal(gst.reshape(-1, 4)[:, :3], st[:, :3])
+    al.close()

@@ -506,5 +506,10 @@ def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
             assert c == block
         if name in ("rerun_capture", "reverse"):
             assert 0 < c < block
-        assert run >= c
+        if name == "reverse":
+            # charged with the prefix rectangles it decides; it evaluates only
+            # their band (the waves' unions of the live cells)
+            assert 0 < run
+        else:
+            assert run >= c
     al.close()
