@@ -35,7 +35,8 @@ TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 PROGRAM_SWMPI, PROGRAM_SWMS = 0, 1
 SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
-OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING, OPT_SCORE_STRIP_ROWS, OPT_TUNE_MIN_CELLS, OPT_REVERSE_STRIP_ROWS = 1, 2, 3, 4, 5
+(OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING, OPT_SCORE_STRIP_ROWS, OPT_TUNE_MIN_CELLS, OPT_REVERSE_STRIP_ROWS,
+ OPT_REVERSE_ROW_CAP) = 1, 2, 3, 4, 5, 6
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
                 7: "global", 8: "reverse", 9: "rect64", 10: "distance"}
 MATRIX_SIZE = 127
@@ -363,6 +364,11 @@ class Aligner:
     def set_reverse_strip(self, rows):
         """Rows per strip of the reverse score pass (8 or 16; 0 = default)."""
         _check(lib().swg_set_option(self._h, OPT_REVERSE_STRIP_ROWS, int(rows)))
+
+    def set_reverse_row_cap(self, rows):
+        """Reverse rows per pair at most (a multiple of 16): pairs whose band
+        reaches past it are re-run over their whole prefix."""
+        _check(lib().swg_set_option(self._h, OPT_REVERSE_ROW_CAP, int(rows)))
 
     def set_tune_min_cells(self, cells):
         """DP cells a block needs for the autotuner to time it."""

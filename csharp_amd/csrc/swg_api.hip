@@ -252,6 +252,7 @@ struct swg_ctx {
     int32_t strip_rows = 0;   // int32 kernels' strip (re-run strip of the two-pass); 0 = autotuned
     int32_t score_strip = 0;  // score pass strip; 0 = autotuned
     int32_t reverse_strip = 16;  // reverse pass strip (SWG_OPT_REVERSE_STRIP_ROWS)
+    int32_t reverse_row_cap = 512;  // reverse rows per pair at most (SWG_OPT_REVERSE_ROW_CAP)
     double tune_min_cells = 5e10;  // smaller blocks run the key's current candidate untimed
     // autotuner cache (entries are never removed while the deque lives)
     std::deque<TuneEntry> tuned;
@@ -559,6 +560,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
         RV.sorted = B.sorted.p; RV.best_cell = B.best_cell.p; RV.best_score = B.best_score.p;
         RV.start_cell = B.start_cell.p; RV.desc = B.desc.p; RV.meta = P.meta; RV.queue = B.queue.p + kQueueReverse;
+        RV.row_cap = c->reverse_row_cap;
         // sort 2: by re-run rectangle
         P2 = P;
         P2.start_cell = B.start_cell.p;
@@ -1462,6 +1464,12 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         if (value != 0 && value != 8 && value != 16)
             return fail(SWG_E_ARGUMENT, "reverse-pass strip rows are 0 (default 16), 8 or 16");
         c->reverse_strip = value == 0 ? 16 : (int32_t)value;
+        return SWG_OK;
+    }
+    if (option == SWG_OPT_REVERSE_ROW_CAP) {
+        if (value < 16 || value > 32768 || value % 16)
+            return fail(SWG_E_ARGUMENT, "the reverse-pass row cap is a multiple of 16 in [16, 32768]");
+        c->reverse_row_cap = (int32_t)value;
         return SWG_OK;
     }
     if (option == SWG_OPT_TUNE_MIN_CELLS) {

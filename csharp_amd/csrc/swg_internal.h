@@ -148,6 +148,8 @@ struct ReverseSet {
     const uint32_t* sorted;
     const uint32_t* best_cell;
     const int16_t* best_score;
+    int32_t row_cap;              // reverse rows evaluated at most (a multiple of the strip height); a pair
+                                  // whose band is still live there gets its whole prefix as rectangle
     uint32_t* start_cell;
     const BlockDesc* desc;        // as SortSet
     int32_t* meta;

@@ -1455,6 +1455,11 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
 // there (the diagonal candidate of a DIAGONAL cell and the gap chain of a gap
 // run lie on the path; the other candidates can only be lower).
 // No hit (only possible when S = 0): the whole prefix rectangle.
+// Row cap: the pass stops after T.row_cap reverse rows; a pair whose band is
+// still live there could start anywhere above, so it gets its whole prefix
+// [1..bi] x [1..wend] as rectangle -- always a valid one (its values are the
+// full DP's).  This bounds the pass's longest tasks, the pairs with the
+// longest alignments, whose rectangle is often their prefix anyway.
 //
 // Two pairs per lane in the int16 halves of one register (v_pk_* with
 // saturation): pair A = slot 128t + lane, pair B = slot 128t + 64 + lane of
@@ -1585,7 +1590,8 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
         int jlo = 1, jtop = kFirstMaxBlock, live_hi = kFirstMaxBlock - 1;
         uint32_t done = 0;  // per half: 0xFFFF once a row without live cells (or past bi) was seen
         uint32_t hitr = 0, hitc = 0;
-        for (int i0 = 0; i0 < Rmax; i0 += R) {
+        int rows_done = 0;  // reverse rows evaluated
+        for (int i0 = 0; i0 < Rmax && i0 < T.row_cap; i0 += R) {
             // per row: JOINT, the table row of the codes pair (ra, rb); else
             // the [c1] rows of both halves
             int rba[R], rbb[R];
@@ -1717,6 +1723,7 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                 done |= dead | out;
             }
             run += 2ull * R * (uint64_t)(jend - jlo + 1) * kWave;
+            rows_done = i0 + R;
             // no live cell in the strip's last row, or every pair past a row
             // without one: no later row holds a hit
             if (lo_next == 0 || __builtin_amdgcn_ballot_w64(done != 0xFFFFFFFFu) == 0) break;
@@ -1728,8 +1735,11 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
         for (int hh = 0; hh < 2; ++hh) {
             if (live[hh]) {
                 const int hr = (int)((hitr >> (16 * hh)) & 0xFFFFu), hc = (int)((hitc >> (16 * hh)) & 0xFFFFu);
-                const int si = (hr > 0 && hc > 0) ? bi[hh] - hr + 1 : 1;
-                const int sj = (hr > 0 && hc > 0) ? wend[hh] - hc + 1 : 1;
+                // stopped by the row cap with live cells left: the whole prefix
+                const bool capped = bi[hh] > rows_done && ((done >> (16 * hh)) & 0xFFFFu) != 0xFFFFu;
+                const bool found = hr > 0 && hc > 0 && !capped;
+                const int si = found ? bi[hh] - hr + 1 : 1;
+                const int sj = found ? wend[hh] - hc + 1 : 1;
                 T.start_cell[p[hh]] = ((uint32_t)si << 16) | (uint32_t)sj;
             }
         }

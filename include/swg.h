@@ -206,6 +206,11 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * reverse score pass (8 or 16; 0 = default 16: measured 54 against 84 ms on
  * a 3125 x 3125 block of the 50,000-sequence DNA workload). */
 #define SWG_OPT_REVERSE_STRIP_ROWS 5
+/* SWG_OPT_REVERSE_ROW_CAP = reverse rows the two-pass schedule's reverse pass
+ * evaluates per pair at most (a multiple of 16; default 512); a pair whose
+ * band of possible alignment starts reaches past it is re-run over its whole
+ * prefix (exact either way; bounds the pass's longest tasks). */
+#define SWG_OPT_REVERSE_ROW_CAP 6
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the

@@ -118,13 +118,15 @@ def test_autotuner_global_strips(wide_set, ednafull):
     al.close()
 
 
-@pytest.mark.parametrize("reverse_strip", [8, 16])
-def test_reverse_strip_heights(wide_set, wide_expected, ednafull, reverse_strip):
+@pytest.mark.parametrize("reverse_strip,row_cap", [(8, 512), (16, 512), (16, 16), (16, 64)])
+def test_reverse_strip_heights(wide_set, wide_expected, ednafull, reverse_strip, row_cap):
     # the two-pass schedule's reverse pass at both strip heights: unrelated
-    # pairs (short bands) and related families (long bands, big rectangles)
-    say("reverse strip %d" % reverse_strip)
+    # pairs (short bands) and related families (long bands, big rectangles);
+    # small row caps send most pairs to their whole prefix instead
+    say("reverse strip %d row cap %d" % (reverse_strip, row_cap))
     al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
     al.set_reverse_strip(reverse_strip)
+    al.set_reverse_row_cap(row_cap)
     al.set_sequences(wide_set)
     got, _, _ = al.compute_block(0, 2999, 3000, 3255, False)
     assert np.array_equal(got, wide_expected)

@@ -242,3 +242,24 @@ def test_band_threshold_is_not_slack(ednafull):
         ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, 16)[:2]
         lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, 16, live_min=4) != ref
     assert lost > 0
+
+
+@pytest.mark.parametrize("o,e", [(16, 4), (0, 0), (2, 5)])
+def test_prefix_rectangle_is_exact(ednafull, o, e):
+    """The reverse pass's row cap gives a pair whose band is still live at
+    the cap its whole prefix [1..bi] x [1..wend] as rectangle: the local DP
+    of the prefix alone gives the full pair's score, first maximum and
+    statistics (its values are the full DP's: a cell depends only on the
+    cells above and to its left)."""
+    rng = np.random.default_rng(31 + o)
+    for s1, s2 in _long_pairs(rng, 30, 20, 200):
+        full = O.sw_literal(s1, s2, ednafull, o, e)
+        best, bi, jb = _rect(s1, s2, ednafull, o, e)
+        if len(s1) == 0 or len(s2) == 0:
+            continue
+        wend = min(jb + 15, len(s2))
+        sub = O.sw_literal(s1[:bi], s2[:wend], ednafull, o, e)
+        assert int(sub["score"]) == best
+        assert (sub["best_row"], sub["best_col"]) == (full["best_row"], full["best_col"])
+        for k in ("columns", "identity", "similarity", "gaps"):
+            assert sub[k] == full[k], (k, s1, s2)
