@@ -306,10 +306,10 @@ int pairs_per_wave(int v) { return v == kVarI16 ? kScoreTile : v == kVarLpp ? kL
 // the CUs: n1 first kernel, n2 re-run, n3 reverse pass and n4 big-rectangle
 // re-run (two-pass variant).
 int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, int* n2, int* n3 = nullptr,
-              int* n4 = nullptr)
+              int* n4 = nullptr, bool nw_region = true)
 {
     const int v = f.v, s1 = f.s1, s2 = f.s2;
-    int a = c->nw        ? nw_blocks_per_cu(s1)
+    int a = c->nw        ? nw_blocks_per_cu(s1, nw_region)
           : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
                          : sw_blocks_per_cu(s1, false);
@@ -468,8 +468,12 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
                 int64_t ld, int32_t* dev_stats, Plan& plan)
 {
     const int v = f.v, s1 = f.s1, s2 = f.s2;
+    // the global kernel carries the aligned-region word only where it is read:
+    // nucleic PercentIdentity, or the statistics (their fourth field)
+    const bool nw_region =
+        c->nw && ((!c->count_similarity && c->dist_kind == kDistFraction) || dev_stats != nullptr);
     int n1, n2, n3, n4;
-    int rc = slots_for(c, ln, f, &n1, &n2, &n3, &n4);
+    int rc = slots_for(c, ln, f, &n1, &n2, &n3, &n4, nw_region);
     if (rc) return rc;
     int want = std::max(std::max(n1, n2), v == kVarI16 ? std::max(n3, n4) : 0);
     rc = ensure_scratch(c, ln, &want);
@@ -602,9 +606,9 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         if (e != hipSuccess) return e;
         const bool persist = v != kVarPlain;
         if (c->nw) {
-            e = launch_nw(S, T, W, s1, persist, n1, st);
+            e = launch_nw(S, T, W, s1, nw_region, persist, n1, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_GLOBAL);
-            if (e == hipSuccess && kimura) e = launch_nw(S2, T2, W, s1, persist, n1, st);
+            if (e == hipSuccess && kimura) e = launch_nw(S2, T2, W, s1, nw_region, persist, n1, st);
             if (e == hipSuccess && kimura) e = after(SWG_KERNEL_GLOBAL);
         } else if (v == kVarI32 || v == kVarPlain) {
             e = launch_sw(S, T, W, s1, persist, n1, st);
@@ -663,7 +667,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
-        append_key(key, c->reverse_strip);
+        append_key(key, c->reverse_strip); append_key(key, nw_region);
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {

@@ -181,7 +181,9 @@ struct Scratch {
 // otherwise grid = nslots with static round-robin over tasks.
 hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
                      bool persistent, int nslots, hipStream_t stream);
-hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows,
+// Global kernel; region: carry the aligned-region word (nucleic
+// PercentIdentity, or statistics requested).
+hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool region,
                      bool persistent, int nslots, hipStream_t stream);
 // Score-only int16x2 pass; strip_rows in {16, 24, 32}.
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
@@ -208,7 +210,7 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
 int score16_blocks_per_cu(int strip_rows, int ncode);
-int nw_blocks_per_cu(int strip_rows);
+int nw_blocks_per_cu(int strip_rows, bool region);
 // Distances of a block from the count words in aux (Jukes-Cantor, Kimura2,
 // MinMaxNormScore): desc[0] is the output desc (rows, aux_ld = columns); the
 // grid (256-thread workgroups) strides over the block, whatever its size.

@@ -411,11 +411,14 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
 // starts with and trail the run of same-direction gap columns it ends with.
 // A second packed word carries them: T2 = (lead << 16) | (up << 15) | run,
 // `run` = length of the gap run the traceback starts with (up = vertical).
+// REGION = false leaves that word out (protein PercentSimilarity over all
+// columns, the word-mode distances, no statistics requested): 12 of the
+// kernel's operations per cell and a third of its boundary traffic.
 constexpr uint32_t kUpBit = 0x8000u;
 
 __device__ __forceinline__ int nw_border(int o, int e, int k) { return -(o + (k - 1) * e); }
 
-template <int R, bool PERSIST>
+template <int R, bool PERSIST, bool REGION>
 __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
 {
     __shared__ int4 lprof[kMaxCodes * (R / 2 + 1)];
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
         // row 0: a horizontal gap run of j columns from the origin
         for (int j = 1; j <= ncols; ++j) {
             bnd[(int64_t)j * kWave + lane] = make_int4(nw_border(o, e, j) - o, kNeg, j * ginc, 0);
-            bnd2[(int64_t)j * kWave + lane] = make_int2((j << 16) | j, 0);
+            if (REGION) bnd2[(int64_t)j * kWave + lane] = make_int2((j << 16) | j, 0);
         }
 
         // result cell (La, Lb): border cells when a sequence is empty
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
                 int c2 = (int)((w >> (8 * ((j - 1) & 3))) & 0xFFu);
                 c2 = j > Lb ? pad : c2;
                 const int4 bu = bnd[(int64_t)j * kWave + lane];
-                const int2 bu2 = bnd2[(int64_t)j * kWave + lane];
+                const int2 bu2 = REGION ? bnd2[(int64_t)j * kWave + lane] : make_int2(0, 0);
                 int tu = bu.x, Gu = bu.y, Tu1 = bu.z, TGu1 = bu.w, Tu2 = bu2.x, TGu2 = bu2.y;
                 int dt = dtb, dT1 = dTb1, dT2 = dTb2;
                 dtb = tu; dTb1 = Tu1; dTb2 = Tu2;
@@ -504,22 +507,28 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
                         const bool cg = g1 > tu;
                         const int G = cg ? g1 : tu;
                         const int TG1 = (cg ? TGu1 : Tu1) + ginc;
-                        const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
-                        const int TG2 = (cg ? TGu2 : baseG) + 1;
+                        int TG2 = 0;
+                        if (REGION) {
+                            const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
+                            TG2 = (cg ? TGu2 : baseG) + 1;
+                        }
                         // horizontal gap
                         const int h1 = Hr[r] - e;
                         const bool ch = h1 > tr[r];
                         Hr[r] = ch ? h1 : tr[r];
-                        const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
                         TH1[r] = (ch ? TH1[r] : Tr1[r]) + ginc;
-                        TH2[r] = (ch ? TH2[r] : baseH) + 1;
+                        if (REGION) {
+                            const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
+                            TH2[r] = (ch ? TH2[r] : baseH) + 1;
+                        }
                         const int f = dt + sc;
                         const int V = max(max(f, G), Hr[r]);
                         int x1 = (V == G) ? TG1 : TH1[r];
-                        int x2 = (V == G) ? (TG2 | (int)kUpBit) : TH2[r];
+                        int x2 = 0;
+                        if (REGION) x2 = (V == G) ? (TG2 | (int)kUpBit) : TH2[r];
                         const bool dg = V == f;
                         x1 = dg ? dT1 + inc : x1;
-                        x2 = dg ? (dT2 & (int)0xFFFF0000u) : x2;
+                        if (REGION) x2 = dg ? (dT2 & (int)0xFFFF0000u) : x2;
                         const int tn = V - o;
                         if (r == rcap) {
                             capV = hitcol ? V : capV;
@@ -532,12 +541,13 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
                     }
                 }
                 bnd[(int64_t)j * kWave + lane] = make_int4(tu, Gu, Tu1, TGu1);
-                bnd2[(int64_t)j * kWave + lane] = make_int2(Tu2, TGu2);
+                if (REGION) bnd2[(int64_t)j * kWave + lane] = make_int2(Tu2, TGu2);
             }
         }
 
         if (b >= 0) {
             const uint32_t cols = cap1 >> 16, cnt = cap1 & 0xFFFFu;
+            if (!REGION) cap2 = 0;  // region not carried: reported as the whole alignment
             const uint32_t lead = cap2 >> 16, trail = cap2 & 0x7FFFu;
             // region = last - first + 1 over the non-gap indices; with both
             // sequences empty both indices are -1 and the region counts 1
@@ -1880,30 +1890,43 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
     }
 }
 
-hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool persistent, int nslots,
-                     hipStream_t st)
+template <int R, bool REGION>
+void launch_nw_r(const SeqSet& S, const TaskSet& T, const Scratch& W, bool persistent, int nslots, hipStream_t st)
+{
+    if (persistent) hipLaunchKernelGGL((nw_kernel<R, true, REGION>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else hipLaunchKernelGGL((nw_kernel<R, false, REGION>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+}
+
+hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool region, bool persistent,
+                     int nslots, hipStream_t st)
 {
     switch (strip_rows) {
     case 8:
-        if (persistent) hipLaunchKernelGGL((nw_kernel<8, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        else hipLaunchKernelGGL((nw_kernel<8, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        if (region) launch_nw_r<8, true>(S, T, W, persistent, nslots, st);
+        else launch_nw_r<8, false>(S, T, W, persistent, nslots, st);
         break;
     case 16:
-        if (persistent) hipLaunchKernelGGL((nw_kernel<16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        else hipLaunchKernelGGL((nw_kernel<16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+        if (region) launch_nw_r<16, true>(S, T, W, persistent, nslots, st);
+        else launch_nw_r<16, false>(S, T, W, persistent, nslots, st);
         break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-int nw_blocks_per_cu(int strip_rows)
+int nw_blocks_per_cu(int strip_rows, bool region)
 {
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
     switch (strip_rows) {
-    case 8: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<8, true>, kWave, 0); break;
-    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<16, true>, kWave, 0); break;
+    case 8:
+        err = region ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<8, true, true>, kWave, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<8, true, false>, kWave, 0);
+        break;
+    case 16:
+        err = region ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<16, true, true>, kWave, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_kernel<16, true, false>, kWave, 0);
+        break;
     default: break;
     }
     return err == hipSuccess ? nb : 0;

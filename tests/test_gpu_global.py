@@ -137,3 +137,33 @@ def test_nw_rejects_score_pass_schedules(dna_set, ednafull):
         with pytest.raises(ca.SwgError):
             al.compute_block(0, 3, 4, 7, False)
     al.close()
+
+
+@pytest.mark.parametrize("kind,dist,strip", [("protein", ca.DIST_PERCENT_IDENTITY, 16),
+                                             ("protein", ca.DIST_PERCENT_IDENTITY, 8),
+                                             ("dna", ca.DIST_PERCENT_IDENTITY, 16),
+                                             ("dna", ca.DIST_JUKES_CANTOR, 16),
+                                             ("dna", ca.DIST_KIMURA2, 8)])
+def test_nw_without_statistics(dna_set, protein_set, ednafull, blosum62, kind, dist, strip):
+    # without statistics the global kernel leaves out the aligned-region word
+    # wherever the distance does not read it (protein PercentSimilarity, the
+    # word-mode distances); the distances are those of the oracle, and the
+    # same as with statistics requested
+    say("nw without statistics %s dist=%d strip=%d" % (kind, dist, strip))
+    seqs, M, o, e = (protein_set, blosum62, 10, 1) if kind == "protein" else (dna_set, ednafull, 16, 4)
+    nucleic = kind == "dna"
+    al = ca.Aligner(M, o, e, mode=ca.MODE_NW_GLOBAL, distance_fn=dist,
+                    alignment_type=ca.ALIGN_NUCLEIC if nucleic else ca.ALIGN_PROTEIN)
+    al.set_schedule(ca.SCHED_PERSISTENT_I32, strip)
+    al.set_sequences(seqs)
+    n = len(seqs)
+    r0, r1, c0, c1 = 0, 49, 50, n - 1
+    got, _, pairs = al.compute_block(r0, r1, c0, c1, False)
+    with_stats, _, _ = al.compute_block(r0, r1, c0, c1, False, stats=True)
+    a, b = np.meshgrid(np.arange(r0, r1 + 1), np.arange(c0, c1 + 1), indexing="ij")
+    want, _ = O.nw_pairs_mt(seqs.residues, seqs.offsets, a.ravel(), b.ravel(), M, o, e, nucleic=nucleic,
+                            dist_fn=dist, threads=THREADS)
+    assert pairs == a.size
+    assert np.array_equal(got.ravel(), want), "%d distances differ" % (got.ravel() != want).sum()
+    assert np.array_equal(got, with_stats)
+    al.close()
