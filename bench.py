@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
+    ap.add_argument("--score-strip", type=int, default=0, help="int16 score-pass strip rows (0 = autotuned)")
     ap.add_argument("--reverse-strip", type=int, default=0, help="reverse-pass strip rows (0 = the default)")
     ap.add_argument("--full-pairs-per-block", type=int, default=200,
                     help="c3full: seeded pairs per block checked against the oracle")
@@ -158,6 +159,7 @@ def main():
     SCHED = {"auto": ca.SCHED_PERSISTENT, "i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
              "lpp16": ca.SCHED_PERSISTENT_LPP16, "plain": ca.SCHED_PLAIN}
     al.set_schedule(SCHED[args.schedule], args.strip)
+    al.set_score_strip(args.score_strip)
     al.set_reverse_strip(args.reverse_strip)
 
     def run_block(k):

@@ -294,7 +294,7 @@ VariantCfg default_cfg(const swg_ctx* c, int v)
     else if (v == kVarLpp) f.s1 = c->strip_rows == 8 ? 8 : 4;
     else if (v == kVarI32) f.s1 = c->strip_rows ? c->strip_rows : 16;
     else {
-        f.s1 = c->score_strip ? c->score_strip : 24;  // score pass (24 measured 9% faster than 16; 32 spills)
+        f.s1 = c->score_strip ? c->score_strip : 24;  // score pass: 24 rows at three waves per SIMD
         f.s2 = c->strip_rows == 32 ? 32 : 16;         // re-run
     }
     return f;
@@ -746,7 +746,7 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
 //            int32; the two-pass first for alphabets of <= 8 codes, the
 //            one-pass first for larger ones, so that a run of a single block
 //            gets the variant that measured faster on such data;
-//   stage 1  the winner's first strip: score strip 16 <-> 24, int32 strips
+//   stage 1  the winner's first strip: score strips 24, 32, 16, int32 strips
 //            8 and 32, rows per lane 8 <-> 4, the global kernel's 8 <-> 16;
 //   stage 2  the two-pass re-run strip 16 <-> 32.
 // Strips fixed by the caller (swg_set_schedule's strip_rows,
@@ -787,7 +787,9 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
     VariantCfg f = b;
     if (t.stage == 1) {
         if (c->nw) { f.s1 = b.s1 == 16 ? 8 : 16; out.push_back(f); }
-        else if (b.v == kVarI16) { f.s1 = b.s1 == 24 ? 16 : 24; out.push_back(f); }
+        else if (b.v == kVarI16)
+            for (int r : {24, 32, 16})
+                if (r != b.s1) { f.s1 = r; out.push_back(f); }
         else if (b.v == kVarLpp) { f.s1 = b.s1 == 4 ? 8 : 4; out.push_back(f); }
         else if (b.v == kVarI32)
             for (int r : {8, 16, 32})

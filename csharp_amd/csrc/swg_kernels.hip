@@ -593,6 +593,9 @@ __device__ __forceinline__ uint32_t u2(s16x2 v) { return __builtin_bit_cast(uint
 __device__ __forceinline__ uint32_t splat16(int x) { return ((uint32_t)x & 0xFFFFu) * 0x10001u; }
 
 constexpr int kPad16 = -16384;
+// __builtin_amdgcn_sched_barrier mask: VALU and SALU instructions may be
+// scheduled across the barrier, memory instructions may not
+constexpr int kSchedAluOnly = 0x0006;
 constexpr int kSat16 = 32766;  // best >= this: the pair may have saturated
 
 template <int R>
@@ -637,14 +640,25 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
     }
 }
 
-// Per row the running maximum and the first kFirstMaxBlock-column block in
-// which it was reached are tracked (the block maximum is folded in once per
-// block); the re-run finds the exact column inside that block.  best_cell
-// holds (bi << 16) | jb, jb = the block's first column.
-// Two waves per SIMD: the register budget lets the scheduler keep the
-// look-ahead profile loads in flight (measured faster than three waves).
+// First maximum: per row only the maximum of the current kFirstMaxBlock-column
+// block is kept; once per block the rows' block maxima are folded, in row
+// order, into the strip's first maximum (value, row, block), which the end of
+// the strip folds into the pair's.  Row-major order decides: a larger value,
+// or an equal value in an earlier row (a later block of the same row never
+// replaces an equal value).  The re-run finds the exact column inside the
+// block.  best_cell holds (bi << 16) | jb, jb = the block's first column.
+// Rows past La (the last strip's padding) never exceed the valid rows above
+// them (their values come only from vertical and horizontal gaps of those
+// rows and a diagonal add of kPad16), and they follow every valid row, so
+// they cannot take the first maximum.
+// The profile words of a column are loaded one column ahead into the
+// registers of the rows just computed (no second set); with three row states
+// per row the kernel runs score16_waves(R) waves per SIMD.
+constexpr int score16_waves(int R) { return R <= 16 ? 4 : R <= 24 ? 3 : 2; }
+
 template <int R, bool JOINT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(score16_waves(R), score16_waves(R)))) void
+score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 {
     static_assert(R % 8 == 0, "strip height must be a multiple of 8");
     constexpr int PS = R / 8 + 1;  // int4 per residue code (R int16 + 8 int16 of padding)
@@ -684,75 +698,75 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
         for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2((int)T0, (int)NEG2);
 
-        int bestA = INT_MIN, bestB = INT_MIN;
-        uint32_t cellA = 0, cellB = 0;
+        // the pair's first maximum: value (-1: none yet), row (1-based) and
+        // first column of its block, one pair per 16-bit half
+        uint32_t GV = M1, GR = 0u, GJ = 0u;
         for (int i0 = 0; i0 < La; i0 += R) {
             __syncthreads();
             if (JOINT) stage_joint16<R>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
             else stage_profile16<R>(lprof, S, s1, La, i0, lane);
             __syncthreads();
 
-            uint32_t tr[R], Hr[R], rb[R], rj[R], bm[R];
+            uint32_t tr[R], Hr[R], bm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; rb[r] = M1; rj[r] = 0u; bm[r] = NEG2; }
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; bm[r] = NEG2; }
+            const uint32_t row1 = splat16(i0 + 1);  // row of strip row 0
             uint32_t dtb = T0;
 
+            // residue words: current block, next block (needed by the
+            // look-ahead load of the last column) and, in the loop, the one
+            // after; loaded before the boundary words, in the loop's order,
+            // so that the loop waits for each load only as long as it must
             uint32_t nwa = wA[0], nwb = wB[0];
-            int2 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
-            int2 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
-            // a column's profile words (R/4 int4) are loaded one column ahead:
-            // joint entries (ca, cc) of the R rows, or the entries of ca and
-            // cc interleaved
-            // (JOINT: ca = the pair's byte offset in a row, cc unused)
-            auto load_col = [&](int ca, int cc, int4* out) {
+            uint32_t n2a = wA[min(1, lastA)], n2b = wB[min(1, lastB)];
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);
+            // boundary words of the next four columns, each reloaded four
+            // columns ahead once its column is done
+            int2 cb[4] = {bnd[1 * kWave + lane], bnd[2 * kWave + lane], bnd[3 * kWave + lane],
+                          bnd[4 * kWave + lane]};
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);
+            // profile words of the current column: group g = rows 4g .. 4g+3
+            // (JOINT: the joint entries at byte offset ca of each row; else
+            // int4 pairs (code ca, code cc) of 8 rows, shared by groups 2m and
+            // 2m+1); load(g, ...) replaces the group's words, once the group
+            // has been computed, with those of the next column
+            int4 cur[R / 4];
+            auto load = [&](int g, uint32_t ca, uint32_t cc) {
                 if (JOINT) {
                     const uint32_t* pj =
                         reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lprof) + ca);
-#pragma unroll
-                    for (int r4 = 0; r4 < R / 4; ++r4)
-                        out[r4] = make_int4((int)pj[(4 * r4) * kJointRow], (int)pj[(4 * r4 + 1) * kJointRow],
-                                            (int)pj[(4 * r4 + 2) * kJointRow], (int)pj[(4 * r4 + 3) * kJointRow]);
-                } else {
-                    const int4* pa = lprof + ca * PS;
-                    const int4* pc = lprof + cc * PS;
-#pragma unroll
-                    for (int r8 = 0; r8 < R / 8; ++r8) { out[2 * r8] = pa[r8]; out[2 * r8 + 1] = pc[r8]; }
+                    cur[g] = make_int4((int)pj[(4 * g) * kJointRow], (int)pj[(4 * g + 1) * kJointRow],
+                                       (int)pj[(4 * g + 2) * kJointRow], (int)pj[(4 * g + 3) * kJointRow]);
+                } else if (g & 1) {
+                    cur[g - 1] = lprof[ca * PS + (g >> 1)];
+                    cur[g] = lprof[cc * PS + (g >> 1)];
                 }
             };
             // JOINT: the byte offsets of the word's four columns
             auto joint_offsets = [](uint32_t wa, uint32_t wb) { return (wa << 5) | (wb << 2); };
-            int4 cur[R / 4];
             uint32_t nxo = JOINT ? joint_offsets(nwa, nwb) : 0u;
-            if (JOINT) load_col((int)(nxo & 0xFFu), 0, cur);
-            else load_col((int)(nwa & 0xFFu), (int)(nwb & 0xFFu), cur);
-            // residue words: current block, next block (needed by the
-            // look-ahead load of the last column) and the one after
-            uint32_t n2a = wA[min(1, lastA)], n2b = wB[min(1, lastB)];
+#pragma unroll
+            for (int g = 0; g < R / 4; ++g) {
+                if (JOINT) load(g, nxo & 0xFFu, 0u);
+                else load(g, nwa & 0xFFu, nwb & 0xFFu);
+            }
             for (int j = 1; j <= ncols; j += 4) {
                 const uint32_t cwa = nwa, cwb = nwb;
                 const uint32_t cxo = nxo;
-                int2 cb[4] = {nb0, nb1, nb2, nb3};
                 nwa = n2a;
                 nwb = n2b;
                 if (JOINT) nxo = joint_offsets(nwa, nwb);
                 n2a = wA[min((j + 7) >> 2, lastA)];
                 n2b = wB[min((j + 7) >> 2, lastB)];
-                const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
-                nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
-                // keep these loads at the top of the step: the whole step hides
-                // their latency (the boundary row comes from HBM)
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
                     const int sh = 8 * ((q + 1) & 3);
-                    int4 nx[R / 4];
-                    if (JOINT) {
-                        load_col((int)(((q < 3 ? cxo : nxo) >> sh) & 0xFFu), 0, nx);
-                    } else {
-                        load_col((int)(((q < 3 ? cwa : nwa) >> sh) & 0xFFu),
-                                 (int)(((q < 3 ? cwb : nwb) >> sh) & 0xFFu), nx);
-                    }
+                    // codes (JOINT: byte offset) of column jj + 1
+                    const uint32_t ca = JOINT ? (((q < 3 ? cxo : nxo) >> sh) & 0xFFu)
+                                              : (((q < 3 ? cwa : nwa) >> sh) & 0xFFu);
+                    const uint32_t cc = JOINT ? 0u : (((q < 3 ? cwb : nwb) >> sh) & 0xFFu);
                     uint32_t tu = (uint32_t)cb[q].x, Gu = (uint32_t)cb[q].y;
                     uint32_t dt = dtb;
                     dtb = tu;
@@ -788,7 +802,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                                     __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))),
                                 v2(tr[r]));
                             const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp4[k]));
-                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G), H);
+                            // G last: the row-to-row chain (t of the row above ->
+                            // G -> V -> t) is three dependent operations
+                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
                             const s16x2 tn = V - O2;
                             bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
                             dt = tr[r];
@@ -797,39 +813,53 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                             tu = u2(tn);
                             Gu = u2(G);
                         }
+                        load(r4, ca, cc);
+                        // the loads stay here (a column ahead of their use):
+                        // only ALU instructions may cross
+                        __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                     }
                     bnd[(int64_t)jj * kWave + lane] = make_int2((int)tu, (int)Gu);
-#pragma unroll
-                    for (int r4 = 0; r4 < R / 4; ++r4) cur[r4] = nx[r4];
+                    cb[q] = bnd[(int64_t)(jj + 4) * kWave + lane];
+                    __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                 }
                 // after the last 4-column step of a 16-column block (or of the
-                // row): block maximum bm > row maximum rb (no wrap: rb >= -1,
-                // bm >= 0): the sign of (rb - bm) per half, spread to a 16-bit
-                // mask, selects the block with one bitfield insert
+                // row): the rows' block maxima, first row first, then the
+                // pair's first maximum.  A mask per half is the sign of a
+                // difference (no wrap: values >= -1, block maxima >= 0, rows
+                // in [0, 32767]), spread to 16 bits; a bitfield insert
+                // selects with it.
                 static_assert(kFirstMaxBlock == 16, "a block is four 4-column steps");
                 if (((j - 1) & 15) == 12 || j + 4 > ncols) {
                     const uint32_t JB = splat16(((j - 1) & ~15) + 1);
+                    uint32_t BV = bm[0], BR = 0u;
+                    bm[0] = NEG2;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const uint32_t d = u2(v2(rb[r]) - v2(bm[r]));
-                        uint32_t mk;
+                    for (int r = 1; r < R; ++r) {
+                        const uint32_t d = u2(v2(BV) - v2(bm[r]));
+                        uint32_t mk;  // bm[r] > BV
                         asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
-                        rj[r] = (mk & JB) | (~mk & rj[r]);
-                        rb[r] = u2(__builtin_elementwise_max(v2(rb[r]), v2(bm[r])));
+                        BR = (mk & splat16(r)) | (~mk & BR);
+                        BV = u2(__builtin_elementwise_max(v2(BV), v2(bm[r])));
                         bm[r] = NEG2;
                     }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (i0 + r < La) {
-                    const int va = (short)(rb[r] & 0xFFFFu), vb = (short)(rb[r] >> 16);
-                    const uint32_t row = (uint32_t)(i0 + r + 1) << 16;
-                    if (va > bestA) { bestA = va; cellA = row | (rj[r] & 0xFFFFu); }
-                    if (vb > bestB) { bestB = vb; cellB = row | (rj[r] >> 16); }
+                    // (BV, row, JB) replaces (GV, GR, GJ) if BV is larger, or
+                    // equal in an earlier row -- only ever a row of this strip,
+                    // as earlier strips hold smaller rows (BV - GV can wrap
+                    // only where GV - BV is negative, which selects by itself)
+                    BR = u2(v2(BR) + v2(row1));
+                    uint32_t gt, lt, rl;
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(gt) : "s"(0x000F000Fu), "v"(u2(v2(GV) - v2(BV))));
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(lt) : "s"(0x000F000Fu), "v"(u2(v2(BV) - v2(GV))));
+                    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(rl) : "s"(0x000F000Fu), "v"(u2(v2(BR) - v2(GR))));
+                    const uint32_t m = gt | (~lt & rl);
+                    GR = (m & BR) | (~m & GR);
+                    GJ = (m & JB) | (~m & GJ);
+                    GV = u2(__builtin_elementwise_max(v2(GV), v2(BV)));
                 }
             }
         }
+        const int bestA = (short)(GV & 0xFFFFu), bestB = (short)(GV >> 16);
+        const uint32_t cellA = ((GR & 0xFFFFu) << 16) | (GJ & 0xFFFFu), cellB = (GR & 0xFFFF0000u) | (GJ >> 16);
         if (bA >= 0) {
             const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
             const int64_t pos = (int64_t)(a - D->r0) * D->ld + (bA - D->c0);

@@ -227,7 +227,7 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
  * key (residue alphabet, mean-length class, block-size class), on the first
  * blocks of the key (each run once, on its real work): the variant (int16
  * two-pass, int32 one-pass, 16 lanes per pair) at its default strips, then
- * the winner's first strip (score pass 16/24, one-pass 8/16/32, rows per lane
+ * the winner's first strip (score pass 24/32/16, one-pass 8/16/32, rows per lane
  * 4/8, global kernel 8/16), then the two-pass re-run strip (16/32); the
  * highest cells/ms wins.  Thin blocks (too few rows to fill the GPU one lane
  * per pair) always take 16 lanes per pair. */
