@@ -1664,7 +1664,8 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
             lob = s2w[wb < 0 ? 0 : wb];
             uint32_t wdaN = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
             uint32_t wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
-            // both pairs' scores of a column, packed, loaded one column ahead
+            // both pairs' scores of the first column, packed (the loop reloads
+            // them row by row, a column ahead)
             auto col_scores = [&](int ca, int cb, uint32_t* out) {
                 if (JOINT) {
                     const int cp = ca * ncode + cb;
@@ -1678,6 +1679,15 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
             };
             uint32_t cur[R];
             col_scores(wend[0] >= jlo ? (int)(wda >> 24) : pad, wend[1] >= jlo ? (int)(wdb >> 24) : pad, cur);
+            // boundary words of the next four columns, each reloaded four
+            // columns ahead once its column is done (columns past jtop are
+            // stale and read as -inf); the residue words one block further
+            // ahead than they are needed
+            int2 bw[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bw[q] = bnd[(int64_t)(jlo + q) * kWave + lane];
+            uint32_t loaP = s2w[wa - 1 < 0 ? 0 : wa - 1], lobP = s2w[wb - 1 < 0 ? 0 : wb - 1];
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);
             int lo_next = 0, hi_next = 0, jend = jlo - 1;
             for (int j = jlo; j <= ncols; j += 4) {
                 uint32_t gm = NEG2, lm = NEG2;  // group maxima: all rows, last row
@@ -1688,11 +1698,8 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                     int cb = (int)(((q < 3 ? wdb : wdbN) >> (8 * (3 - ((q + 1) & 3)))) & 0xFFu);
                     ca = jj + 1 > wend[0] ? pad : ca;
                     cb = jj + 1 > wend[1] ? pad : cb;
-                    uint32_t nx[R];
-                    col_scores(ca, cb, nx);
-                    __builtin_amdgcn_sched_barrier(0);
-                    const int2 bu = bnd[(int64_t)jj * kWave + lane];  // columns past jtop: stale, read as -inf
-                    uint32_t tu = jj <= jtop ? (uint32_t)bu.x : NEG2, Gu = jj <= jtop ? (uint32_t)bu.y : NEG2;
+                    const int cp = ca * ncode + cb;
+                    uint32_t tu = jj <= jtop ? (uint32_t)bw[q].x : NEG2, Gu = jj <= jtop ? (uint32_t)bw[q].y : NEG2;
                     uint32_t dt = dtb;
                     dtb = tu;
                     s16x2 cm = v2(NEG2);
@@ -1703,7 +1710,7 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                         const s16x2 H = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Hr[r]), E2),
                                                                   v2(tr[r]));
                         const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
-                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, G), H);
+                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
                         rm[r] = u2(__builtin_elementwise_max(v2(rm[r]), V));
                         cm = __builtin_elementwise_max(cm, V);
                         if (r == R - 1) lm = u2(__builtin_elementwise_max(v2(lm), V));
@@ -1713,8 +1720,16 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                         Hr[r] = u2(H);
                         tu = tn;
                         Gu = u2(G);
+                        // the next column's scores of this row, into the
+                        // register just read; the loads stay a column ahead
+                        // of their use (only ALU instructions may cross)
+                        if (JOINT) cur[r] = jt[rba[r] + cp];
+                        else cur[r] = (uint32_t)(uint16_t)tab16[rba[r] + ca] | ((uint32_t)(uint16_t)tab16[rbb[r] + cb] << 16);
+                        if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                     }
                     bnd[(int64_t)jj * kWave + lane] = make_int2((int)tu, (int)Gu);
+                    bw[q] = bnd[(int64_t)(jj + 4) * kWave + lane];
+                    __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                     // column hit: cm == S (cm <= S, so S - cm >= 0 is 0 only on a
                     // hit) inside the pair's columns; the largest column wins
                     const uint32_t JJ = splat16(jj);
@@ -1725,13 +1740,12 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                     asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(out) : "s"(0x000F000Fu), "v"(dw));
                     hitc = u2(__builtin_elementwise_max(v2(hitc), v2(JJ & ~miss & ~out)));
                     gm = u2(__builtin_elementwise_max(v2(gm), cm));
-#pragma unroll
-                    for (int r = 0; r < R; ++r) cur[r] = nx[r];
                 }
                 wda = wdaN; wdb = wdbN;
-                hia = loa; hib = lob; --wa; --wb;
-                loa = s2w[wa < 0 ? 0 : wa];
-                lob = s2w[wb < 0 ? 0 : wb];
+                hia = loa; hib = lob; loa = loaP; lob = lobP; --wa; --wb;
+                loaP = s2w[wa - 1 < 0 ? 0 : wa - 1];
+                lobP = s2w[wb - 1 < 0 ? 0 : wb - 1];
+                __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                 wdaN = __builtin_amdgcn_alignbyte(hia, loa, sh[0]);
                 wdbN = __builtin_amdgcn_alignbyte(hib, lob, sh[1]);
                 jend = j + 3;
