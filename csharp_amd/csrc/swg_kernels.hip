@@ -92,6 +92,10 @@ __device__ __forceinline__ void write_distance(const SeqSet& S, const BlockDesc&
 
 // Stage the strip's query profile: lprof[c2][r] = prof[s1[i0+r]][c2] for
 // every residue code c2 (rows past the end of s1 use the pad code).
+// __builtin_amdgcn_sched_barrier mask: VALU and SALU instructions may be
+// scheduled across the barrier, memory instructions may not
+constexpr int kSchedAluOnly = 0x0006;
+
 template <int R>
 __device__ __forceinline__ void stage_profile(int4* lprof, const SeqSet& S, const uint8_t* s1,
                                               int La, int i0, int lane)
@@ -156,32 +160,49 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
             }
             int dtb = -o, dTb = 0;  // V[i0][0]-o and T[i0][0] (border: 0, STOP)
 
-            uint32_t nw = s2w[0];
-            int4 nb0 = bnd[1 * kWave + lane], nb1 = bnd[2 * kWave + lane];
-            int4 nb2 = bnd[3 * kWave + lane], nb3 = bnd[4 * kWave + lane];
+            // residue words: columns j .. j+3 and the next block, loaded a
+            // block ahead (word (Lb + 3) / 4 holds only pad codes, and the
+            // words before it pad codes past Lb)
+            const int lastW = (Lb + 3) >> 2;
+            uint32_t wcur = s2w[0], wnx = s2w[min(1, lastW)];
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);  // in the loop's order
+            // boundary words of the next two columns, each reloaded two
+            // columns ahead once its column is done
+            int4 bw[2] = {bnd[1 * kWave + lane], bnd[2 * kWave + lane]};
+            // profile words (score, increment) of the current column, two rows
+            // per int4, each reloaded with the next column's once its rows
+            // are done
+            int4 pp[R / 2];
+            {
+                const int c2 = 1 > Lb ? pad : (int)(wcur & 0xFFu);
+#pragma unroll
+                for (int r2 = 0; r2 < R / 2; ++r2) pp[r2] = lprof[c2 * (R / 2 + 1) + r2];
+            }
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);
             for (int j = 1; j <= ncols; j += 4) {
-                const uint32_t w = nw;
-                int4 cb[4] = {nb0, nb1, nb2, nb3};
-                nw = s2w[(j + 3) >> 2];
-                const int64_t nxt = (int64_t)(j + 4) * kWave + lane;
-                nb0 = bnd[nxt]; nb1 = bnd[nxt + kWave]; nb2 = bnd[nxt + 2 * kWave]; nb3 = bnd[nxt + 3 * kWave];
+                const uint32_t wc = wcur;
+                wcur = wnx;
+                wnx = s2w[min(((j + 3) >> 2) + 1, lastW)];
+                __builtin_amdgcn_sched_barrier(kSchedAluOnly);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int jj = j + q;
-                    int c2 = (int)((w >> (8 * q)) & 0xFFu);
-                    c2 = jj > Lb ? pad : c2;
-                    int tu = cb[q].x, Gu = cb[q].y, Tu = cb[q].z, TGu = cb[q].w;
+                    // code of column jj + 1 (its profile words are loaded
+                    // during this column)
+                    int c2n = (int)(((q < 3 ? wc : wcur) >> (8 * ((q + 1) & 3))) & 0xFFu);
+                    c2n = jj + 1 > Lb ? pad : c2n;
+                    const int4* prn = lprof + c2n * (R / 2 + 1);
+                    int tu = bw[q & 1].x, Gu = bw[q & 1].y, Tu = bw[q & 1].z, TGu = bw[q & 1].w;
                     int dt = dtb, dT = dTb;
                     dtb = tu; dTb = Tu;
-                    const int4* pr = lprof + c2 * (R / 2 + 1);
 #pragma unroll
                     for (int r2 = 0; r2 < R / 2; ++r2) {
-                        const int4 pp = pr[r2];
+                        const int4 pq = pp[r2];
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             const int r = 2 * r2 + h;
-                            const int sc = h ? pp.z : pp.x;
-                            const int inc = h ? pp.w : pp.y;
+                            const int sc = h ? pq.z : pq.x;
+                            const int inc = h ? pq.w : pq.y;
                             // vertical gap (SmithWatermanGotoh.cs:159-172)
                             const int g1 = Gu - e;
                             const bool cg = g1 > tu;
@@ -211,8 +232,14 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
                             tr[r] = tn; Tr[r] = Tn;
                             tu = tn; Gu = G; Tu = Tn; TGu = TG;
                         }
+                        pp[r2] = prn[r2];
+                        // the loads stay a column ahead of their use (only ALU
+                        // instructions may cross)
+                        if ((r2 & 1) == 1) __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                     }
                     bnd[(int64_t)jj * kWave + lane] = make_int4(tu, Gu, Tu, TGu);
+                    bw[q & 1] = bnd[(int64_t)(jj + 2) * kWave + lane];
+                    __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                 }
             }
             // rows in order: first strict maximum in row-major order
@@ -483,65 +510,103 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
             if (i0 == 0) { dtb = -o; dTb1 = 0; dTb2 = 0; }
             else { dtb = nw_border(o, e, i0) - o; dTb1 = i0 * ginc; dTb2 = (i0 << 16) | (int)kUpBit | i0; }
 
-            for (int j = 1; j <= ncols; ++j) {
-                const uint32_t w = s2w[(j - 1) >> 2];
-                int c2 = (int)((w >> (8 * ((j - 1) & 3))) & 0xFFu);
-                c2 = j > Lb ? pad : c2;
-                const int4 bu = bnd[(int64_t)j * kWave + lane];
-                const int2 bu2 = REGION ? bnd2[(int64_t)j * kWave + lane] : make_int2(0, 0);
-                int tu = bu.x, Gu = bu.y, Tu1 = bu.z, TGu1 = bu.w, Tu2 = bu2.x, TGu2 = bu2.y;
-                int dt = dtb, dT1 = dTb1, dT2 = dTb2;
-                dtb = tu; dTb1 = Tu1; dTb2 = Tu2;
-                const bool hitcol = j == Lb;
-                const int4* pr = lprof + c2 * (R / 2 + 1);
+            // residue words: columns j .. j+3 and the next block, loaded a
+            // block ahead (word (Lb + 3) / 4 holds only pad codes, and the
+            // words before it pad codes past Lb)
+            const int lastW = (Lb + 3) >> 2;
+            uint32_t wcur = s2w[0], wnx = s2w[min(1, lastW)];
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);  // in the loop's order
+            // boundary words of the next two columns, each reloaded two
+            // columns ahead once its column is done
+            int4 bw[2] = {bnd[(int64_t)1 * kWave + lane], bnd[(int64_t)2 * kWave + lane]};
+            int2 bw2[2] = {make_int2(0, 0), make_int2(0, 0)};
+            if (REGION) { bw2[0] = bnd2[(int64_t)1 * kWave + lane]; bw2[1] = bnd2[(int64_t)2 * kWave + lane]; }
+            // profile words (score, increment) of the current column, two rows
+            // per int4, each reloaded with the next column's once its rows
+            // are done
+            int4 pp[R / 2];
+            {
+                const int c2 = 1 > Lb ? pad : (int)(wcur & 0xFFu);
 #pragma unroll
-                for (int r2 = 0; r2 < R / 2; ++r2) {
-                    const int4 pp = pr[r2];
+                for (int r2 = 0; r2 < R / 2; ++r2) pp[r2] = lprof[c2 * (R / 2 + 1) + r2];
+            }
+            __builtin_amdgcn_sched_barrier(kSchedAluOnly);
+
+            for (int j = 1; j <= ncols; j += 4) {
+                const uint32_t wc = wcur;
+                wcur = wnx;
+                wnx = s2w[min(((j + 3) >> 2) + 1, lastW)];
+                __builtin_amdgcn_sched_barrier(kSchedAluOnly);
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int r = 2 * r2 + h;
-                        const int sc = h ? pp.z : pp.x;
-                        const int inc = h ? pp.w : pp.y;
-                        // vertical gap: extend TG or open from T (keeping its run if vertical)
-                        const int g1 = Gu - e;
-                        const bool cg = g1 > tu;
-                        const int G = cg ? g1 : tu;
-                        const int TG1 = (cg ? TGu1 : Tu1) + ginc;
-                        int TG2 = 0;
-                        if (REGION) {
-                            const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
-                            TG2 = (cg ? TGu2 : baseG) + 1;
+                for (int q = 0; q < 4; ++q) {
+                    const int jj = j + q;
+                    // code of column jj + 1 (its profile words are loaded
+                    // during this column)
+                    int c2n = (int)(((q < 3 ? wc : wcur) >> (8 * ((q + 1) & 3))) & 0xFFu);
+                    c2n = jj + 1 > Lb ? pad : c2n;
+                    const int4* prn = lprof + c2n * (R / 2 + 1);
+                    const int4 bu = bw[q & 1];
+                    const int2 bu2 = bw2[q & 1];
+                    int tu = bu.x, Gu = bu.y, Tu1 = bu.z, TGu1 = bu.w, Tu2 = bu2.x, TGu2 = bu2.y;
+                    int dt = dtb, dT1 = dTb1, dT2 = dTb2;
+                    dtb = tu; dTb1 = Tu1; dTb2 = Tu2;
+                    const bool hitcol = jj == Lb;
+#pragma unroll
+                    for (int r2 = 0; r2 < R / 2; ++r2) {
+                        const int4 pq = pp[r2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int r = 2 * r2 + h;
+                            const int sc = h ? pq.z : pq.x;
+                            const int inc = h ? pq.w : pq.y;
+                            // vertical gap: extend TG or open from T (keeping its run if vertical)
+                            const int g1 = Gu - e;
+                            const bool cg = g1 > tu;
+                            const int G = cg ? g1 : tu;
+                            const int TG1 = (cg ? TGu1 : Tu1) + ginc;
+                            int TG2 = 0;
+                            if (REGION) {
+                                const int baseG = (Tu2 & (int)kUpBit) ? (Tu2 ^ (int)kUpBit) : (Tu2 & (int)0xFFFF0000u);
+                                TG2 = (cg ? TGu2 : baseG) + 1;
+                            }
+                            // horizontal gap
+                            const int h1 = Hr[r] - e;
+                            const bool ch = h1 > tr[r];
+                            Hr[r] = ch ? h1 : tr[r];
+                            TH1[r] = (ch ? TH1[r] : Tr1[r]) + ginc;
+                            if (REGION) {
+                                const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
+                                TH2[r] = (ch ? TH2[r] : baseH) + 1;
+                            }
+                            const int f = dt + sc;
+                            const int V = max(max(f, G), Hr[r]);
+                            int x1 = (V == G) ? TG1 : TH1[r];
+                            int x2 = 0;
+                            if (REGION) x2 = (V == G) ? (TG2 | (int)kUpBit) : TH2[r];
+                            const bool dg = V == f;
+                            x1 = dg ? dT1 + inc : x1;
+                            if (REGION) x2 = dg ? (dT2 & (int)0xFFFF0000u) : x2;
+                            const int tn = V - o;
+                            if (r == rcap) {
+                                capV = hitcol ? V : capV;
+                                cap1 = hitcol ? (uint32_t)x1 : cap1;
+                                cap2 = hitcol ? (uint32_t)x2 : cap2;
+                            }
+                            dt = tr[r]; dT1 = Tr1[r]; dT2 = Tr2[r];
+                            tr[r] = tn; Tr1[r] = x1; Tr2[r] = x2;
+                            tu = tn; Gu = G; Tu1 = x1; TGu1 = TG1; Tu2 = x2; TGu2 = TG2;
                         }
-                        // horizontal gap
-                        const int h1 = Hr[r] - e;
-                        const bool ch = h1 > tr[r];
-                        Hr[r] = ch ? h1 : tr[r];
-                        TH1[r] = (ch ? TH1[r] : Tr1[r]) + ginc;
-                        if (REGION) {
-                            const int baseH = (Tr2[r] & (int)kUpBit) ? (Tr2[r] & (int)0xFFFF0000u) : Tr2[r];
-                            TH2[r] = (ch ? TH2[r] : baseH) + 1;
-                        }
-                        const int f = dt + sc;
-                        const int V = max(max(f, G), Hr[r]);
-                        int x1 = (V == G) ? TG1 : TH1[r];
-                        int x2 = 0;
-                        if (REGION) x2 = (V == G) ? (TG2 | (int)kUpBit) : TH2[r];
-                        const bool dg = V == f;
-                        x1 = dg ? dT1 + inc : x1;
-                        if (REGION) x2 = dg ? (dT2 & (int)0xFFFF0000u) : x2;
-                        const int tn = V - o;
-                        if (r == rcap) {
-                            capV = hitcol ? V : capV;
-                            cap1 = hitcol ? (uint32_t)x1 : cap1;
-                            cap2 = hitcol ? (uint32_t)x2 : cap2;
-                        }
-                        dt = tr[r]; dT1 = Tr1[r]; dT2 = Tr2[r];
-                        tr[r] = tn; Tr1[r] = x1; Tr2[r] = x2;
-                        tu = tn; Gu = G; Tu1 = x1; TGu1 = TG1; Tu2 = x2; TGu2 = TG2;
+                        pp[r2] = prn[r2];
+                        // the loads stay a column ahead of their use (only ALU
+                        // instructions may cross)
+                        if ((r2 & 1) == 1) __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                     }
+                    bnd[(int64_t)jj * kWave + lane] = make_int4(tu, Gu, Tu1, TGu1);
+                    if (REGION) bnd2[(int64_t)jj * kWave + lane] = make_int2(Tu2, TGu2);
+                    bw[q & 1] = bnd[(int64_t)(jj + 2) * kWave + lane];
+                    if (REGION) bw2[q & 1] = bnd2[(int64_t)(jj + 2) * kWave + lane];
+                    __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                 }
-                bnd[(int64_t)j * kWave + lane] = make_int4(tu, Gu, Tu1, TGu1);
-                if (REGION) bnd2[(int64_t)j * kWave + lane] = make_int2(Tu2, TGu2);
             }
         }
 
@@ -593,9 +658,6 @@ __device__ __forceinline__ uint32_t u2(s16x2 v) { return __builtin_bit_cast(uint
 __device__ __forceinline__ uint32_t splat16(int x) { return ((uint32_t)x & 0xFFFFu) * 0x10001u; }
 
 constexpr int kPad16 = -16384;
-// __builtin_amdgcn_sched_barrier mask: VALU and SALU instructions may be
-// scheduled across the barrier, memory instructions may not
-constexpr int kSchedAluOnly = 0x0006;
 constexpr int kSat16 = 32766;  // best >= this: the pair may have saturated
 
 template <int R>
