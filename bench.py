@@ -38,13 +38,15 @@ ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture"
 # unrolled), counted by hand from this build's gfx950 assembly of
 # swg_kernels.hip (hipcc -S: the loop body's VALU instructions over the cells
 # it evaluates); a packed instruction counts once for the two cells it
-# evaluates.  Reported for information.
+# evaluates.  The global kernel's figure is for its variant without the
+# aligned-region word (protein PercentSimilarity; with it, 35.9).  Reported
+# for information.
 # The reverse pass evaluates a data-dependent band (the DP cells that can lie
 # on an optimal path back from the maximum), so its work is the cells it
 # evaluates, not the prefix rectangles it decides.
 EVALUATED_IS_WORK = {"reverse"}
-VALU_PER_CELL = {"score16": 4.745, "reverse": 7.469, "rerun_capture": 21.797, "rerun_track": 23.906, "one_pass": 23.906,
-                 "global": 38.875}
+VALU_PER_CELL = {"score16": 4.757, "reverse": 5.672, "rerun_capture": 21.797, "rerun_track": 25.25, "one_pass": 23.375,
+                 "global": 20.906}
 
 
 def log(msg):
