@@ -31,7 +31,7 @@ PEAK_OPS = {"int32": 256 * 4 * 16 * 2.4e9, "int16x2": 2 * 256 * 4 * 16 * 2.4e9}
 # update (the score pass; the reverse pass tracks where the maximum is
 # reached instead); the global variant carries one more word (lead
 # and trailing gap run), 30.  Mode = the arithmetic the kernel runs in.
-ALG_OPS = {"score16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
+ALG_OPS = {"score16": (9, "int16x2"), "lps16": (9, "int16x2"), "reverse": (9, "int16x2"), "rerun_capture": (24, "int32"), "rerun_track": (24, "int32"),
            "rect64": (24, "int32"),
            "one_pass": (24, "int32"), "lpp16": (24, "int32"), "global": (30, "int32")}
 # VALU instructions per DP cell in each kernel's inner loop (4 columns x R rows
@@ -54,7 +54,8 @@ def log(msg):
 
 
 # kernel variants (SWG_SCHED_* values of swg_block_config.variant)
-VNAME = {1: "plain", 2: "int32 one-pass", 3: "int16 score pass + sorted int32 re-run", 4: "16 lanes per pair"}
+VNAME = {1: "plain", 2: "int32 one-pass", 3: "int16 score pass + sorted int32 re-run", 4: "16 lanes per pair",
+         5: "int16 score pass with 16 lanes per pair + sorted int32 re-run"}
 TUNING = {0: "fixed", 1: "autotuner trial", 2: "autotuner choice"}
 
 
@@ -90,7 +91,7 @@ def main():
     ap.add_argument("--reverse-strip", type=int, default=0, help="reverse-pass strip rows (0 = the default)")
     ap.add_argument("--full-pairs-per-block", type=int, default=200,
                     help="c3full: seeded pairs per block checked against the oracle")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "plain"],
+    ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "lps16", "plain"],
                     help="force one kernel variant for the timed steps (auto = autotuned)")
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2nw", "c3", "c3full", "c4"],
                     help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity "
@@ -100,6 +101,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-rates", default="10,30,100,300,1000")
+    ap.add_argument("--serve-schedule", default="auto", choices=["auto", "i16", "lpp16", "lps16"],
+                    help="serving batch kernels: auto (the library's policy: the int16 16-lanes-per-pair score "
+                         "pass where the alphabet allows, else 16 lanes per pair for batches too small to fill "
+                         "the GPU), "
+                         "i16 (the two-pass schedule for every batch), lpp16 (the int32 16-lanes-per-pair kernel "
+                         "always) or lps16 (the int16 16-lanes-per-pair score pass always)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -159,7 +166,7 @@ def main():
                     schedule=ca.SCHED_PERSISTENT)
     al.set_sequences(seqs)
     SCHED = {"auto": ca.SCHED_PERSISTENT, "i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
-             "lpp16": ca.SCHED_PERSISTENT_LPP16, "plain": ca.SCHED_PLAIN}
+             "lpp16": ca.SCHED_PERSISTENT_LPP16, "lps16": ca.SCHED_PERSISTENT_LPS16, "plain": ca.SCHED_PLAIN}
     al.set_schedule(SCHED[args.schedule], args.strip)
     al.set_score_strip(args.score_strip)
     al.set_reverse_strip(args.reverse_strip)
@@ -539,8 +546,8 @@ def full_c3_run(args):
     }
 
 
-# queries per serving batch: batches of a few queries run 16 lanes per pair
-# (latency), deep queues fill batches large enough for the two-pass schedule
+# queries per serving batch: a deep queue fills batches of up to this many
+# queries (one launch per batch)
 SERVE_MAX_BATCH = 32
 
 
@@ -561,13 +568,17 @@ def serve_bench(args):
     qset = WL.sequences("c5_queries")
     M, o, e = WL.scoring("c5_panel")
     qmax = int(qset.lengths.max())
-    srv = ca.Server(panel, M, o, e, max_query_len=qmax, max_batch=SERVE_MAX_BATCH, lanes=3, hbm_budget=4 << 30, stats=False)
+    sched = {"auto": ca.SCHED_PERSISTENT, "i16": ca.SCHED_PERSISTENT_I16, "lpp16": ca.SCHED_PERSISTENT_LPP16,
+             "lps16": ca.SCHED_PERSISTENT_LPS16}
+    srv = ca.Server(panel, M, o, e, max_query_len=qmax, max_batch=SERVE_MAX_BATCH, lanes=3, hbm_budget=4 << 30,
+                    stats=False, schedule=sched[args.serve_schedule])
     # warm-up: scratch allocation and graph capture on every lane
     warm = [srv.submit(qset.seq(i)) for i in range(6)]
     for t in warm:
         srv.wait(t)
         srv.release(t)
-    out = {"panel": len(panel), "max_batch": SERVE_MAX_BATCH, "lanes": 3, "results": []}
+    out = {"panel": len(panel), "max_batch": SERVE_MAX_BATCH, "lanes": 3, "schedule": args.serve_schedule,
+           "results": []}
     base = 6
 
     def take(k):

@@ -34,11 +34,12 @@ MODE_SW_LOCAL, MODE_NW_GLOBAL = 0, 1
 TDIST_INT16, TDIST_DOUBLE = 0, 1
 DIAG_LOWER_MIRROR, DIAG_FULL = 0, 1
 PROGRAM_SWMPI, PROGRAM_SWMS = 0, 1
-SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16 = 0, 1, 2, 3, 4
+(SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16,
+ SCHED_PERSISTENT_LPS16) = 0, 1, 2, 3, 4, 5
 (OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING, OPT_SCORE_STRIP_ROWS, OPT_TUNE_MIN_CELLS, OPT_REVERSE_STRIP_ROWS,
  OPT_REVERSE_ROW_CAP) = 1, 2, 3, 4, 5, 6
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
-                7: "global", 8: "reverse", 9: "rect64", 10: "distance"}
+                7: "global", 8: "reverse", 9: "rect64", 10: "distance", 11: "lps16"}
 MATRIX_SIZE = 127
 
 ABI_SYMBOLS = (
@@ -494,11 +495,11 @@ class Server:
 
     def __init__(self, panel, scores, gap_open, gap_ext, alignment_type=ALIGN_NUCLEIC,
                  distance_fn=DIST_PERCENT_IDENTITY, tdistance=TDIST_INT16, max_query_len=2000, max_batch=16,
-                 lanes=2, hbm_budget=1 << 30, stats=True, device=0):
+                 lanes=2, hbm_budget=1 << 30, stats=True, device=0, schedule=SCHED_PERSISTENT):
         self._scores = np.ascontiguousarray(scores, dtype=np.float32).reshape(MATRIX_SIZE, MATRIX_SIZE)
         p = swg_params()
         p.mode, p.alignment_type, p.distance_fn = MODE_SW_LOCAL, alignment_type, distance_fn
-        p.tdistance, p.diag_mode, p.schedule = tdistance, DIAG_FULL, SCHED_PERSISTENT
+        p.tdistance, p.diag_mode, p.schedule = tdistance, DIAG_FULL, schedule
         p.gap_open, p.gap_ext = float(gap_open), float(gap_ext)
         p.scores = self._scores.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         p.device = device

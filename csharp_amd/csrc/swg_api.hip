@@ -90,10 +90,14 @@ enum Variant : int {
     kVarI32 = 1,       // sw_kernel<R, true>: one pass, int32, statistics carried
     kVarI16 = 2,       // score16_kernel (int16x2) + sort by first-maximum cell + rerun_kernel
     kVarLpp = 3,       // lpp16_kernel: 16 lanes per pair, 4 pairs per wave
-    kNumVariants = 4,
+    kVarLps = 4,       // lps16_kernel (int16x2, 16 lanes per pair, 8 pairs per wave) + the two-pass re-run
+    kNumVariants = 5,
 };
 
 constexpr int kScoreTile = 2 * kWave;  // pairs per score-pass task
+
+// The score pass, the sorts, the reverse pass and the re-runs.
+bool two_pass(int v) { return v == kVarI16 || v == kVarLps; }
 
 // Work counters of a block (int32 words of BlockBufs::queue, zeroed by a fill
 // kernel at the start of every block): one per persistent launch.
@@ -217,6 +221,7 @@ struct swg_ctx {
     int32_t dist_kind = kDistFraction;  // DistKind
     bool bio_helpers = false;       // SWMS / NW ComputeDistance helpers (SWG_PROGRAM_SWMS or global mode)
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
+    bool lps_ok = false;            // and the 16-lanes-per-pair score pass (<= 8 codes, score + open in [0, 255])
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
 
     // sequences
@@ -292,6 +297,10 @@ VariantCfg default_cfg(const swg_ctx* c, int v)
     if (c->nw) f.s1 = c->strip_rows == 8 ? 8 : 16;
     else if (v == kVarPlain) f.s1 = 16;
     else if (v == kVarLpp) f.s1 = c->strip_rows == 8 ? 8 : 4;
+    else if (v == kVarLps) {
+        f.s1 = 8;                                     // rows per lane of the score pass
+        f.s2 = c->strip_rows == 32 ? 32 : 16;         // re-run
+    }
     else if (v == kVarI32) f.s1 = c->strip_rows ? c->strip_rows : 16;
     else {
         f.s1 = c->score_strip ? c->score_strip : 24;  // score pass: 24 rows at three waves per SIMD
@@ -300,7 +309,10 @@ VariantCfg default_cfg(const swg_ctx* c, int v)
     return f;
 }
 
-int pairs_per_wave(int v) { return v == kVarI16 ? kScoreTile : v == kVarLpp ? kLppTile : kWave; }
+int pairs_per_wave(int v)
+{
+    return v == kVarI16 ? kScoreTile : v == kVarLpp ? kLppTile : v == kVarLps ? kLpsTile : kWave;
+}
 
 // Resident workgroups of the configuration's kernels on the lane's share of
 // the CUs: n1 first kernel, n2 re-run, n3 reverse pass and n4 big-rectangle
@@ -312,11 +324,12 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
     int a = c->nw        ? nw_blocks_per_cu(s1, nw_region)
           : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
+          : v == kVarLps ? lps16_blocks_per_cu(s1)
                          : sw_blocks_per_cu(s1, false);
-    int b = v == kVarI16 ? rerun_blocks_per_cu(s2, c->ncode) : 0;
-    int r = v == kVarI16 ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
-    int q = v == kVarI16 ? rect64_blocks_per_cu(c->ncode) : 0;
-    if (a <= 0 || (v == kVarI16 && (b <= 0 || r <= 0 || q <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
+    int b = two_pass(v) ? rerun_blocks_per_cu(s2, c->ncode) : 0;
+    int r = two_pass(v) ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
+    int q = two_pass(v) ? rect64_blocks_per_cu(c->ncode) : 0;
+    if (a <= 0 || (two_pass(v) && (b <= 0 || r <= 0 || q <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
     *n2 = std::max(1, b * c->num_cu / ln.share);
     if (n3) *n3 = std::max(1, r * c->num_cu / ln.share);
@@ -475,7 +488,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
     int n1, n2, n3, n4;
     int rc = slots_for(c, ln, f, &n1, &n2, &n3, &n4, nw_region);
     if (rc) return rc;
-    int want = std::max(std::max(n1, n2), v == kVarI16 ? std::max(n3, n4) : 0);
+    int want = std::max(std::max(n1, n2), two_pass(v) ? std::max(n3, n4) : 0);
     rc = ensure_scratch(c, ln, &want);
     if (rc) return rc;
     n1 = std::min(n1, want);
@@ -485,8 +498,9 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
 
     const hipStream_t st = ln.stream;
     BlockDesc desc[2] = {};
-    if (v == kVarI16) {
-        plan_tiles(c, g, true, kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col, B.h_tile_maxlen, plan);
+    if (two_pass(v)) {
+        plan_tiles(c, g, true, v == kVarLps ? kLpsTile : kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col,
+                   B.h_tile_maxlen, plan);
         if ((rc = upload(st, B.s_task_row, B.h_task_row)) || (rc = upload(st, B.s_task_tile, B.h_task_tile)) ||
             (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)))
             return rc;
@@ -553,13 +567,14 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
     // descriptor: a fixed grid, so that a captured graph fits every block size
     const int pair_grid = std::max(1, c->num_cu * 4 / ln.share);
     const int tdist = c->p.tdistance == SWG_TDIST_DOUBLE ? 1 : 0;
-    if (v == kVarI16) {
+    if (two_pass(v)) {
         ST.task_row = B.s_task_row.p; ST.task_tile = B.s_task_tile.p; ST.tile_col = B.s_tile_col.p;
         ST.tile_maxlen = B.s_tile_maxlen.p; ST.desc = B.desc.p + 1; ST.queue = B.queue.p; ST.best_cell = B.best_cell.p;
         ST.best_score = B.best_score.p;
         // sort 1: by first-maximum cell, for the reverse pass
         P.best_cell = B.best_cell.p; P.start_cell = nullptr; P.desc = B.desc.p; P.len = c->len.p;
         P.best_score = B.best_score.p;
+        P.big_rect = v == kVarLps ? kBigRectThin : kBigRect;
         P.meta = B.sortbuf.p; P.count = B.sortbuf.p + kMetaWords;
         P.cursor = P.count + kNumBuckets; P.base = P.count + 2 * kNumBuckets; P.sorted = B.sorted.p;
         RV.sorted = B.sorted.p; RV.best_cell = B.best_cell.p; RV.best_score = B.best_score.p;
@@ -623,8 +638,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         } else {
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
-            if (e == hipSuccess) e = launch_score16(S, ST, W, s1, n1, st);
-            if (e == hipSuccess) e = after(SWG_KERNEL_SCORE16);
+            if (e == hipSuccess) e = v == kVarLps ? launch_lps16(S, ST, W, s1, n1, st) : launch_score16(S, ST, W, s1, n1, st);
+            if (e == hipSuccess) e = after(v == kVarLps ? SWG_KERNEL_LPS16 : SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
             if (e == hipSuccess) e = launch_reverse(S, RV, W, c->reverse_strip, n3, st);
@@ -692,8 +707,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
     if (timing) {
         c->kt_n = 0;
         c->kt_plan = plan;
-        c->kt_meta = v == kVarI16 ? P.meta : nullptr;
-        c->kt_meta2 = v == kVarI16 ? P2.meta : nullptr;
+        c->kt_meta = two_pass(v) ? P.meta : nullptr;
+        c->kt_meta2 = two_pass(v) ? P2.meta : nullptr;
     }
     SWG_HIP(hipEventRecord(ln.ev0, st));
     if (plan.ntasks == 0 && g.diag_lower) SWG_HIP(launch_zero_diagonal(tdist, B.desc.p, pair_grid, st));
@@ -713,7 +728,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
     SWG_HIP(hipEventRecord(ln.ev1, st));
     SWG_HIP(hipEventRecord(B.done, st));
     B.pending = true;
-    c->last_strip = v == kVarI16 ? s2 : s1;
+    c->last_strip = two_pass(v) ? s2 : s1;
     c->last_variant = v;
     c->last_cfg = f;
     return SWG_OK;
@@ -838,12 +853,17 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* c
     *entry = nullptr;
     *trial = false;
     if (c->schedule == SWG_SCHED_PLAIN) { *cfg = default_cfg(c, kVarPlain); return SWG_OK; }
-    if (c->nw && (c->schedule == SWG_SCHED_PERSISTENT_I16 || c->schedule == SWG_SCHED_PERSISTENT_LPP16))
+    if (c->nw && (c->schedule == SWG_SCHED_PERSISTENT_I16 || c->schedule == SWG_SCHED_PERSISTENT_LPP16 ||
+                  c->schedule == SWG_SCHED_PERSISTENT_LPS16))
         return fail(SWG_E_UNSUPPORTED, "the global alignment has only the int32 one-pass kernel");
     if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) { *cfg = default_cfg(c, kVarLpp); return SWG_OK; }
+    if (c->schedule == SWG_SCHED_PERSISTENT_LPS16) {
+        *cfg = default_cfg(c, c->lps_ok ? kVarLps : kVarLpp);
+        return SWG_OK;
+    }
     if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *cfg = default_cfg(c, kVarI32); return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *cfg = default_cfg(c, kVarI16); return SWG_OK; }
-    if (!c->nw && thin_block(c, ln, g)) { *cfg = default_cfg(c, kVarLpp); return SWG_OK; }
+    if (!c->nw && thin_block(c, ln, g)) { *cfg = default_cfg(c, c->lps_ok ? kVarLps : kVarLpp); return SWG_OK; }
     const int64_t key = tune_key(c, g);
     TuneEntry* t = nullptr;
     for (auto& e : c->tuned)
@@ -908,7 +928,9 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
 // Residue coding + device upload of a sequence set, optionally followed by
 // `nslots` empty slots of capacity `slot_cap` residues (serving queries are
 // written into them later).  With `matrix_symbols`, every symbol that has a
-// non-zero score also gets a code so that later queries can use it.
+// non-zero score also gets a code so that later queries can use it; the
+// set's own symbols are then coded first (the slots are only ever row
+// sequences, so the column alphabet stays the set's).
 int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, int32_t n, int32_t nslots,
                    int32_t slot_cap, bool matrix_symbols)
 {
@@ -952,14 +974,20 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
                     return fail(SWG_E_UNSUPPORTED, "Kimura2/JukesCantor with the gap character '-' in a sequence");
         seen[(int)'-'] = false;
     }
+    bool in_set[256] = {false};
+    for (int32_t i = 0; i < n; ++i)
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) in_set[residues[k]] = true;
     int32_t ncode = 0;
-    for (int s = 0; s < 256; ++s) {
-        c->code_of[s] = -1;
-        if (seen[s]) {
+    for (int s = 0; s < 256; ++s) c->code_of[s] = -1;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int s = 0; s < 256; ++s) {
+            if (!seen[s] || c->code_of[s] >= 0 || (pass == 0 && !in_set[s])) continue;
             if (s >= SWG_MATRIX_SIZE) return fail(SWG_E_SYMBOL, "residue symbol >= 127 (the reference indexes a 127x127 matrix)");
             c->code_of[s] = (int16_t)ncode++;
         }
-    }
+    int32_t set_codes = 0;  // codes of the set's own symbols: 0 .. set_codes - 1
+    for (int s = 0; s < 256; ++s)
+        if (in_set[s] && c->code_of[s] >= 0) ++set_codes;
     if (ncode + 1 > kMaxCodes) return fail(SWG_E_CAPACITY, "more than 63 distinct residue symbols");
     const int32_t pad = ncode;
     c->ncode = ncode + 1;
@@ -994,6 +1022,16 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
                 }
                 prof[((size_t)tb * c->ncode + x) * c->ncode + y] = v;
             }
+
+    // the 16-lanes-per-pair score pass keeps, per row, score + gap_open of
+    // column codes 0..7 as bytes and scores the pad code -gap_open (exact
+    // only for gap_open >= 1: pad cells then stay below every real
+    // maximum); its column sequences are the set's, whose codes must be
+    // < 8, and the pad code must be < 8 or >= 12 (it selects the byte 0)
+    c->lps_ok = c->int16_ok && !c->nw && c->gap_open >= 1 && set_codes <= 7 && (pad < 8 || pad >= 12);
+    for (int x = 0; x < pad && c->lps_ok; ++x)
+        for (int y = 0; y < set_codes; ++y)
+            if (prof[(size_t)x * c->ncode + y].x < 0 || prof[(size_t)x * c->ncode + y].x > 255) c->lps_ok = false;
 
     // residue codes, each sequence 4-byte aligned and followed by pad codes
     // up to the end of its next word (seq_span), then the slots, then a tail
@@ -1074,7 +1112,7 @@ int swg_create(swg_ctx** out, const swg_params* p)
         return fail(SWG_E_ARGUMENT, "bad alignment_type");
     if (p->tdistance != SWG_TDIST_INT16 && p->tdistance != SWG_TDIST_DOUBLE) return fail(SWG_E_ARGUMENT, "bad tdistance");
     if (p->diag_mode != SWG_DIAG_LOWER_MIRROR && p->diag_mode != SWG_DIAG_FULL) return fail(SWG_E_ARGUMENT, "bad diag_mode");
-    if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPP16)
+    if (p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPS16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
     if (p->program != SWG_PROGRAM_SWMPI && p->program != SWG_PROGRAM_SWMS) return fail(SWG_E_ARGUMENT, "bad program");
     if (p->distance_fn < SWG_DIST_PERCENT_IDENTITY || p->distance_fn > SWG_DIST_MINMAX_NORM)
@@ -1436,7 +1474,7 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
 int swg_set_schedule(swg_ctx* c, int32_t schedule, int32_t strip_rows)
 {
     if (!c) return fail(SWG_E_ARGUMENT, "null argument");
-    if (schedule < SWG_SCHED_PERSISTENT || schedule > SWG_SCHED_PERSISTENT_LPP16)
+    if (schedule < SWG_SCHED_PERSISTENT || schedule > SWG_SCHED_PERSISTENT_LPS16)
         return fail(SWG_E_ARGUMENT, "bad schedule");
     if (strip_rows != 0 && strip_rows != 8 && strip_rows != 16 && strip_rows != 32)
         return fail(SWG_E_ARGUMENT, "strip_rows must be 0 (default), 8, 16 or 32");
@@ -1493,9 +1531,10 @@ int swg_last_block_config(swg_ctx* c, swg_block_config* out)
     out->variant = f.v == kVarPlain ? SWG_SCHED_PLAIN
                  : f.v == kVarI32   ? SWG_SCHED_PERSISTENT_I32
                  : f.v == kVarLpp   ? SWG_SCHED_PERSISTENT_LPP16
+                 : f.v == kVarLps   ? SWG_SCHED_PERSISTENT_LPS16
                                     : SWG_SCHED_PERSISTENT_I16;
     out->strip_rows = f.s1;
-    out->rerun_strip_rows = f.v == kVarI16 ? f.s2 : 0;
+    out->rerun_strip_rows = two_pass(f.v) ? f.s2 : 0;
     out->pairs_per_wave = pairs_per_wave(f.v);
     out->tuning = c->last_tuning;
     out->trials = c->last_trials;
@@ -1524,6 +1563,7 @@ int swg_last_kernel_times(swg_ctx* c, swg_kernel_time* out, int32_t max, int32_t
         t.cells_run = 0.0;
         switch (t.kernel) {
         case SWG_KERNEL_ONE_PASS: case SWG_KERNEL_GLOBAL: case SWG_KERNEL_LPP16: case SWG_KERNEL_SCORE16:
+        case SWG_KERNEL_LPS16:
             t.cells = c->kt_plan.cells;
             t.cells_run = c->kt_plan.cells_run;
             break;
@@ -1564,6 +1604,7 @@ int swg_last_kernel_stats(swg_ctx* c, double* kernel_ms, double* cells, int32_t*
         *variant = c->last_variant == kVarPlain ? SWG_SCHED_PLAIN
                  : c->last_variant == kVarI32   ? SWG_SCHED_PERSISTENT_I32
                  : c->last_variant == kVarLpp   ? SWG_SCHED_PERSISTENT_LPP16
+                 : c->last_variant == kVarLps   ? SWG_SCHED_PERSISTENT_LPS16
                                                 : SWG_SCHED_PERSISTENT_I16;
     return SWG_OK;
 }
@@ -1651,6 +1692,7 @@ struct ServeLane {
 struct swg_server {
     swg_ctx* c = nullptr;
     int32_t npanel = 0, max_batch = 0, max_qlen = 0, nlanes = 0;
+    int32_t sched = SWG_SCHED_PERSISTENT;  // batch kernel policy (swg_params.schedule)
     bool want_stats = true;
     int64_t budget = 0, used = 0;
     std::vector<std::unique_ptr<ServeLane>> lanes;
@@ -1710,9 +1752,12 @@ int launch_batch(swg_server* s, ServeLane& L)
     SWG_HIP(hipMemcpyAsync(c->len.p + L.slot_base, L.h_codes + pos, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
     if (c->dist_kind == kDistJukesCantor)
         SWG_HIP(launch_fill_i32(L.lane.undef.p, 0, 1 + nb, st));
-    // batches too small to fill the GPU one lane per pair use 16 lanes per
-    // pair; larger ones the int16 two-pass (or int32) variant -- no timing
-    // trials on the serving path
+    // batches use the int16 score pass with 16 lanes per pair where the
+    // alphabet allows (its waves hold 8 pairs: a query against the panel
+    // fills the GPU, and a deep queue's batches end evenly); else batches too
+    // small to fill the GPU one lane per pair use the int32 16-lanes-per-pair
+    // kernel and larger ones the int16 two-pass (or int32) variant -- no
+    // timing trials on the serving path
     // a partial batch that starts while another lane's batch is on the
     // device gets a 1/lanes share of the resident slots, so that neither
     // lane's later kernels wait for the other's persistent grid to drain
@@ -1723,10 +1768,15 @@ int launch_batch(swg_server* s, ServeLane& L)
     L.lane.share = others && nb < s->max_batch ? s->nlanes : 1;
     BlockGeom g;
     g.r0 = L.slot_base; g.r1 = L.slot_base + nb - 1; g.c0 = 0; g.c1 = s->npanel - 1; g.diag_lower = false;
-    c->schedule = c->nw                                  ? SWG_SCHED_PERSISTENT_I32
-                : thin_block(c, L.lane, g)                ? SWG_SCHED_PERSISTENT_LPP16
-                : c->int16_ok                            ? SWG_SCHED_PERSISTENT_I16
-                                                         : SWG_SCHED_PERSISTENT_I32;
+    const bool i16 = c->int16_ok && s->sched != SWG_SCHED_PERSISTENT_I32;
+    const int thin = c->lps_ok ? SWG_SCHED_PERSISTENT_LPS16 : SWG_SCHED_PERSISTENT_LPP16;
+    c->schedule = c->nw                                        ? SWG_SCHED_PERSISTENT_I32
+                : s->sched == SWG_SCHED_PERSISTENT_LPP16        ? SWG_SCHED_PERSISTENT_LPP16
+                : s->sched == SWG_SCHED_PERSISTENT_LPS16        ? thin
+                : s->sched == SWG_SCHED_PERSISTENT_I16 && i16   ? SWG_SCHED_PERSISTENT_I16
+                : s->sched == SWG_SCHED_PERSISTENT && (c->lps_ok || thin_block(c, L.lane, g)) ? thin
+                : i16                                           ? SWG_SCHED_PERSISTENT_I16
+                                                                : SWG_SCHED_PERSISTENT_I32;
     int64_t pairs = 0;
     int rc = enqueue_block(c, L.lane, L.slot_base, L.slot_base + nb - 1, 0, s->npanel - 1, 0, L.dout.p, s->npanel,
                            s->want_stats ? L.dstats.p : nullptr, &pairs);
@@ -1853,6 +1903,8 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
     *out = nullptr;
     if (p->alignment_type == SWG_ALIGN_NUCLEIC && p->distance_fn == SWG_DIST_MINMAX_NORM)
         return fail(SWG_E_UNSUPPORTED, "the server does not implement MinMaxNormScore");
+    if (p->schedule == SWG_SCHED_PLAIN || p->schedule < SWG_SCHED_PERSISTENT || p->schedule > SWG_SCHED_PERSISTENT_LPS16)
+        return fail(SWG_E_ARGUMENT, "server schedule: SWG_SCHED_PERSISTENT, _PERSISTENT_I32, _I16, _LPP16 or _LPS16");
     swg_params q = *p;
     q.schedule = SWG_SCHED_PERSISTENT_I32;
     swg_ctx* c = nullptr;
@@ -1863,6 +1915,7 @@ int swg_serve_create(swg_server** out, const swg_params* p, const uint8_t* panel
     auto* s = new swg_server();
     s->c = c;
     s->npanel = npanel; s->max_batch = max_batch; s->max_qlen = max_query_len; s->nlanes = lanes;
+    s->sched = p->schedule;
     s->want_stats = want_stats != 0;
     s->budget = hbm_budget_bytes;
     s->t0 = Clock::now();

@@ -105,13 +105,16 @@ constexpr uint32_t kCellTrack = 0x80000000u;
 // Sort of the block's pairs by first-maximum cell for the re-run: buckets of
 // wr x wc cells on a kBucketDim x kBucketDim grid, plus two lists, each
 // padded to whole 64-lane tiles: flagged pairs (the last bucket) and, in the
-// sort by re-run rectangle, rectangles of more than kBigRect cells (the one
-// before it), which one wave per pair re-runs.
+// sort by re-run rectangle, rectangles of more than SortSet::big_rect cells
+// (the one before it), which one wave per pair re-runs: kBigRect for
+// throughput, kBigRectThin for thin blocks, where the longest re-run of one
+// lane bounds the latency.
 constexpr int kBucketDim = 128;
 constexpr int kNumBuckets = kBucketDim * kBucketDim + 2;
 constexpr int kFlaggedBucket = kNumBuckets - 1;
 constexpr int kBigBucket = kNumBuckets - 2;
 constexpr int64_t kBigRect = 65536;
+constexpr int64_t kBigRectThin = 8192;
 
 // meta words (int32 index; the 64-bit counters at meta64[k] = meta + 2k)
 constexpr int kMetaFlaggedTiles = 0;   // tiles of the flagged list (slots first)
@@ -131,6 +134,7 @@ struct SortSet {
     const uint32_t* start_cell;   // null: key = (first-maximum score, min(bi, wend)) for the reverse pass;
                                   // else key = the re-run rectangle [si..bi] x [sj..min(jb+kFirstMaxBlock-1, Lb)]
     const int16_t* best_score;    // the first-maximum score of each pair (reverse-pass key)
+    int64_t big_rect;             // rectangles of more cells go to the one-wave-per-pair re-run
     const BlockDesc* desc;        // desc[0]: output (mirror), desc[1]: score pass (r0, c0, ld, rows, wr, wc)
     const int32_t* len;           // sequence lengths
     int32_t* meta;                // kMetaWords words, then the bucket counts
@@ -188,6 +192,16 @@ hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 // Score-only int16x2 pass; strip_rows in {16, 24, 32}.
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
                           int nslots, hipStream_t stream);
+// Score-only int16x2 pass with sixteen lanes per pair (thin blocks): tiles
+// of kLpsTile column sequences (lane group g: tile_col[tile*8 + g] in the low
+// half, tile_col[tile*8 + 4 + g] in the high half), the same best_cell /
+// best_score output; strip_rows (rows per lane) in {8, 16}; column
+// sequences coded < 8 with scores plus gap_open in [0, 255], gap_open >= 1
+// and a pad code < 8 or >= 12 (the host checks this: swg_ctx::lps_ok).
+constexpr int kLpsTile = 8;
+hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
+                        hipStream_t stream);
+int lps16_blocks_per_cu(int strip_rows);
 // Sort of the score pass's first-maximum cells (count, scan, scatter).
 hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
 // Re-run of the sorted pairs; strip_rows in {16, 32}; track = the flagged list.

@@ -937,6 +937,230 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Score pass with sixteen lanes per pair, for blocks with few row sequences
+// (serving batches, small blocks): the score16_kernel recurrence, saturation
+// rule and first-maximum output, laid out like lpp16_kernel.  A wave aligns
+// s1 (uniform) against kLpsTile sequences s2: one 16-lane DPP row per group,
+// two sequences per group in the int16 halves (pair A low, pair B high).
+// Lane k of a group owns R consecutive rows of a stripe of 16R rows and runs
+// one column behind lane k-1: at step t it computes column j = t - k + 1 of
+// its rows, taking the row above its first row (V - o and G of lane k-1's
+// last row, computed at step t - 1) and the selector of column j's codes
+// with DPP row shifts, while lane 0 reads the previous stripe's bottom row
+// from the boundary buffer and lane 15 writes this stripe's.
+//
+// Scores come from registers: per row, the (score + gap_open) of column codes
+// 0..7 as the bytes of two words (the column sequences' codes are < 8 and
+// their scores plus gap_open lie in [0, 255]; the row may hold any code), and
+// one v_perm per row picks both halves' bytes into the int16 halves: the
+// selector is the pair of column codes, min(code, 12) per half, so that a
+// pad code of 12 or more selects the byte 0.  The pad code scores 0, i.e.
+// -gap_open: pad columns
+// (past a sequence's end), pad rows (past s1's end) and the pre-columns a
+// lane runs before its first column hold values at most M - gap_open, M the
+// pair's maximum over its real cells (every path into them takes a step
+// scoring at most -gap_open), so they never take the first maximum; a
+// pre-column cell is exactly DP column 0 (V = 0, t = -o, H' = 0).
+//
+// The first maximum is folded per lane once every 16 steps: lane k's block
+// is then the columns s - k + 1 .. s - k + 16 of the steps s .. s + 15 (the
+// first column clamped to 1; the reverse pass and the re-run take a block at
+// any column), and at the end the 16 lanes' first maxima are merged: the
+// larger value, then the earlier row.
+constexpr int kLpsGroups = 4;  // 16-lane groups per wave
+
+template <int R>
+__device__ __forceinline__ void lps_fold(uint32_t (&bm)[R], uint32_t& GV, uint32_t& GR, uint32_t& GJ, uint32_t row1,
+                                         uint32_t JB)
+{
+    const uint32_t NEG2 = splat16(kPad16);
+    uint32_t BV = bm[0], BR = 0u;
+    bm[0] = NEG2;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        uint32_t mk;  // bm[r] > BV
+        asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(u2(v2(BV) - v2(bm[r]))));
+        BR = (mk & splat16(r)) | (~mk & BR);
+        BV = u2(__builtin_elementwise_max(v2(BV), v2(bm[r])));
+        bm[r] = NEG2;
+    }
+    // (BV, row, JB) replaces (GV, GR, GJ) if BV is larger, or equal in an
+    // earlier row (the lane's rows of earlier stripes are earlier rows)
+    BR = u2(v2(BR) + v2(row1));
+    uint32_t gt, lt, rl;
+    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(gt) : "s"(0x000F000Fu), "v"(u2(v2(GV) - v2(BV))));
+    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(lt) : "s"(0x000F000Fu), "v"(u2(v2(BV) - v2(GV))));
+    asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(rl) : "s"(0x000F000Fu), "v"(u2(v2(BR) - v2(GR))));
+    const uint32_t m = gt | (~lt & rl);
+    GR = (m & BR) | (~m & GR);
+    GJ = (m & JB) | (~m & GJ);
+    GV = u2(__builtin_elementwise_max(v2(GV), v2(BV)));
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
+{
+    // [c1][c2], 64 x 8: score + gap_open, 0 for the pad code (and codes past it)
+    __shared__ uint32_t tab[kMaxCodes * kJointMaxCodes / 4];
+    __shared__ uint32_t mrg[3 * kWave];  // per lane: first maximum (value, row, block)
+
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, k = lane & 15;
+    const int ncode = S.ncode, pad = ncode - 1;
+    for (int x = lane; x < kMaxCodes * kJointMaxCodes; x += kWave) {
+        const int c1 = x >> 3, c2 = x & 7;
+        const uint32_t b = c1 < pad && c2 < pad ? (uint32_t)S.prof[c1 * ncode + c2].x & 0xFFu : 0u;
+        reinterpret_cast<uint8_t*>(tab)[x] = (uint8_t)b;
+    }
+    __syncthreads();
+
+    int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
+    const BlockDesc* D = T.desc;
+    const int ntasks = D->ntasks;
+    const int o = S.gap_open, e = S.gap_ext;
+    const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
+    const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
+    const uint32_t padsel = (uint32_t)min(pad, 12) * 0x00010001u | 0x0C000C00u;
+    const u16x2 C12 = __builtin_bit_cast(u16x2, splat16(12));
+
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(T.queue, 1);
+        const int t_task = uni(v);
+        if (t_task >= ntasks) break;
+
+        const int a = uni(T.task_row[t_task]);
+        const int tile = uni(T.task_tile[t_task]);
+        const int maxLb = uni(T.tile_maxlen[tile]);
+        const int bA = T.tile_col[(int64_t)tile * kLpsTile + g];
+        const int bB = T.tile_col[(int64_t)tile * kLpsTile + kLpsGroups + g];
+        const int La = uni(S.len[a]);
+        const int LbA = bA >= 0 ? S.len[bA] : 0;
+        const int LbB = bB >= 0 ? S.len[bB] : 0;
+        const uint8_t* s1 = S.res + S.off[a];
+        const uint32_t* wA = reinterpret_cast<const uint32_t*>(S.res + (bA >= 0 ? S.off[bA] : S.pad_off));
+        const uint32_t* wB = reinterpret_cast<const uint32_t*>(S.res + (bB >= 0 ? S.off[bB] : S.pad_off));
+        // word (L + 3) / 4 holds only pad codes (and the words before it pad
+        // codes past L): word min(k, last) gives the pad code past the end
+        const int lastA = (LbA + 3) >> 2, lastB = (LbB + 3) >> 2;
+        const int nsteps = (maxLb + (kLpp - 1) + 3) & ~3;  // lane 15 reaches column maxLb
+
+        // row 0 of the DP (t = -o, G = -inf), also past the last column so
+        // that lane 0's look-ahead reads defined values
+        for (int j = 1 + lane; j <= nsteps + 8; j += kWave)
+            for (int q = 0; q < kLpsGroups; ++q) bnd[(int64_t)j * kLpsGroups + q] = make_int2((int)T0, (int)NEG2);
+        __syncthreads();  // other lanes read these columns
+
+        uint32_t GV = M1, GR = 0u, GJ = 0u;
+        for (int i0 = 0; i0 < La; i0 += kLpp * R) {
+            // this lane's rows: score bytes of codes 0..7 (pad rows: 0)
+            uint32_t tlo[R], thi[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = i0 + k * R + r;
+                const int c1 = row < La ? (int)s1[row] : pad;
+                tlo[r] = tab[2 * c1];
+                thi[r] = tab[2 * c1 + 1];
+            }
+            uint32_t tr[R], Hr[R], bm[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; bm[r] = NEG2; }
+            const uint32_t row1 = splat16(i0 + k * R + 1);  // (1-based) row of this lane's first row
+            uint32_t ot = T0, oG = NEG2;  // this lane's last row (t, G) of the previous step
+            uint32_t pt = T0;             // the row above, received at the previous step (diagonal of row 0)
+            uint32_t sel = padsel;        // this lane's column codes as a byte selector
+
+            uint32_t nwa = wA[0], nwb = wB[0];
+            int2 nb[4] = {bnd[1 * kLpsGroups + g], bnd[2 * kLpsGroups + g], bnd[3 * kLpsGroups + g],
+                          bnd[4 * kLpsGroups + g]};
+            for (int t = 0; t < nsteps; t += 4) {
+                const uint32_t cwa = nwa, cwb = nwb;
+                const int2 cb[4] = {nb[0], nb[1], nb[2], nb[3]};
+                nwa = wA[min((t >> 2) + 1, lastA)];
+                nwb = wB[min((t >> 2) + 1, lastB)];
+                const int64_t nxt = (int64_t)(t + 5) * kLpsGroups + g;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) nb[q] = bnd[nxt + q * kLpsGroups];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int tt = t + q;
+                    // lane 0: the codes of column tt + 1 (byte q of the words) as
+                    // a selector of bytes (cA, 0, cB, 0); lane k: lane k-1's
+                    const uint32_t s0 =
+                        __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2,
+                            __builtin_amdgcn_perm(cwb, cwa, 0x0C040C00u + (uint32_t)q * 0x00010001u)), C12)) |
+                        0x0C000C00u;
+                    sel = (uint32_t)row_shr1((int)s0, (int)sel);
+                    uint32_t tu = (uint32_t)row_shr1(cb[q].x, (int)ot);
+                    uint32_t Gu = (uint32_t)row_shr1(cb[q].y, (int)oG);
+                    uint32_t dt = pt;
+                    pt = tu;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const uint32_t sp = __builtin_amdgcn_perm(thi[r], tlo[r], sel);
+                        const s16x2 G = __builtin_elementwise_max(v2(Gu) - E2, v2(tu));
+                        // H' = max(H, 0), see score16_kernel
+                        const s16x2 H = __builtin_elementwise_max(
+                            __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                                __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))),
+                            v2(tr[r]));
+                        const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
+                        const s16x2 tn = V - O2;
+                        bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
+                        dt = tr[r];
+                        tr[r] = u2(tn);
+                        Hr[r] = u2(H);
+                        tu = u2(tn);
+                        Gu = u2(G);
+                    }
+                    ot = tu;
+                    oG = Gu;
+                    // lane 15: this stripe's last row at its column tt - 14
+                    if (k == kLpp - 1 && tt >= kLpp - 1)
+                        bnd[(int64_t)(tt - (kLpp - 2)) * kLpsGroups + g] = make_int2((int)ot, (int)oG);
+                }
+                if (((t + 3) & 15) == 15 || t + 4 >= nsteps)
+                    lps_fold<R>(bm, GV, GR, GJ, row1, splat16(max(1, ((t + 3) & ~15) - k + 1)));
+            }
+        }
+
+        // merge the group's 16 lanes: the larger value, then the earlier row
+        mrg[lane] = GV;
+        mrg[kWave + lane] = GR;
+        mrg[2 * kWave + lane] = GJ;
+        __syncthreads();
+        if (k == 0) {
+            int best[2] = {-1, -1}, brow[2] = {0, 0}, bcol[2] = {0, 0};
+            for (int x = 0; x < kLpp; ++x) {
+                const uint32_t xv = mrg[lane + x], xr = mrg[kWave + lane + x], xj = mrg[2 * kWave + lane + x];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int val = (short)(xv >> (16 * h)), row = (int)((xr >> (16 * h)) & 0xFFFFu);
+                    if (val > best[h] || (val == best[h] && row < brow[h])) {
+                        best[h] = val;
+                        brow[h] = row;
+                        bcol[h] = (int)((xj >> (16 * h)) & 0xFFFFu);
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int b = h ? bB : bA, Lb = h ? LbB : LbA;
+                if (b < 0) continue;
+                const uint32_t cell = (La == 0 || Lb == 0 || best[h] >= kSat16)
+                                          ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)Lb)
+                                          : (((uint32_t)brow[h] << 16) | (uint32_t)bcol[h]);
+                const int64_t pos = (int64_t)(a - D->r0) * D->ld + (b - D->c0);
+                T.best_cell[pos] = cell;
+                T.best_score[pos] = (int16_t)best[h];
+            }
+        }
+        __syncthreads();  // mrg is reused by the next task
+    }
+}
+
 __device__ __forceinline__ int wave_max(int v)
 {
 #pragma unroll
@@ -1006,7 +1230,7 @@ __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1
     *cells = (uint64_t)h * (uint64_t)w;
     if (!P.start_cell)
         return score_bucket((int)P.best_score[pos]) * kBucketDim + side_bucket(min(bi, wend), max(D1.wr, D1.wc), false);
-    if ((int64_t)*cells > kBigRect) return kBigBucket;
+    if ((int64_t)*cells > P.big_rect) return kBigBucket;
     return side_bucket(h, D1.wr, true) * kBucketDim + side_bucket(w, D1.wc, true);
 }
 
@@ -1091,36 +1315,54 @@ __device__ __forceinline__ int rank_slots(const SortSet& P, int q)
     return q <= 1 ? (n + kWave - 1) / kWave * kWave : n;
 }
 
-// One wave: exclusive scan of the slot counts in rank order.
-__global__ __launch_bounds__(64) void sort_scan_kernel(SortSet P)
+// One workgroup of kScanThreads: exclusive scan of the slot counts in rank
+// order, kScanPer consecutive ranks per thread (their counts loaded at once,
+// then a scan over the threads: waves by shuffles, the wave totals in LDS).
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = (kNumBuckets + kScanThreads - 1) / kScanThreads;
+
+__global__ __launch_bounds__(kScanThreads) void sort_scan_kernel(SortSet P)
 {
-    const int lane = threadIdx.x;
-    constexpr int per = (kNumBuckets + kWave - 1) / kWave;
-    const int q0 = lane * per;
+    __shared__ int wsum[kScanThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    const int q0 = tid * kScanPer;
+    int cnt[kScanPer];
     int sum = 0;
-    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) sum += rank_slots(P, q);
-    // inclusive prefix over lanes
+#pragma unroll
+    for (int u = 0; u < kScanPer; ++u) {
+        cnt[u] = q0 + u < kNumBuckets ? rank_slots(P, q0 + u) : 0;
+        sum += cnt[u];
+    }
+    // inclusive prefix over the wave's lanes, then over the waves
     int incl = sum;
 #pragma unroll
-    for (int s = 1; s < kWave; s <<= 1) {
-        const int y = __shfl_up(incl, s, kWave);
-        if (lane >= s) incl += y;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int y = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += y;
     }
-    int slot = incl - sum;
-    for (int q = q0; q < q0 + per && q < kNumBuckets; ++q) {
-        const int k = bucket_at_rank(q);
-        P.base[k] = slot;
-        P.cursor[k] = 0;
-        slot += rank_slots(P, q);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int x = 0; x < kScanThreads / kWave; ++x) {
+        before += x < w ? wsum[x] : 0;
+        total += wsum[x];
     }
-    if (lane == 0) {
+    int slot = before + incl - sum;
+#pragma unroll
+    for (int u = 0; u < kScanPer; ++u) {
+        if (q0 + u < kNumBuckets) {
+            const int k = bucket_at_rank(q0 + u);
+            P.base[k] = slot;
+            P.cursor[k] = 0;
+        }
+        slot += cnt[u];
+    }
+    if (tid == 0) {
         const int tf = (P.count[kFlaggedBucket] + kWave - 1) / kWave;
         P.meta[kMetaFlaggedTiles] = tf;
         P.meta[kMetaBigTiles] = tf + (P.count[kBigBucket] + kWave - 1) / kWave;
-    }
-    if (lane == kWave - 1) {
-        P.meta[kMetaTiles] = (incl + kWave - 1) / kWave;
-        P.meta[kMetaSlots] = incl;
+        P.meta[kMetaTiles] = (total + kWave - 1) / kWave;
+        P.meta[kMetaSlots] = total;
     }
 }
 
@@ -1373,7 +1615,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
 }
 
 // ---------------------------------------------------------------------------
-// Re-run of the big rectangles (more than kBigRect cells), one wave per pair.
+// Re-run of the big rectangles (more than SortSet::big_rect cells), one wave per pair.
 // One lane per pair would leave a single lane with millions of cells while the
 // rest of the GPU idles.  Lane k owns R consecutive rows of a stripe of 64*R
 // rows of the rectangle and runs one column behind lane k-1 (anti-diagonal
@@ -2097,12 +2339,35 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
     return hipGetLastError();
 }
 
+hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
+                        hipStream_t st)
+{
+    switch (strip_rows) {
+    case 8: hipLaunchKernelGGL((lps16_kernel<8>), dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
+    case 16: hipLaunchKernelGGL((lps16_kernel<16>), dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int lps16_blocks_per_cu(int strip_rows)
+{
+    int nb = 0;
+    hipError_t err = hipErrorInvalidValue;
+    switch (strip_rows) {
+    case 8: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8>, kWave, 0); break;
+    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<16>, kWave, 0); break;
+    default: break;
+    }
+    return err == hipSuccess ? nb : 0;
+}
+
 hipError_t launch_sort(const SortSet& P, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(kSortThreads), 0, st, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kWave), 0, st, P);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(kSortThreads), 0, st, P);
     return hipGetLastError();

@@ -73,6 +73,12 @@ extern "C" {
 #define SWG_SCHED_PERSISTENT_I16 3  /* persistent, forced int16 score pass + bounded int32 re-run */
 #define SWG_SCHED_PERSISTENT_LPP16 4 /* persistent, 16 lanes per pair (anti-diagonal, DPP row shifts),
                                        4 pairs per wave: latency when few row sequences are in flight */
+#define SWG_SCHED_PERSISTENT_LPS16 5 /* persistent, the int16 score pass with 16 lanes per pair (8 pairs per
+                                       wave) + the bounded int32 re-run, where the column sequences use
+                                       at most 7 symbols, score + gap_open lies in [0, 255] for them and
+                                       gap_open >= 1 (e.g. DNA with EDNAFULL), else as
+                                       _PERSISTENT_LPP16.  The automatic choice for thin blocks and
+                                       small serving batches */
 
 #define SWG_MATRIX_SIZE 127      /* ScoringMatrix.SIZE, Salsa.CoreTPL/Bio/Algorithms/ScoringMatrix.cs:34 */
 #define SWG_MAX_SEQ_LEN 32767    /* gap-size arrays are short in the reference (SmithWatermanGotoh.cs:83-84) */
@@ -230,14 +236,17 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
  * the winner's first strip (score pass 24/32/16, one-pass 8/16/32, rows per lane
  * 4/8, global kernel 8/16), then the two-pass re-run strip (16/32); the
  * highest cells/ms wins.  Thin blocks (too few rows to fill the GPU one lane
- * per pair) always take 16 lanes per pair. */
+ * per pair) always take 16 lanes per pair: the int16 score pass so
+ * (_PERSISTENT_LPS16) where its alphabet and scores allow, else the int32
+ * kernel (_PERSISTENT_LPP16). */
 typedef struct swg_block_config {
-    int32_t variant;          /* SWG_SCHED_PLAIN, _PERSISTENT_I32, _PERSISTENT_I16 or _PERSISTENT_LPP16 */
+    int32_t variant;          /* SWG_SCHED_PLAIN, _PERSISTENT_I32, _PERSISTENT_I16, _PERSISTENT_LPP16
+                                 or _PERSISTENT_LPS16 */
     int32_t strip_rows;       /* rows per strip of the first kernel (score pass, one-pass or global
                                  kernel) or rows per lane (16 lanes per pair) */
     int32_t rerun_strip_rows; /* two-pass: rows per strip of the int32 re-run; 0 otherwise */
     int32_t pairs_per_wave;   /* 128 score pass (two pairs per lane), 64 one-pass / global, 4 sixteen
-                                 lanes per pair */
+                                 lanes per pair, 8 score pass with sixteen lanes per pair */
     int32_t tuning;           /* 0 fixed by the schedule (or a thin block), 1 an autotuner trial,
                                  2 the autotuner's choice */
     int32_t trials;           /* configurations the autotuner has timed for this block's key */
@@ -257,6 +266,7 @@ int  swg_last_block_config(swg_ctx* ctx, swg_block_config* out);
 #define SWG_KERNEL_RECT64        9  /* re-run of the big rectangles, one wave per pair */
 #define SWG_KERNEL_DISTANCE     10  /* Jukes-Cantor / Kimura2 distances from the carried counts
                                        (Kimura2 runs the statistics kernels twice before it) */
+#define SWG_KERNEL_LPS16        11  /* int16x2 score pass, 16 lanes per pair */
 typedef struct swg_kernel_time {
     int32_t kernel;     /* SWG_KERNEL_* */
     int32_t reserved;

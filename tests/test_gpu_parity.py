@@ -78,9 +78,10 @@ def protein_set():
 
 SCHEDULES = [(ca.SCHED_PERSISTENT, 0), (ca.SCHED_PLAIN, 16), (ca.SCHED_PERSISTENT_I32, 8),
              (ca.SCHED_PERSISTENT_I32, 16), (ca.SCHED_PERSISTENT_I32, 32), (ca.SCHED_PERSISTENT_I16, 16),
-             (ca.SCHED_PERSISTENT_I16, 32), (ca.SCHED_PERSISTENT_LPP16, 0), (ca.SCHED_PERSISTENT_LPP16, 8)]
+             (ca.SCHED_PERSISTENT_I16, 32), (ca.SCHED_PERSISTENT_LPP16, 0), (ca.SCHED_PERSISTENT_LPP16, 8),
+             (ca.SCHED_PERSISTENT_LPS16, 0), (ca.SCHED_PERSISTENT_LPS16, 32)]
 BLOCK_SCHEDULES = [ca.SCHED_PERSISTENT, ca.SCHED_PLAIN, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16,
-                   ca.SCHED_PERSISTENT_LPP16]
+                   ca.SCHED_PERSISTENT_LPP16, ca.SCHED_PERSISTENT_LPS16]
 
 
 @pytest.mark.parametrize("schedule,strip", SCHEDULES)
@@ -218,7 +219,8 @@ def test_full_matrix_read_by_consumers(tmp_path, dna_set, ednafull):
     assert np.array_equal(full, full.T)
 
 
-@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16])
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16,
+                                      ca.SCHED_PERSISTENT_LPS16])
 def test_long_sequences(ednafull, schedule):
     say("long sequences schedule=%d" % schedule)
     s = ca.synth_dna(13, 70, 3, 0.10, 0.02, 0, [1500, 2000])
@@ -231,7 +233,7 @@ def test_long_sequences(ednafull, schedule):
 
 
 @pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT, ca.SCHED_PERSISTENT_I32, ca.SCHED_PERSISTENT_I16,
-                                      ca.SCHED_PERSISTENT_LPP16])
+                                      ca.SCHED_PERSISTENT_LPP16, ca.SCHED_PERSISTENT_LPS16])
 def test_very_long_sequences(ednafull, schedule):
     """7,800-8,000-residue relatives (they score above the int16 range: the
     score pass flags them for the whole-matrix int32 re-run), unrelated
@@ -382,14 +384,17 @@ def test_autotuned_blocks_match(ednafull):
     al.close()
 
 
-def test_serving_same_alone_and_in_mix(ednafull):
+@pytest.mark.parametrize("schedule", [ca.SCHED_PERSISTENT, ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_LPP16,
+                                      ca.SCHED_PERSISTENT_LPS16])
+def test_serving_same_alone_and_in_mix(ednafull, schedule):
     # each query's distances must be identical when it runs alone and when it
     # shares batches with others, and equal to the oracle's
-    say("serving alone vs mix")
+    say("serving alone vs mix, schedule=%d" % schedule)
     panel = ca.synth_dna(21, 300, 10, 0.10, 0.02, 0, [30, 250])
     qs = ca.synth_dna(22, 24, 10, 0.10, 0.02, 0, [1, 400])
     queries = [qs.seq(i) for i in range(len(qs))] + [b"", b"NACGU"]
-    srv = ca.Server(panel, ednafull, 16, 4, max_query_len=600, max_batch=5, lanes=2, hbm_budget=64 << 20)
+    srv = ca.Server(panel, ednafull, 16, 4, max_query_len=600, max_batch=5, lanes=2, hbm_budget=64 << 20,
+                    schedule=schedule)
     alone = []
     for q in queries:
         t = srv.submit(q)
@@ -476,6 +481,7 @@ def test_swms_restart_reproduces_block_files(tmp_path, ednafull):
 
 @pytest.mark.parametrize("schedule,names", [
     (ca.SCHED_PERSISTENT_I16, ["score16", "sort", "reverse", "sort", "rerun_track", "rerun_capture", "rect64"]),
+    (ca.SCHED_PERSISTENT_LPS16, ["lps16", "sort", "reverse", "sort", "rerun_track", "rerun_capture", "rect64"]),
     (ca.SCHED_PERSISTENT_I32, ["one_pass"]), (ca.SCHED_PLAIN, ["one_pass"])])
 def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
     say("kernel timing schedule=%d" % schedule)
@@ -502,7 +508,7 @@ def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
     L = dna_set.lengths.astype(np.float64)
     block = float(L[0:90].sum() * L[90:n].sum())
     for name, _, c, run in kt:
-        if name in ("score16", "one_pass"):
+        if name in ("score16", "one_pass", "lps16"):
             assert c == block
         if name in ("rerun_capture", "reverse"):
             assert 0 < c < block
