@@ -181,7 +181,8 @@ struct VariantCfg {
 // measured so far and their rates, the candidates of the current stage not
 // yet measured, and the choice once every stage is done.
 struct TuneEntry {
-    struct Trial { VariantCfg cfg; double rate; };
+    // chunk: timed on a trial chunk of the host entry point (first stage)
+    struct Trial { VariantCfg cfg; double rate; bool chunk; };
     int64_t key = 0;
     int stage = 0;
     bool done = false;
@@ -267,6 +268,7 @@ struct swg_ctx {
     VariantCfg last_cfg{};
     int32_t last_tuning = 0;   // 0 fixed, 1 autotuner trial, 2 autotuner choice
     int32_t last_trials = 0;   // trials measured for the last block's key
+    bool trial_chunk = false;  // the block being enqueued is a trial chunk (compute_block_streamed)
 
     // kernel-timing mode: a stamp kernel before the block's kernels and after
     // each of them (inside the block's graph), wall-clock ticks at wall_khz
@@ -800,6 +802,10 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
     if (c->strip_rows != 0 || c->score_strip != 0) return out;
     const VariantCfg b = t.best;
     VariantCfg f = b;
+    // the incumbent again, unless it was timed on a whole block (tune_advance
+    // drops configurations timed so): a trial chunk's rate is not comparable
+    // with a whole block's
+    if (t.stage == 1 || (t.stage == 2 && b.v == kVarI16 && !c->nw)) out.push_back(b);
     if (t.stage == 1) {
         if (c->nw) { f.s1 = b.s1 == 16 ? 8 : 16; out.push_back(f); }
         else if (b.v == kVarI16)
@@ -817,17 +823,19 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
 }
 
 // Move the entry on once its stage's candidates are all measured: the best
-// so far, then the next stage's untried candidates, or done.
+// so far (after the variant stage, leaving out trial-chunk rates, which are
+// not comparable with whole chunks'), then the next stage's candidates not
+// yet timed on a whole chunk, or done.
 void tune_advance(const swg_ctx* c, TuneEntry& t)
 {
     while (t.pending.empty() && !t.done) {
         double best_rate = -1.0;
         for (const auto& x : t.trials)
-            if (x.rate > best_rate) { best_rate = x.rate; t.best = x.cfg; }
+            if (x.rate > best_rate && (t.stage <= 1 || !x.chunk)) { best_rate = x.rate; t.best = x.cfg; }
         if (t.stage >= 3) { t.done = true; break; }
         for (const VariantCfg& f : stage_candidates(c, t)) {
             bool tried = false;
-            for (const auto& x : t.trials) tried = tried || x.cfg == f;
+            for (const auto& x : t.trials) tried = tried || (x.cfg == f && !x.chunk);
             if (!tried) t.pending.push_back(f);
         }
         ++t.stage;
@@ -881,6 +889,29 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* c
     return SWG_OK;
 }
 
+// True while the autotuner has timed the first of its kernel variants for
+// the key of block g and still has others to time (its next block of at
+// least tune_min_cells runs one).
+bool tuning_pending(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
+{
+    if (c->schedule != SWG_SCHED_PERSISTENT || (!c->nw && thin_block(c, ln, g))) return false;
+    const int64_t key = tune_key(c, g);
+    for (const auto& e : c->tuned)
+        if (e.key == key) return !e.done && e.stage <= 1 && !e.trials.empty();
+    return false;
+}
+
+// Rows of a trial chunk: the variants after the first one (the expected
+// best, timed on a whole chunk) may run several times slower, so the host
+// entry point times them on chunks of about kTrialPairs pairs (still in the
+// large size class of tune_key) rather than on whole ones; their rates only
+// decide the variant, whose differences are large.  The strip stages, whose
+// candidates differ by a few percent, are timed on whole chunks and compared
+// with whole chunks only (a small chunk's end of the persistent queue weighs
+// more, and differently for each strip height).
+constexpr double kTrialPairs = 4.5e6;
+int32_t trial_rows(int64_t cols) { return (int32_t)std::max<int64_t>(1, (int64_t)std::ceil(kTrialPairs / cols)); }
+
 // Enqueue one block on a lane's stream into device buffers.
 int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int32_t c1, int32_t is_diag,
                   void* dev_out, int64_t ld, int32_t* dev_stats, int64_t* pairs_done)
@@ -915,7 +946,7 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
         float ms = 0.0f;
         SWG_HIP(hipEventElapsedTime(&ms, ln.ev0, ln.ev1));
         c->last_kernel_ms = ms;
-        entry->trials.push_back({cfg, plan.cells / std::max(1e-3, (double)ms)});
+        entry->trials.push_back({cfg, plan.cells / std::max(1e-3, (double)ms), c->trial_chunk});
         entry->pending.erase(entry->pending.begin());
         tune_advance(c, *entry);
     }
@@ -1371,6 +1402,8 @@ namespace {
 // A block whose output exceeds one chunk is computed in row chunks: chunk k
 // runs on the compute stream into device buffer k%2 while the copy stream
 // moves chunk k-1 to pinned memory and the host copies chunk k-2 into `out`.
+// While the autotuner times configurations for the chunks' key, the chunks
+// are trial chunks (trial_rows).
 int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1, void* out, int64_t out_ld,
                            int64_t* pairs_done)
 {
@@ -1395,25 +1428,38 @@ int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32
         c->stream_host_bytes = chunk_bytes;
     }
     for (int b = 0; b < 2; ++b) SWG_HIP(c->stream_dev[b].reserve(chunk_bytes));
-    const int32_t nchunks = (int32_t)((r1 - r0 + 1 + chunk_rows - 1) / chunk_rows);
+    const int32_t trial = std::min(chunk_rows, trial_rows(cols));
     int64_t total_pairs = 0;
     double cells = 0.0;
+    int32_t span[2][2] = {};  // rows of the chunk in buffer b
     auto consume = [&](int32_t k) -> int {
         const int b = k & 1;
-        const int32_t a0 = r0 + k * chunk_rows, a1 = std::min(r1, a0 + chunk_rows - 1);
+        const int32_t a0 = span[b][0], a1 = span[b][1];
         SWG_HIP(hipEventSynchronize(c->ev_copy[b]));
         uint8_t* dst = static_cast<uint8_t*>(out) + (int64_t)(a0 - r0) * out_ld * es;
         for (int32_t i = 0; i < a1 - a0 + 1; ++i)
             std::memcpy(dst + (int64_t)i * out_ld * es, c->stream_host[b] + (int64_t)i * row_bytes, row_bytes);
         return SWG_OK;
     };
-    for (int32_t k = 0; k < nchunks; ++k) {
+    int32_t k = 0;
+    for (int32_t a0 = r0; a0 <= r1; a0 = span[k & 1][1] + 1, ++k) {
         const int b = k & 1;
-        const int32_t a0 = r0 + k * chunk_rows, a1 = std::min(r1, a0 + chunk_rows - 1);
+        BlockGeom g;
+        g.r0 = a0; g.r1 = std::min(r1, a0 + chunk_rows - 1); g.c0 = c0; g.c1 = c1; g.diag_lower = false;
+        int32_t a1 = g.r1;
+        if (tuning_pending(c, c->main, g)) {
+            a1 = std::min(r1, a0 + trial - 1);
+            if (r1 - a1 < trial) a1 = g.r1;  // no remainder below the trial size
+        }
+        c->trial_chunk = a1 - a0 + 1 < g.r1 - a0 + 1;
         // device buffer b is free once chunk k-2's copy finished
         SWG_HIP(hipStreamWaitEvent(c->main.stream, c->ev_copy[b], 0));
+        // (chunk k-2's rows were consumed before this chunk was planned)
+        span[b][0] = a0;
+        span[b][1] = a1;
         int64_t p = 0;
         int rc = enqueue_block(c, c->main, a0, a1, c0, c1, 0, c->stream_dev[b].p, cols, nullptr, &p);
+        c->trial_chunk = false;
         if (rc) return rc;
         total_pairs += p;
         cells += c->last_cells;
@@ -1424,7 +1470,7 @@ int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32
         SWG_HIP(hipEventRecord(c->ev_copy[b], c->copy_stream));
         if (k >= 1 && (rc = consume(k - 1))) return rc;
     }
-    int rc = consume(nchunks - 1);
+    int rc = consume(k - 1);
     if (rc) return rc;
     SWG_HIP(hipStreamSynchronize(c->main.stream));
     for (auto& B : c->main.bufs) B.pending = false;
@@ -1447,9 +1493,15 @@ int swg_compute_block(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32_t c1
     if (out_ld < cols) return fail(SWG_E_ARGUMENT, "out_ld smaller than the block width");
     const int es = elem_size(c);
     const bool mirror = is_diag && c->p.diag_mode == SWG_DIAG_LOWER_MIRROR;
-    if (!mirror && !stats && (size_t)(rows * cols * es) > c->stream_chunk_bytes && c->res.p && r0 >= 0 && c0 >= 0 &&
-        r1 < c->nseq && c1 < c->nseq)
-        return compute_block_streamed(c, r0, r1, c0, c1, out, out_ld, pairs_done);
+    if (!mirror && !stats && c->res.p && r0 >= 0 && c0 >= 0 && r1 < c->nseq && c1 < c->nseq) {
+        // in chunks: output larger than one chunk, or a block the autotuner
+        // would time whole while trial chunks would do
+        BlockGeom g;
+        g.r0 = r0; g.r1 = r1; g.c0 = c0; g.c1 = c1; g.diag_lower = false;
+        if ((size_t)(rows * cols * es) > c->stream_chunk_bytes ||
+            (rows >= 2 * (int64_t)trial_rows(cols) && tuning_pending(c, c->main, g)))
+            return compute_block_streamed(c, r0, r1, c0, c1, out, out_ld, pairs_done);
+    }
     const size_t out_bytes = (size_t)(rows * cols * es + 3) & ~size_t(3);
     SWG_HIP(c->out_stage.reserve(out_bytes));
     if (stats) SWG_HIP(c->stats_stage.reserve((size_t)rows * cols * 4));

@@ -139,3 +139,35 @@ def test_reverse_strip_heights(wide_set, wide_expected, ednafull, reverse_strip,
     assert np.array_equal(got.ravel(), d)
     assert np.array_equal(gst.reshape(-1, 4)[:, :3], st[:, :3])
     al.close()
+
+
+def test_trial_chunks_of_a_large_block(ednafull):
+    """The autotuner times its first variant on a whole block, then the host
+    entry point computes the next large block in trial chunks of about 4.5e6
+    pairs, one further variant each, and the strip stages on whole blocks;
+    every call gives the oracle's distances (checked on a seeded sample, then
+    byte for byte against the first call)."""
+    say("trial chunks")
+    s = ca.synth_dna(62, 6400, 40, 0.10, 0.02, 0, [30, 90])
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(s)
+    al.set_tune_min_cells(1e9)
+    got, _, pairs = al.compute_block(0, 2399, 2400, 6399, False)
+    assert pairs == 2400 * 4000
+    assert al.last_block_config()["trials"] == 1
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2400, 20000)
+    b = rng.integers(2400, 6400, 20000)
+    want, _ = O.sw_pairs_mt(s.residues, s.offsets, a, b, ednafull, 16, 4, threads=THREADS)
+    assert np.array_equal(got[a, b - 2400], want)
+    again, _, _ = al.compute_block(0, 2399, 2400, 6399, False)
+    assert np.array_equal(again, got)
+    # two variant trials in one call: a trial chunk and the rest of the block
+    assert al.last_block_config()["trials"] == 3
+    for _ in range(8):  # the strip stages, until the choice is made
+        again, _, _ = al.compute_block(0, 2399, 2400, 6399, False)
+        assert np.array_equal(again, got)
+        if al.last_block_config()["tuning"] == 2:
+            break
+    assert al.last_block_config()["tuning"] == 2
+    al.close()
