@@ -181,8 +181,8 @@ struct VariantCfg {
 // measured so far and their rates, the candidates of the current stage not
 // yet measured, and the choice once every stage is done.
 struct TuneEntry {
-    // chunk: timed on a trial chunk of the host entry point (first stage)
-    struct Trial { VariantCfg cfg; double rate; bool chunk; };
+    // stage: the search stage the trial belonged to (0: the variants)
+    struct Trial { VariantCfg cfg; double rate; int stage; };
     int64_t key = 0;
     int stage = 0;
     bool done = false;
@@ -268,7 +268,6 @@ struct swg_ctx {
     VariantCfg last_cfg{};
     int32_t last_tuning = 0;   // 0 fixed, 1 autotuner trial, 2 autotuner choice
     int32_t last_trials = 0;   // trials measured for the last block's key
-    bool trial_chunk = false;  // the block being enqueued is a trial chunk (compute_block_streamed)
 
     // kernel-timing mode: a stamp kernel before the block's kernels and after
     // each of them (inside the block's graph), wall-clock ticks at wall_khz
@@ -802,10 +801,13 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
     if (c->strip_rows != 0 || c->score_strip != 0) return out;
     const VariantCfg b = t.best;
     VariantCfg f = b;
-    // the incumbent again, unless it was timed on a whole block (tune_advance
-    // drops configurations timed so): a trial chunk's rate is not comparable
-    // with a whole block's
-    if (t.stage == 1 || (t.stage == 2 && b.v == kVarI16 && !c->nw)) out.push_back(b);
+    // the strip stages start with the incumbent again when several variants
+    // were timed: their rates come from trial chunks and other blocks (a
+    // diagonal block, a block computed whole), not comparable with the
+    // strips' few percent
+    int variants_timed = 0;
+    for (const auto& x : t.trials) variants_timed += x.stage == 0 ? 1 : 0;
+    if (t.stage == 1 && variants_timed >= 2) out.push_back(b);
     if (t.stage == 1) {
         if (c->nw) { f.s1 = b.s1 == 16 ? 8 : 16; out.push_back(f); }
         else if (b.v == kVarI16)
@@ -823,19 +825,19 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
 }
 
 // Move the entry on once its stage's candidates are all measured: the best
-// so far (after the variant stage, leaving out trial-chunk rates, which are
-// not comparable with whole chunks'), then the next stage's candidates not
-// yet timed on a whole chunk, or done.
+// so far (after the variant stage, among the strip stages' trials only),
+// then the next stage's candidates not yet timed in the strip stages, or
+// done.
 void tune_advance(const swg_ctx* c, TuneEntry& t)
 {
     while (t.pending.empty() && !t.done) {
         double best_rate = -1.0;
         for (const auto& x : t.trials)
-            if (x.rate > best_rate && (t.stage <= 1 || !x.chunk)) { best_rate = x.rate; t.best = x.cfg; }
+            if (x.rate > best_rate && (t.stage <= 1 || x.stage >= 1)) { best_rate = x.rate; t.best = x.cfg; }
         if (t.stage >= 3) { t.done = true; break; }
         for (const VariantCfg& f : stage_candidates(c, t)) {
             bool tried = false;
-            for (const auto& x : t.trials) tried = tried || (x.cfg == f && !x.chunk);
+            for (const auto& x : t.trials) tried = tried || (x.cfg == f && t.stage >= 1 && x.stage >= 1);
             if (!tried) t.pending.push_back(f);
         }
         ++t.stage;
@@ -907,8 +909,8 @@ bool tuning_pending(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
 // large size class of tune_key) rather than on whole ones; their rates only
 // decide the variant, whose differences are large.  The strip stages, whose
 // candidates differ by a few percent, are timed on whole chunks and compared
-// with whole chunks only (a small chunk's end of the persistent queue weighs
-// more, and differently for each strip height).
+// among themselves (a small chunk's end of the persistent queue weighs more,
+// and differently for each strip height).
 constexpr double kTrialPairs = 4.5e6;
 int32_t trial_rows(int64_t cols) { return (int32_t)std::max<int64_t>(1, (int64_t)std::ceil(kTrialPairs / cols)); }
 
@@ -946,7 +948,7 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
         float ms = 0.0f;
         SWG_HIP(hipEventElapsedTime(&ms, ln.ev0, ln.ev1));
         c->last_kernel_ms = ms;
-        entry->trials.push_back({cfg, plan.cells / std::max(1e-3, (double)ms), c->trial_chunk});
+        entry->trials.push_back({cfg, plan.cells / std::max(1e-3, (double)ms), entry->stage - 1});
         entry->pending.erase(entry->pending.begin());
         tune_advance(c, *entry);
     }
@@ -1451,7 +1453,6 @@ int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32
             a1 = std::min(r1, a0 + trial - 1);
             if (r1 - a1 < trial) a1 = g.r1;  // no remainder below the trial size
         }
-        c->trial_chunk = a1 - a0 + 1 < g.r1 - a0 + 1;
         // device buffer b is free once chunk k-2's copy finished
         SWG_HIP(hipStreamWaitEvent(c->main.stream, c->ev_copy[b], 0));
         // (chunk k-2's rows were consumed before this chunk was planned)
@@ -1459,7 +1460,6 @@ int compute_block_streamed(swg_ctx* c, int32_t r0, int32_t r1, int32_t c0, int32
         span[b][1] = a1;
         int64_t p = 0;
         int rc = enqueue_block(c, c->main, a0, a1, c0, c1, 0, c->stream_dev[b].p, cols, nullptr, &p);
-        c->trial_chunk = false;
         if (rc) return rc;
         total_pairs += p;
         cells += c->last_cells;

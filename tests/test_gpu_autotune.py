@@ -76,15 +76,17 @@ def test_autotuner_searches_variant_then_tile_shape(wide_set, wide_expected, edn
     al.set_tune_min_cells(0)  # time every block, however small
     trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), wide_expected)
     seen = [shape(t) for t in trials]
-    assert len(set(seen)) == len(seen), "a configuration was measured twice"
     # stage 0: the three variants at their default tile shapes, the two-pass
     # first for a 4-code alphabet
     assert seen[:3] == [(ca.SCHED_PERSISTENT_I16, 24, 16), (ca.SCHED_PERSISTENT_I32, 16, 0),
                         (ca.SCHED_PERSISTENT_LPP16, 4, 0)]
-    # later stages vary the winner's strips only
+    # later stages vary the winner's strips only, starting with the winner
+    # itself timed again; no other configuration is measured twice
     winner = [t for t in trials[3:]]
     assert winner, "no tile-shape stage ran"
     assert len({t["variant"] for t in winner}) == 1
+    assert seen[3] in seen[:3]
+    assert len(set(seen[3:])) == len(seen[3:]), "a configuration was measured twice in the strip stages"
     assert shape(final) in seen and final["trials"] == len(trials)
     assert [t["trials"] for t in trials] == list(range(1, len(trials) + 1))
     # the choice sticks
