@@ -254,6 +254,15 @@ struct swg_ctx {
     size_t stream_host_bytes = 0;
     size_t stream_chunk_bytes = size_t(64) << 20;
 
+    // column tiles of the last off-diagonal block (a server's panel is the
+    // same column range for every batch): valid for (c0, c1, width) of this
+    // sequence set
+    struct ColTiles {
+        int32_t c0 = -1, c1 = -1, width = 0;
+        std::vector<int32_t> tcol, tmax;
+        double sum_cols = 0.0;
+    } col_tiles;
+
     int32_t schedule = SWG_SCHED_PERSISTENT;
     int32_t strip_rows = 0;   // int32 kernels' strip (re-run strip of the two-pass); 0 = autotuned
     int32_t score_strip = 0;  // score pass strip; 0 = autotuned
@@ -383,13 +392,22 @@ void order_block(const swg_ctx* c, const BlockGeom& g, bool sorted, std::vector<
 // diagonal blocks share one tile set across rows; a lower-mirror diagonal
 // block gets per-row tiles of its columns j < i.  Tasks are emitted row by
 // row, rows longest first, so the queue drains long work first.
-void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, std::vector<int32_t>& trow,
+void plan_tiles(swg_ctx* c, const BlockGeom& g, bool sorted, int width, std::vector<int32_t>& trow,
                 std::vector<int32_t>& ttile, std::vector<int32_t>& tcol, std::vector<int32_t>& tmax, Plan& plan)
 {
     const std::vector<int32_t>& L = c->h_len;
     trow.clear(); ttile.clear(); tcol.clear(); tmax.clear();
     std::vector<int32_t> rows, cols;
-    order_block(c, g, sorted, rows, cols);
+    swg_ctx::ColTiles& ct = c->col_tiles;
+    const bool cached = sorted && !g.diag_lower && ct.c0 == g.c0 && ct.c1 == g.c1 && ct.width == width;
+    if (cached) {
+        // rows only: the columns' sorted tiles are those of the last block
+        BlockGeom gr = g;
+        gr.c0 = 0; gr.c1 = -1;
+        order_block(c, gr, sorted, rows, cols);
+    } else {
+        order_block(c, g, sorted, rows, cols);
+    }
     auto emit_tile = [&](const int32_t* ids, int n) -> int32_t {
         const int32_t t = (int32_t)tmax.size();
         int32_t mx = 0;
@@ -405,16 +423,26 @@ void plan_tiles(const swg_ctx* c, const BlockGeom& g, bool sorted, int width, st
     plan.cells = 0.0;
     plan.cells_run = 0.0;
     if (!g.diag_lower) {
-        std::vector<int32_t> tiles;
         double sum_cols = 0.0;
-        for (size_t k = 0; k < cols.size(); k += width)
-            tiles.push_back(emit_tile(&cols[k], (int)std::min<size_t>(width, cols.size() - k)));
-        for (int32_t x : cols) sum_cols += L[x];
+        if (cached) {
+            tcol = ct.tcol;
+            tmax = ct.tmax;
+            sum_cols = ct.sum_cols;
+        } else {
+            for (size_t k = 0; k < cols.size(); k += width)
+                emit_tile(&cols[k], (int)std::min<size_t>(width, cols.size() - k));
+            for (int32_t x : cols) sum_cols += L[x];
+            if (sorted) {
+                ct.c0 = g.c0; ct.c1 = g.c1; ct.width = width;
+                ct.tcol = tcol; ct.tmax = tmax; ct.sum_cols = sum_cols;
+            }
+        }
+        const int32_t ntiles = (int32_t)tmax.size();
         for (int32_t a : rows) {
-            for (int32_t t : tiles) { trow.push_back(a); ttile.push_back(t); }
+            for (int32_t t = 0; t < ntiles; ++t) { trow.push_back(a); ttile.push_back(t); }
             plan.cells += (double)L[a] * sum_cols;
         }
-        plan.pairs = (int64_t)rows.size() * (int64_t)cols.size();
+        plan.pairs = (int64_t)rows.size() * (int64_t)(g.c1 - g.c0 + 1);
     } else {
         std::vector<int32_t> valid;
         valid.reserve(cols.size());
@@ -974,6 +1002,7 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     SWG_HIP(hipStreamSynchronize(c->main.stream));
     for (auto& B : c->main.bufs) B.pending = false;
     c->tuned.clear();
+    c->col_tiles = swg_ctx::ColTiles();
     // residue codes: distinct symbols of the set (and of the matrix), in ASCII order, + pad
     bool seen[256] = {false};
     int32_t maxlen = slot_cap;
