@@ -884,15 +884,16 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                     cb[q] = bnd[(int64_t)(jj + 4) * kWave + lane];
                     __builtin_amdgcn_sched_barrier(kSchedAluOnly);
                 }
-                // after the last 4-column step of a 16-column block (or of the
+                // after the last 4-column step of a kFirstMaxBlock-column block (or of the
                 // row): the rows' block maxima, first row first, then the
                 // pair's first maximum.  A mask per half is the sign of a
                 // difference (no wrap: values >= -1, block maxima >= 0, rows
                 // in [0, 32767]), spread to 16 bits; a bitfield insert
                 // selects with it.
-                static_assert(kFirstMaxBlock == 16, "a block is four 4-column steps");
-                if (((j - 1) & 15) == 12 || j + 4 > ncols) {
-                    const uint32_t JB = splat16(((j - 1) & ~15) + 1);
+                static_assert(kFirstMaxBlock % 4 == 0 && (kFirstMaxBlock & (kFirstMaxBlock - 1)) == 0,
+                              "a block is a power-of-two number of 4-column steps");
+                if (((j - 1) & (kFirstMaxBlock - 1)) == kFirstMaxBlock - 4 || j + 4 > ncols) {
+                    const uint32_t JB = splat16(((j - 1) & ~(kFirstMaxBlock - 1)) + 1);
                     uint32_t BV = bm[0], BR = 0u;
                     bm[0] = NEG2;
 #pragma unroll
@@ -963,11 +964,12 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 // scoring at most -gap_open), so they never take the first maximum; a
 // pre-column cell is exactly DP column 0 (V = 0, t = -o, H' = 0).
 //
-// The first maximum is folded per lane once every 16 steps: lane k's block
-// is then the columns s - k + 1 .. s - k + 16 of the steps s .. s + 15 (the
-// first column clamped to 1; the reverse pass and the re-run take a block at
-// any column), and at the end the 16 lanes' first maxima are merged: the
-// larger value, then the earlier row.
+// The first maximum is folded per lane once every kFirstMaxBlock steps: lane
+// k's block is then the columns s - k + 1 .. s - k + kFirstMaxBlock of the
+// steps s .. s + kFirstMaxBlock - 1 (the first column clamped to 1; the
+// reverse pass and the re-run take a block at any column), and at the end
+// the 16 lanes' first maxima are merged: the larger value, then the earlier
+// row.
 constexpr int kLpsGroups = 4;  // 16-lane groups per wave
 
 template <int R>
@@ -1121,8 +1123,8 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
                     if (k == kLpp - 1 && tt >= kLpp - 1)
                         bnd[(int64_t)(tt - (kLpp - 2)) * kLpsGroups + g] = make_int2((int)ot, (int)oG);
                 }
-                if (((t + 3) & 15) == 15 || t + 4 >= nsteps)
-                    lps_fold<R>(bm, GV, GR, GJ, row1, splat16(max(1, ((t + 3) & ~15) - k + 1)));
+                if (((t + 3) & (kFirstMaxBlock - 1)) == kFirstMaxBlock - 1 || t + 4 >= nsteps)
+                    lps_fold<R>(bm, GV, GR, GJ, row1, splat16(max(1, ((t + 3) & ~(kFirstMaxBlock - 1)) - k + 1)));
             }
         }
 

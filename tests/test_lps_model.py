@@ -21,7 +21,7 @@ from oracle import oracle as O
 from tests.test_oracle import random_related
 
 LANES = 16
-BLOCK = 16
+BLOCK = 16  # kFirstMaxBlock: columns per first-maximum block, steps per fold
 PAD16 = -16384
 
 
@@ -100,8 +100,8 @@ def lps_model(s1, s2, M, o, e, R, extra=b""):
                 if k == LANES - 1 and tt >= LANES - 1:
                     new_bnd_t[tt - (LANES - 2)] = ot[k]
                     new_bnd_G[tt - (LANES - 2)] = oG[k]
-            if (tt & 15) == 15 or tt + 1 >= nsteps:
-                start = tt & ~15
+            if (tt & (BLOCK - 1)) == BLOCK - 1 or tt + 1 >= nsteps:
+                start = tt & ~(BLOCK - 1)
                 for k in range(LANES):
                     jb = max(1, start - k + 1)
                     BV, BR = bm[k][0], 0

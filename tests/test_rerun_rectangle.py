@@ -20,8 +20,9 @@ import pytest
 from oracle import oracle as O
 
 # columns per first-maximum block of the score pass (16); the rectangle and
-# band rules hold for any block width, 4 is checked as well
-BLOCKS = (4, 16)
+# band rules hold for any block width, 4 and 32 are checked as well
+BLOCKS = (4, 16, 32)
+KBLOCK = 16
 
 
 def reverse_start(s1, s2, M, o, e, bi, jb, best, BLOCK):
@@ -208,7 +209,7 @@ def _long_pairs(rng, n, lo, hi):
     return out
 
 
-def _rect(s1, s2, M, o, e, BLOCK=16):
+def _rect(s1, s2, M, o, e, BLOCK=KBLOCK):
     full = O.sw_literal(s1, s2, M, o, e)
     best = int(full["score"])
     bi, bj = full["best_row"], full["best_col"]
@@ -225,8 +226,8 @@ def test_band_prunes_and_keeps_rectangle(ednafull):
         best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
         if best == 0:
             continue
-        si, sj, _ = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, 16)
-        assert reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 16, 16, stats=stats) == (si, sj)
+        si, sj, _ = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, KBLOCK)
+        assert reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 16, KBLOCK, stats=stats) == (si, sj)
     assert stats["evaluated"] < 0.8 * stats["prefix"], stats
 
 
@@ -239,8 +240,8 @@ def test_band_threshold_is_not_slack(ednafull):
         best, bi, jb = _rect(s1, s2, ednafull, 16, 4)
         if best == 0:
             continue
-        ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, 16)[:2]
-        lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, 16, live_min=4) != ref
+        ref = reverse_start(s1, s2, ednafull, 16, 4, bi, jb, best, KBLOCK)[:2]
+        lost += reverse_start_banded(s1, s2, ednafull, 16, 4, bi, jb, best, 2, KBLOCK, live_min=4) != ref
     assert lost > 0
 
 
@@ -257,7 +258,7 @@ def test_prefix_rectangle_is_exact(ednafull, o, e):
         best, bi, jb = _rect(s1, s2, ednafull, o, e)
         if len(s1) == 0 or len(s2) == 0:
             continue
-        wend = min(jb + 15, len(s2))
+        wend = min(jb + KBLOCK - 1, len(s2))
         sub = O.sw_literal(s1[:bi], s2[:wend], ednafull, o, e)
         assert int(sub["score"]) == best
         assert (sub["best_row"], sub["best_col"]) == (full["best_row"], full["best_col"])

@@ -17,7 +17,7 @@ import pytest
 from oracle import oracle as O
 from tests.test_oracle import random_related
 
-BLOCK = 16
+BLOCK = 16  # kFirstMaxBlock
 NEG = -(1 << 20)
 
 
