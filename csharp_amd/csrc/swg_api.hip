@@ -174,7 +174,8 @@ struct VariantCfg {
     int v = kVarI32;
     int s1 = 16;
     int s2 = 0;
-    bool operator==(const VariantCfg& o) const { return v == o.v && s1 == o.s1 && s2 == o.s2; }
+    int lanes = 16;  // kVarLps: lanes per pair (16 or 32)
+    bool operator==(const VariantCfg& o) const { return v == o.v && s1 == o.s1 && s2 == o.s2 && lanes == o.lanes; }
 };
 
 // Autotuner record of one key (see choose_variant): the configurations
@@ -268,6 +269,7 @@ struct swg_ctx {
     int32_t score_strip = 0;  // score pass strip; 0 = autotuned
     int32_t reverse_strip = 16;  // reverse pass strip (SWG_OPT_REVERSE_STRIP_ROWS)
     int32_t reverse_row_cap = 512;  // reverse rows per pair at most (SWG_OPT_REVERSE_ROW_CAP)
+    int32_t lps_lanes = 0;          // lanes per pair of the lanes-per-pair score pass; 0 = by block (SWG_OPT_LPS_LANES)
     double tune_min_cells = 5e10;  // smaller blocks run the key's current candidate untimed
     // autotuner cache (entries are never removed while the deque lives)
     std::deque<TuneEntry> tuned;
@@ -319,9 +321,9 @@ VariantCfg default_cfg(const swg_ctx* c, int v)
     return f;
 }
 
-int pairs_per_wave(int v)
+int pairs_per_wave(const VariantCfg& f)
 {
-    return v == kVarI16 ? kScoreTile : v == kVarLpp ? kLppTile : v == kVarLps ? kLpsTile : kWave;
+    return f.v == kVarI16 ? kScoreTile : f.v == kVarLpp ? kLppTile : f.v == kVarLps ? lps_tile(f.lanes) : kWave;
 }
 
 // Resident workgroups of the configuration's kernels on the lane's share of
@@ -334,7 +336,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
     int a = c->nw        ? nw_blocks_per_cu(s1, nw_region)
           : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
-          : v == kVarLps ? lps16_blocks_per_cu(s1)
+          : v == kVarLps ? lps16_blocks_per_cu(s1, f.lanes)
                          : sw_blocks_per_cu(s1, false);
     int b = two_pass(v) ? rerun_blocks_per_cu(s2, c->ncode) : 0;
     int r = two_pass(v) ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
@@ -528,7 +530,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
     const hipStream_t st = ln.stream;
     BlockDesc desc[2] = {};
     if (two_pass(v)) {
-        plan_tiles(c, g, true, v == kVarLps ? kLpsTile : kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col,
+        plan_tiles(c, g, true, v == kVarLps ? lps_tile(f.lanes) : kScoreTile, B.h_task_row, B.h_task_tile, B.h_tile_col,
                    B.h_tile_maxlen, plan);
         if ((rc = upload(st, B.s_task_row, B.h_task_row)) || (rc = upload(st, B.s_task_tile, B.h_task_tile)) ||
             (rc = upload(st, B.s_tile_col, B.h_tile_col)) || (rc = upload(st, B.s_tile_maxlen, B.h_tile_maxlen)))
@@ -667,7 +669,8 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         } else {
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
-            if (e == hipSuccess) e = v == kVarLps ? launch_lps16(S, ST, W, s1, n1, st) : launch_score16(S, ST, W, s1, n1, st);
+            if (e == hipSuccess)
+                e = v == kVarLps ? launch_lps16(S, ST, W, s1, f.lanes, n1, st) : launch_score16(S, ST, W, s1, n1, st);
             if (e == hipSuccess) e = after(v == kVarLps ? SWG_KERNEL_LPS16 : SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
@@ -711,7 +714,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, s1); append_key(key, s2); append_key(key, n1); append_key(key, n2);
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
-        append_key(key, c->reverse_strip); append_key(key, nw_region);
+        append_key(key, c->reverse_strip); append_key(key, nw_region); append_key(key, f.lanes);
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -883,6 +886,21 @@ bool thin_block(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
     return waves < 2.0 * n1;
 }
 
+// The lanes-per-pair score pass for block g: 32 lanes per pair when its
+// 16-lane waves would not fill the lane's resident slots (a lone query: each
+// wave then runs half as many stripes), else 16 (less stripe padding).
+VariantCfg lps_cfg(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
+{
+    VariantCfg f = default_cfg(c, kVarLps);
+    const double rows = g.r1 - g.r0 + 1, cols = g.c1 - g.c0 + 1;
+    const double tile = lps_tile(16);
+    const double waves = g.diag_lower ? rows * std::ceil(cols / 2.0 / tile) : rows * std::ceil(cols / tile);
+    const int slots = std::max(1, lps16_blocks_per_cu(f.s1, 16) * c->num_cu / ln.share);
+    if (waves < slots) f.lanes = 32;
+    if (c->lps_lanes) f.lanes = c->lps_lanes;
+    return f;
+}
+
 // The configuration of a block: fixed by the schedule, the 16-lane kernel for
 // a thin block, or the autotuner's (*entry: its record; *trial: the
 // configuration is a candidate still to be measured).
@@ -896,12 +914,12 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* c
         return fail(SWG_E_UNSUPPORTED, "the global alignment has only the int32 one-pass kernel");
     if (c->schedule == SWG_SCHED_PERSISTENT_LPP16) { *cfg = default_cfg(c, kVarLpp); return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_LPS16) {
-        *cfg = default_cfg(c, c->lps_ok ? kVarLps : kVarLpp);
+        *cfg = c->lps_ok ? lps_cfg(c, ln, g) : default_cfg(c, kVarLpp);
         return SWG_OK;
     }
     if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *cfg = default_cfg(c, kVarI32); return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *cfg = default_cfg(c, kVarI16); return SWG_OK; }
-    if (!c->nw && thin_block(c, ln, g)) { *cfg = default_cfg(c, c->lps_ok ? kVarLps : kVarLpp); return SWG_OK; }
+    if (!c->nw && thin_block(c, ln, g)) { *cfg = c->lps_ok ? lps_cfg(c, ln, g) : default_cfg(c, kVarLpp); return SWG_OK; }
     const int64_t key = tune_key(c, g);
     TuneEntry* t = nullptr;
     for (auto& e : c->tuned)
@@ -1591,6 +1609,12 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         c->reverse_strip = value == 0 ? 16 : (int32_t)value;
         return SWG_OK;
     }
+    if (option == SWG_OPT_LPS_LANES) {
+        if (value != 0 && value != 16 && value != 32)
+            return fail(SWG_E_ARGUMENT, "lanes per pair of the lanes-per-pair score pass: 0 (by block), 16 or 32");
+        c->lps_lanes = (int32_t)value;
+        return SWG_OK;
+    }
     if (option == SWG_OPT_REVERSE_ROW_CAP) {
         if (value < 16 || value > 32768 || value % 16)
             return fail(SWG_E_ARGUMENT, "the reverse-pass row cap is a multiple of 16 in [16, 32768]");
@@ -1616,7 +1640,7 @@ int swg_last_block_config(swg_ctx* c, swg_block_config* out)
                                     : SWG_SCHED_PERSISTENT_I16;
     out->strip_rows = f.s1;
     out->rerun_strip_rows = two_pass(f.v) ? f.s2 : 0;
-    out->pairs_per_wave = pairs_per_wave(f.v);
+    out->pairs_per_wave = pairs_per_wave(f);
     out->tuning = c->last_tuning;
     out->trials = c->last_trials;
     return SWG_OK;

@@ -192,16 +192,17 @@ hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 // Score-only int16x2 pass; strip_rows in {16, 24, 32}.
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
                           int nslots, hipStream_t stream);
-// Score-only int16x2 pass with sixteen lanes per pair (thin blocks): tiles
-// of kLpsTile column sequences (lane group g: tile_col[tile*8 + g] in the low
-// half, tile_col[tile*8 + 4 + g] in the high half), the same best_cell /
-// best_score output; strip_rows (rows per lane) in {8, 16}; column
-// sequences coded < 8 with scores plus gap_open in [0, 255], gap_open >= 1
-// and a pad code < 8 or >= 12 (the host checks this: swg_ctx::lps_ok).
-constexpr int kLpsTile = 8;
-hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
-                        hipStream_t stream);
-int lps16_blocks_per_cu(int strip_rows);
+// Score-only int16x2 pass with `lanes` = 16 or 32 lanes per pair (thin
+// blocks): tiles of lps_tile(lanes) = 2 * 64 / lanes column sequences (lane
+// group g: tile_col[tile*T + g] in the low half, tile_col[tile*T + T/2 + g]
+// in the high half), the same best_cell / best_score output; strip_rows
+// (rows per lane) 8; column sequences coded < 8 with scores plus gap_open in
+// [0, 255], gap_open >= 1 and a pad code < 8 or >= 12 (the host checks this:
+// swg_ctx::lps_ok).
+constexpr int lps_tile(int lanes) { return 2 * kWave / lanes; }
+hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int lanes,
+                        int nslots, hipStream_t stream);
+int lps16_blocks_per_cu(int strip_rows, int lanes);
 // Sort of the score pass's first-maximum cells (count, scan, scatter).
 hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
 // Re-run of the sorted pairs; strip_rows in {16, 32}; track = the flagged list.

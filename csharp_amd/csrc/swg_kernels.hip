@@ -939,17 +939,21 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 }
 
 // ---------------------------------------------------------------------------
-// Score pass with sixteen lanes per pair, for blocks with few row sequences
-// (serving batches, small blocks): the score16_kernel recurrence, saturation
-// rule and first-maximum output, laid out like lpp16_kernel.  A wave aligns
-// s1 (uniform) against kLpsTile sequences s2: one 16-lane DPP row per group,
-// two sequences per group in the int16 halves (pair A low, pair B high).
-// Lane k of a group owns R consecutive rows of a stripe of 16R rows and runs
-// one column behind lane k-1: at step t it computes column j = t - k + 1 of
-// its rows, taking the row above its first row (V - o and G of lane k-1's
-// last row, computed at step t - 1) and the selector of column j's codes
-// with DPP row shifts, while lane 0 reads the previous stripe's bottom row
-// from the boundary buffer and lane 15 writes this stripe's.
+// Score pass with G = 16 or 32 lanes per pair, for blocks with few row
+// sequences (serving batches, small blocks): the score16_kernel recurrence,
+// saturation rule and first-maximum output, laid out like lpp16_kernel.  A
+// wave aligns s1 (uniform) against 2 * 64 / G sequences s2: one G-lane group
+// per one (G = 16) or two (G = 32) DPP rows, two sequences per group in the
+// int16 halves (pair A low, pair B high).  Lane k of a group owns R
+// consecutive rows of a stripe of G * R rows and runs one column behind lane
+// k-1: at step t it computes column j = t - k + 1 of its rows, taking the row
+// above its first row (V - o and G of lane k-1's last row, computed at step
+// t - 1) and the selector of column j's codes with a DPP shift by one lane
+// (row_shr for G = 16; wave_shr for G = 32, the group's first lanes then
+// taking their own values with a select), while lane 0 reads the previous
+// stripe's bottom row from the boundary buffer and lane G - 1 writes this
+// stripe's.  G = 32 halves a wave's stripes, for a query alone on the GPU
+// (its waves then finish sooner, at the cost of taller stripe padding).
 //
 // Scores come from registers: per row, the (score + gap_open) of column codes
 // 0..7 as the bytes of two words (the column sequences' codes are < 8 and
@@ -968,9 +972,18 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 // k's block is then the columns s - k + 1 .. s - k + kFirstMaxBlock of the
 // steps s .. s + kFirstMaxBlock - 1 (the first column clamped to 1; the
 // reverse pass and the re-run take a block at any column), and at the end
-// the 16 lanes' first maxima are merged: the larger value, then the earlier
+// the G lanes' first maxima are merged: the larger value, then the earlier
 // row.
-constexpr int kLpsGroups = 4;  // 16-lane groups per wave
+
+// The value lane k-1 of the lane's LN-lane group holds; `old` in the
+// group's first lane.
+template <int LN>
+__device__ __forceinline__ uint32_t group_shr1(uint32_t old, uint32_t src, bool first)
+{
+    if (LN == 16) return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, 0x111, 0xF, 0xF, false);
+    const uint32_t v = (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, 0x138, 0xF, 0xF, false);  // wave_shr:1
+    return first ? old : v;
+}
 
 template <int R>
 __device__ __forceinline__ void lps_fold(uint32_t (&bm)[R], uint32_t& GV, uint32_t& GR, uint32_t& GJ, uint32_t row1,
@@ -1000,15 +1013,19 @@ __device__ __forceinline__ void lps_fold(uint32_t (&bm)[R], uint32_t& GV, uint32
     GV = u2(__builtin_elementwise_max(v2(GV), v2(BV)));
 }
 
-template <int R>
+template <int R, int LN>
 __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 {
+    static_assert(LN == 16 || LN == 32, "16 or 32 lanes per pair");
+    constexpr int NG = kWave / LN;  // lane groups per wave
+    constexpr int TILE = 2 * NG;   // column sequences per task
     // [c1][c2], 64 x 8: score + gap_open, 0 for the pad code (and codes past it)
     __shared__ uint32_t tab[kMaxCodes * kJointMaxCodes / 4];
     __shared__ uint32_t mrg[3 * kWave];  // per lane: first maximum (value, row, block)
 
     const int lane = threadIdx.x;
-    const int g = lane >> 4, k = lane & 15;
+    const int g = lane / LN, k = lane % LN;
+    const bool first = k == 0;
     const int ncode = S.ncode, pad = ncode - 1;
     for (int x = lane; x < kMaxCodes * kJointMaxCodes; x += kWave) {
         const int c1 = x >> 3, c2 = x & 7;
@@ -1035,8 +1052,8 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
         const int a = uni(T.task_row[t_task]);
         const int tile = uni(T.task_tile[t_task]);
         const int maxLb = uni(T.tile_maxlen[tile]);
-        const int bA = T.tile_col[(int64_t)tile * kLpsTile + g];
-        const int bB = T.tile_col[(int64_t)tile * kLpsTile + kLpsGroups + g];
+        const int bA = T.tile_col[(int64_t)tile * TILE + g];
+        const int bB = T.tile_col[(int64_t)tile * TILE + NG + g];
         const int La = uni(S.len[a]);
         const int LbA = bA >= 0 ? S.len[bA] : 0;
         const int LbB = bB >= 0 ? S.len[bB] : 0;
@@ -1046,16 +1063,16 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
         // word (L + 3) / 4 holds only pad codes (and the words before it pad
         // codes past L): word min(k, last) gives the pad code past the end
         const int lastA = (LbA + 3) >> 2, lastB = (LbB + 3) >> 2;
-        const int nsteps = (maxLb + (kLpp - 1) + 3) & ~3;  // lane 15 reaches column maxLb
+        const int nsteps = (maxLb + (LN - 1) + 3) & ~3;  // lane LN - 1 reaches column maxLb
 
         // row 0 of the DP (t = -o, G = -inf), also past the last column so
         // that lane 0's look-ahead reads defined values
         for (int j = 1 + lane; j <= nsteps + 8; j += kWave)
-            for (int q = 0; q < kLpsGroups; ++q) bnd[(int64_t)j * kLpsGroups + q] = make_int2((int)T0, (int)NEG2);
+            for (int q = 0; q < NG; ++q) bnd[(int64_t)j * NG + q] = make_int2((int)T0, (int)NEG2);
         __syncthreads();  // other lanes read these columns
 
         uint32_t GV = M1, GR = 0u, GJ = 0u;
-        for (int i0 = 0; i0 < La; i0 += kLpp * R) {
+        for (int i0 = 0; i0 < La; i0 += LN * R) {
             // this lane's rows: score bytes of codes 0..7 (pad rows: 0)
             uint32_t tlo[R], thi[R];
 #pragma unroll
@@ -1074,16 +1091,15 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
             uint32_t sel = padsel;        // this lane's column codes as a byte selector
 
             uint32_t nwa = wA[0], nwb = wB[0];
-            int2 nb[4] = {bnd[1 * kLpsGroups + g], bnd[2 * kLpsGroups + g], bnd[3 * kLpsGroups + g],
-                          bnd[4 * kLpsGroups + g]};
+            int2 nb[4] = {bnd[1 * NG + g], bnd[2 * NG + g], bnd[3 * NG + g], bnd[4 * NG + g]};
             for (int t = 0; t < nsteps; t += 4) {
                 const uint32_t cwa = nwa, cwb = nwb;
                 const int2 cb[4] = {nb[0], nb[1], nb[2], nb[3]};
                 nwa = wA[min((t >> 2) + 1, lastA)];
                 nwb = wB[min((t >> 2) + 1, lastB)];
-                const int64_t nxt = (int64_t)(t + 5) * kLpsGroups + g;
+                const int64_t nxt = (int64_t)(t + 5) * NG + g;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) nb[q] = bnd[nxt + q * kLpsGroups];
+                for (int q = 0; q < 4; ++q) nb[q] = bnd[nxt + q * NG];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int tt = t + q;
@@ -1093,9 +1109,9 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
                         __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2,
                             __builtin_amdgcn_perm(cwb, cwa, 0x0C040C00u + (uint32_t)q * 0x00010001u)), C12)) |
                         0x0C000C00u;
-                    sel = (uint32_t)row_shr1((int)s0, (int)sel);
-                    uint32_t tu = (uint32_t)row_shr1(cb[q].x, (int)ot);
-                    uint32_t Gu = (uint32_t)row_shr1(cb[q].y, (int)oG);
+                    sel = group_shr1<LN>(s0, sel, first);
+                    uint32_t tu = group_shr1<LN>((uint32_t)cb[q].x, ot, first);
+                    uint32_t Gu = group_shr1<LN>((uint32_t)cb[q].y, oG, first);
                     uint32_t dt = pt;
                     pt = tu;
 #pragma unroll
@@ -1119,23 +1135,22 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
                     }
                     ot = tu;
                     oG = Gu;
-                    // lane 15: this stripe's last row at its column tt - 14
-                    if (k == kLpp - 1 && tt >= kLpp - 1)
-                        bnd[(int64_t)(tt - (kLpp - 2)) * kLpsGroups + g] = make_int2((int)ot, (int)oG);
+                    // lane LN - 1: this stripe's last row at its column tt - (LN - 2)
+                    if (k == LN - 1 && tt >= LN - 1) bnd[(int64_t)(tt - (LN - 2)) * NG + g] = make_int2((int)ot, (int)oG);
                 }
                 if (((t + 3) & (kFirstMaxBlock - 1)) == kFirstMaxBlock - 1 || t + 4 >= nsteps)
                     lps_fold<R>(bm, GV, GR, GJ, row1, splat16(max(1, ((t + 3) & ~(kFirstMaxBlock - 1)) - k + 1)));
             }
         }
 
-        // merge the group's 16 lanes: the larger value, then the earlier row
+        // merge the group's LN lanes: the larger value, then the earlier row
         mrg[lane] = GV;
         mrg[kWave + lane] = GR;
         mrg[2 * kWave + lane] = GJ;
         __syncthreads();
         if (k == 0) {
             int best[2] = {-1, -1}, brow[2] = {0, 0}, bcol[2] = {0, 0};
-            for (int x = 0; x < kLpp; ++x) {
+            for (int x = 0; x < LN; ++x) {
                 const uint32_t xv = mrg[lane + x], xr = mrg[kWave + lane + x], xj = mrg[2 * kWave + lane + x];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
@@ -2341,26 +2356,26 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
     return hipGetLastError();
 }
 
-hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
-                        hipStream_t st)
+hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int lanes,
+                        int nslots, hipStream_t st)
 {
-    switch (strip_rows) {
-    case 8: hipLaunchKernelGGL((lps16_kernel<8>), dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
-    case 16: hipLaunchKernelGGL((lps16_kernel<16>), dim3(nslots), dim3(kWave), 0, st, S, T, W); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (strip_rows == 8 && lanes == 16)
+        hipLaunchKernelGGL((lps16_kernel<8, 16>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else if (strip_rows == 8 && lanes == 32)
+        hipLaunchKernelGGL((lps16_kernel<8, 32>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-int lps16_blocks_per_cu(int strip_rows)
+int lps16_blocks_per_cu(int strip_rows, int lanes)
 {
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
-    switch (strip_rows) {
-    case 8: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8>, kWave, 0); break;
-    case 16: err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<16>, kWave, 0); break;
-    default: break;
-    }
+    if (strip_rows == 8 && lanes == 16)
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 16>, kWave, 0);
+    else if (strip_rows == 8 && lanes == 32)
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 32>, kWave, 0);
     return err == hipSuccess ? nb : 0;
 }
 

@@ -217,6 +217,11 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * band of possible alignment starts reaches past it is re-run over its whole
  * prefix (exact either way; bounds the pass's longest tasks). */
 #define SWG_OPT_REVERSE_ROW_CAP 6
+/* SWG_OPT_LPS_LANES = lanes per pair of the int16 lanes-per-pair score pass
+ * (SWG_SCHED_PERSISTENT_LPS16): 0 (default) chooses per block -- 32 when the
+ * block's 16-lane waves would not fill the GPU (a lone query), else 16 --;
+ * 16 or 32 fix it. */
+#define SWG_OPT_LPS_LANES 7
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the

@@ -1,7 +1,8 @@
-"""CPU model of the sixteen-lanes-per-pair int16 score pass (lps16_kernel),
+"""CPU model of the lanes-per-pair int16 score pass (lps16_kernel, 16 or 32
+lanes per pair),
 checked against the oracle's literal aligner (SmithWatermanGotoh.cs:119-219).
 
-The model follows the kernel's data flow step by step: 16 lanes of R rows per
+The model follows the kernel's data flow step by step: 16 or 32 lanes of R rows per
 stripe, lane k computing column t - k + 1 at step t with the row above and
 the column's codes taken from lane k-1's previous step (lane 0 from the
 boundary row of the previous stripe), pre-columns and pad columns / rows
@@ -20,7 +21,6 @@ import pytest
 from oracle import oracle as O
 from tests.test_oracle import random_related
 
-LANES = 16
 BLOCK = 16  # kFirstMaxBlock: columns per first-maximum block, steps per fold
 PAD16 = -16384
 
@@ -29,7 +29,7 @@ def sat16(x):
     return max(-32768, min(32767, x))
 
 
-def lps_model(s1, s2, M, o, e, R, extra=b""):
+def lps_model(s1, s2, M, o, e, R, extra=b"", LANES=16):
     """(best, row, jb) of pair (s1, s2) as lps16_kernel<R> computes it, for
     one half (the other half of a lane runs the same recurrence on its own
     sequence).  Codes as the library assigns them: the column symbols
@@ -137,10 +137,11 @@ def cases(ednafull):
     return out
 
 
-@pytest.mark.parametrize("R,extra", [(1, b""), (2, b""), (8, b""), (2, b"BDHKMRSUVWY")])
-def test_lps_model_matches_the_oracle(ednafull, R, extra):
+@pytest.mark.parametrize("R,extra,lanes", [(1, b"", 16), (2, b"", 16), (8, b"", 16), (2, b"BDHKMRSUVWY", 16),
+                                           (1, b"", 32), (8, b"", 32)])
+def test_lps_model_matches_the_oracle(ednafull, R, extra, lanes):
     for a, b, o, e in cases(ednafull):
-        best, bi, jb = lps_model(a, b, ednafull, o, e, R, extra)
+        best, bi, jb = lps_model(a, b, ednafull, o, e, R, extra, LANES=lanes)
         r = O.sw_literal(a, b, ednafull, o, e)
         assert best == int(r["score"]), (a, b, o, e)
         assert bi == r["best_row"], (a, b, o, e, bi, r["best_row"])
