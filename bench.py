@@ -93,9 +93,10 @@ def main():
                     help="c3full: seeded pairs per block checked against the oracle")
     ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "lps16", "plain"],
                     help="force one kernel variant for the timed steps (auto = autotuned)")
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2nw", "c3", "c3full", "c4"],
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2nw", "c3", "c3full", "c4", "c5thin"],
                     help="c3 = the headline block benchmark; c1/c2/c4 run that workload end to end with its parity "
-                         "check; c3full = every block of the 50,000-sequence matrix at P = --world")
+                         "check; c3full = every block of the 50,000-sequence matrix at P = --world; c5thin = one "
+                         "and four serving queries against the panel as thin blocks, with per-kernel times")
     ap.add_argument("--distance", default="pid", choices=["pid", "kimura2", "jc"],
                     help="c3 distance function: PercentIdentity (the headline), Kimura2 or JukesCantor")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) measurement")
@@ -137,6 +138,9 @@ def main():
         return
     if args.workload == "c3full":
         print(json.dumps(full_c3_run(args)), flush=True)
+        return
+    if args.workload == "c5thin":
+        print(json.dumps(thin_blocks(args)), flush=True)
         return
     if args.workload != "c3":
         print(json.dumps(workload_run(args)), flush=True)
@@ -412,15 +416,22 @@ def workload_run(args):
     al = ca.Aligner(M, o, e, alignment_type=cfg["alignment_type"], tdistance=cfg["tdistance"],
                     mode=ca.MODE_NW_GLOBAL if glob else ca.MODE_SW_LOCAL)
     al.set_sequences(seqs)
+    if args.schedule != "auto":  # a forced schedule, for comparison with the autotuned one
+        al.set_schedule({"i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
+                         "lpp16": ca.SCHED_PERSISTENT_LPP16, "lps16": ca.SCHED_PERSISTENT_LPS16,
+                         "plain": ca.SCHED_PLAIN}[args.schedule], args.strip)
     rng = np.random.default_rng(7)
     out = {"workload": name, "sequences": n}
     if name in ("c1", "c2", "c2nw"):
+        al.set_kernel_timing(True)  # a stamp kernel around each of the block's kernels
         t = time.perf_counter()
         full, pairs = al.compute_full_matrix(world=1)
         wall = time.perf_counter() - t
         cells = float((np.cumsum(L)[:-1] * L[1:]).sum())
         out.update(pairs=pairs, cells=cells, seconds=round(wall, 3), gcups=round(cells / wall / 1e9, 1),
-                   sha256=hashlib.sha256(full.tobytes()).hexdigest())
+                   sha256=hashlib.sha256(full.tobytes()).hexdigest(), launch_config=al.last_block_config())
+        out["kernels"] = [{"kernel": k[0], "ms": round(k[1], 3)} for k in al.kernel_times()]
+        al.set_kernel_timing(False)
         if name == "c1":
             a, b = np.tril_indices(n, -1)  # every pair, s1 = the larger index (diagonal block)
         else:
@@ -488,6 +499,47 @@ def workload_run(args):
                    cpu_baseline={"value": round(float((L[A] * L[B]).sum()) / dt / 1e9, 3), "unit": "GCUPS",
                                  "cores": th, "kind": "port", "sample": "%d pairs" % len(A)})
     al.close()
+    return out
+
+
+def thin_blocks(args):
+    """Thin blocks of the serving workload without the server: the first
+    query (395 residues), then the first four, against the 20,000-sequence
+    panel through the host entry point with the library's choice for thin
+    blocks (the lanes-per-pair int16 score pass and the int32 re-run), three
+    times each, the last in kernel-timing mode; bit-exact against the oracle
+    on 2,000 seeded panel pairs per query."""
+    import csharp_amd as ca
+    from csharp_amd import workloads as WL
+    panel = WL.sequences("c5_panel")
+    qset = WL.sequences("c5_queries")
+    M, o, e = WL.scoring("c5_panel")
+    rng = np.random.default_rng(7)
+    out = {"workload": "c5thin", "panel": len(panel), "blocks": []}
+    for nq in (1, 4):
+        qs = [qset.seq(i) for i in range(nq)]
+        s = ca.SequenceSet.from_strings(qs + [panel.seq(i) for i in range(len(panel))])
+        al = ca.Aligner(M, o, e)
+        al.set_sequences(s)
+        n = len(s)
+        for rep_ in range(3):
+            al.set_kernel_timing(rep_ == 2)
+            t = time.perf_counter()
+            got, _, pairs = al.compute_block(0, nq - 1, nq, n - 1, False)
+            dt = time.perf_counter() - t
+        kt = al.kernel_times()
+        ms, cells, _, _ = al.last_kernel_stats()
+        a = np.repeat(np.arange(nq), 2000)
+        b = np.concatenate([rng.choice(np.arange(nq, n), 2000, replace=False) for _ in range(nq)])
+        d, _, _ = oracle_sample(s, a, b, M, o, e, True, False)
+        blk = {"queries": nq, "query_lengths": [len(q) for q in qs], "pairs": int(pairs), "cells": cells,
+               "host_ms": round(dt * 1e3, 2), "kernel_ms": round(ms, 2), "gcups": round(cells / ms / 1e6, 1),
+               "launch_config": al.last_block_config(),
+               "kernels": [{"kernel": k[0], "ms": round(k[1], 3)} for k in kt],
+               "parity": {"checked_pairs": int(len(a)), "mismatches": int((got[a, b - nq] != d).sum())}}
+        log("c5thin %d queries: %s" % (nq, blk))
+        out["blocks"].append(blk)
+        al.close()
     return out
 
 
