@@ -47,6 +47,12 @@ ALG_OPS = {"score16": (9, "int16x2"), "lps16": (9, "int16x2"), "reverse": (9, "i
 EVALUATED_IS_WORK = {"reverse"}
 VALU_PER_CELL = {"score16": 4.757, "reverse": 5.672, "rerun_capture": 21.797, "rerun_track": 25.25, "one_pass": 23.375,
                  "global": 20.906}
+# the score pass by (max3 form, strip rows): loop body plus a quarter of the
+# first-maximum fold (once per 16 columns).  The max3 form spends 7.5 packed
+# instructions on the 9 operations of a cell pair (its three-way maxima are
+# one instruction each), so its roofline fraction at 9 operations per cell
+# can reach 9 / 7.5 = 1.2 of the issue rate.
+VALU_PER_CELL_SCORE16 = {(True, 32): 3.981, (True, 24): 4.007, (False, 32): 4.731, (False, 24): 4.757}
 
 
 def log(msg):
@@ -89,6 +95,8 @@ def main():
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
     ap.add_argument("--score-strip", type=int, default=0, help="int16 score-pass strip rows (0 = autotuned)")
     ap.add_argument("--reverse-strip", type=int, default=0, help="reverse-pass strip rows (0 = the default)")
+    ap.add_argument("--score-max3", type=int, default=1, choices=[0, 1],
+                    help="1: the score pass's max3 form where the set allows it (the default); 0: its plain form")
     ap.add_argument("--full-pairs-per-block", type=int, default=200,
                     help="c3full: seeded pairs per block checked against the oracle")
     ap.add_argument("--schedule", default="auto", choices=["auto", "i32", "i16", "lpp16", "lps16", "plain"],
@@ -174,6 +182,7 @@ def main():
     al.set_schedule(SCHED[args.schedule], args.strip)
     al.set_score_strip(args.score_strip)
     al.set_reverse_strip(args.reverse_strip)
+    al.set_score_max3(bool(args.score_max3))
 
     def run_block(k):
         (r0, r1), (c0, c1) = blocks[k % len(blocks)]
@@ -346,6 +355,7 @@ def main():
             "pairs_per_step": tot_pairs // max(args.steps, 1),
             "cells_per_step": tot_cells / max(args.steps, 1),
             "output": "Int16 distances in pinned host memory through swg_compute_block (D2H included)",
+            "score_pass_form": "max3" if args.score_max3 else "plain",
         },
         "kernel_gcups": round(kernel_gcups, 2),
         "device_entry_gcups": round(device_entry_gcups, 2),
@@ -368,7 +378,11 @@ def main():
             "kernel_ms": round(dms, 2),
             "cells": dcells,
             "cells_evaluated": drun,
-            "valu_instr_per_cell": VALU_PER_CELL.get(dname),
+            "valu_instr_per_cell": (VALU_PER_CELL_SCORE16.get((bool(args.score_max3), cfg.get("strip_rows")))
+                                    if dname == "score16" else VALU_PER_CELL.get(dname)),
+            "note": ("the max3 form of the score pass issues 7.5 packed instructions for the 9 operations of a "
+                     "cell pair (three-way maxima in one instruction): at full VALU issue its fraction is 1.2"
+                     if dname == "score16" and args.score_max3 else None),
         },
         "kernels": [{"kernel": k[0], "ms": round(k[1], 2), "cells": k[2], "cells_evaluated": k[3],
                      "frac": (round((k[3] if k[0] in EVALUATED_IS_WORK else k[2]) * ALG_OPS[k[0]][0] / (k[1] / 1e3)

@@ -37,7 +37,7 @@ PROGRAM_SWMPI, PROGRAM_SWMS = 0, 1
 (SCHED_PERSISTENT, SCHED_PLAIN, SCHED_PERSISTENT_I32, SCHED_PERSISTENT_I16, SCHED_PERSISTENT_LPP16,
  SCHED_PERSISTENT_LPS16) = 0, 1, 2, 3, 4, 5
 (OPT_STREAM_CHUNK_BYTES, OPT_KERNEL_TIMING, OPT_SCORE_STRIP_ROWS, OPT_TUNE_MIN_CELLS, OPT_REVERSE_STRIP_ROWS,
- OPT_REVERSE_ROW_CAP, OPT_LPS_LANES) = 1, 2, 3, 4, 5, 6, 7
+ OPT_REVERSE_ROW_CAP, OPT_LPS_LANES, OPT_SCORE_MAX3) = 1, 2, 3, 4, 5, 6, 7, 8
 KERNEL_NAMES = {1: "one_pass", 2: "score16", 3: "sort", 4: "rerun_track", 5: "rerun_capture", 6: "lpp16",
                 7: "global", 8: "reverse", 9: "rect64", 10: "distance", 11: "lps16"}
 MATRIX_SIZE = 127
@@ -369,6 +369,12 @@ class Aligner:
     def set_lps_lanes(self, lanes):
         """Lanes per pair of the lanes-per-pair score pass: 0 (by block), 16 or 32."""
         _check(lib().swg_set_option(self._h, OPT_LPS_LANES, int(lanes)))
+
+    def set_score_max3(self, on):
+        """The score pass's max3 form (values biased by gap_open, three-way
+        maxima in one packed instruction) where the set allows it: True
+        (default) or False; the results are identical."""
+        _check(lib().swg_set_option(self._h, OPT_SCORE_MAX3, 1 if on else 0))
 
     def set_reverse_row_cap(self, rows):
         """Reverse rows per pair at most (a multiple of 16): pairs whose band

@@ -224,6 +224,7 @@ struct swg_ctx {
     bool bio_helpers = false;       // SWMS / NW ComputeDistance helpers (SWG_PROGRAM_SWMS or global mode)
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
     bool lps_ok = false;            // and the 16-lanes-per-pair score pass (<= 8 codes, score + open in [0, 255])
+    bool mx3_ok = false;            // and the score pass's max3 form (score + open >= 0 for the set's codes)
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
 
     // sequences
@@ -270,6 +271,7 @@ struct swg_ctx {
     int32_t reverse_strip = 16;  // reverse pass strip (SWG_OPT_REVERSE_STRIP_ROWS)
     int32_t reverse_row_cap = 512;  // reverse rows per pair at most (SWG_OPT_REVERSE_ROW_CAP)
     int32_t lps_lanes = 0;          // lanes per pair of the lanes-per-pair score pass; 0 = by block (SWG_OPT_LPS_LANES)
+    int32_t score_max3 = 1;         // the score pass's max3 form where the set allows it (SWG_OPT_SCORE_MAX3)
     double tune_min_cells = 5e10;  // smaller blocks run the key's current candidate untimed
     // autotuner cache (entries are never removed while the deque lives)
     std::deque<TuneEntry> tuned;
@@ -301,6 +303,10 @@ int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 
 
 // The configuration of variant v with its default strips, or those the
 // caller fixed (swg_set_schedule's strip_rows, SWG_OPT_SCORE_STRIP_ROWS).
+// The score pass runs its max3 form where the sequence set allows it,
+// unless SWG_OPT_SCORE_MAX3 turned it off.
+bool use_mx3(const swg_ctx* c) { return c->mx3_ok && c->score_max3 != 0; }
+
 VariantCfg default_cfg(const swg_ctx* c, int v)
 {
     VariantCfg f;
@@ -334,7 +340,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
 {
     const int v = f.v, s1 = f.s1, s2 = f.s2;
     int a = c->nw        ? nw_blocks_per_cu(s1, nw_region)
-          : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode)
+          : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode, use_mx3(c))
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
           : v == kVarLps ? lps16_blocks_per_cu(s1, f.lanes)
                          : sw_blocks_per_cu(s1, false);
@@ -670,7 +676,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess)
-                e = v == kVarLps ? launch_lps16(S, ST, W, s1, f.lanes, n1, st) : launch_score16(S, ST, W, s1, n1, st);
+                e = v == kVarLps ? launch_lps16(S, ST, W, s1, f.lanes, n1, st) : launch_score16(S, ST, W, s1, use_mx3(c), n1, st);
             if (e == hipSuccess) e = after(v == kVarLps ? SWG_KERNEL_LPS16 : SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
@@ -715,6 +721,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
         append_key(key, c->reverse_strip); append_key(key, nw_region); append_key(key, f.lanes);
+        append_key(key, use_mx3(c));
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -1112,6 +1119,12 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     for (int x = 0; x < pad && c->lps_ok; ++x)
         for (int y = 0; y < set_codes; ++y)
             if (prof[(size_t)x * c->ncode + y].x < 0 || prof[(size_t)x * c->ncode + y].x > 255) c->lps_ok = false;
+    // the score pass's max3 form keeps every value biased by gap_open and
+    // >= 0: score + gap_open >= 0 for every pair of codes
+    c->mx3_ok = c->int16_ok && !c->nw;
+    for (int x = 0; x < pad && c->mx3_ok; ++x)
+        for (int y = 0; y < pad; ++y)
+            if (prof[(size_t)x * c->ncode + y].x < 0) c->mx3_ok = false;
 
     // residue codes, each sequence 4-byte aligned and followed by pad codes
     // up to the end of its next word (seq_span), then the slots, then a tail
@@ -1607,6 +1620,11 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         if (value != 0 && value != 8 && value != 16)
             return fail(SWG_E_ARGUMENT, "reverse-pass strip rows are 0 (default 16), 8 or 16");
         c->reverse_strip = value == 0 ? 16 : (int32_t)value;
+        return SWG_OK;
+    }
+    if (option == SWG_OPT_SCORE_MAX3) {
+        if (value != 0 && value != 1) return fail(SWG_E_ARGUMENT, "the score pass's max3 form: 0 (off) or 1 (on)");
+        c->score_max3 = (int32_t)value;
         return SWG_OK;
     }
     if (option == SWG_OPT_LPS_LANES) {

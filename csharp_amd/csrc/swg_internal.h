@@ -189,8 +189,9 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 // PercentIdentity, or statistics requested).
 hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool region,
                      bool persistent, int nslots, hipStream_t stream);
-// Score-only int16x2 pass; strip_rows in {16, 24, 32}.
-hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows,
+// Score-only int16x2 pass; strip_rows in {16, 24, 32}; mx3: the biased
+// max3 form (needs score + gap_open >= 0 for every pair of the set's codes).
+hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, bool mx3,
                           int nslots, hipStream_t stream);
 // Score-only int16x2 pass with `lanes` = 16 or 32 lanes per pair (thin
 // blocks): tiles of lps_tile(lanes) = 2 * 64 / lanes column sequences (lane
@@ -224,7 +225,7 @@ hipError_t launch_lpp16(const SeqSet& S, const TaskSet& T, const Scratch& W, int
 int lpp16_blocks_per_cu(int strip_rows, int ncode);
 // Occupancy: resident workgroups (one wave each) per CU for a variant.
 int sw_blocks_per_cu(int strip_rows, bool nw);
-int score16_blocks_per_cu(int strip_rows, int ncode);
+int score16_blocks_per_cu(int strip_rows, int ncode, bool mx3);
 int nw_blocks_per_cu(int strip_rows, bool region);
 // Distances of a block from the count words in aux (Jukes-Cantor, Kimura2,
 // MinMaxNormScore): desc[0] is the output desc (rows, aux_ld = columns); the

@@ -660,7 +660,32 @@ __device__ __forceinline__ uint32_t splat16(int x) { return ((uint32_t)x & 0xFFF
 constexpr int kPad16 = -16384;
 constexpr int kSat16 = 32766;  // best >= this: the pair may have saturated
 
-template <int R>
+// The max3 form of the score pass (MX3): every value is kept biased by
+// gap_open, so that all of them are >= 0 -- t = V - o + o = V, G + o,
+// H' + o, the diagonal candidate V_diag + score + o (the profile holds
+// score + o, required >= 0 for the set's codes, and 0 for the pad code:
+// a pad score of -o) -- and then, as int16 bit patterns in [0, 0x7BFF], they
+// order exactly as the fp16 numbers they spell (non-negative fp16 patterns
+// below the infinity are ordered like integers; f16 denormals are not
+// flushed in these kernels).  gfx950's v_pk_maximum3_f16 is then an exact
+// integer max of three: V = max3(f, G, H') in one packed operation instead
+// of two, and the row's running maximum takes two columns per operation
+// (max3(bm, t of column j-1, t of column j) every other column).  A value
+// past 0x7BFF is an infinity or a positive NaN pattern; the first such value
+// comes from exact operands, every later operation keeps a result >= 0x7C00
+// - o, and the pair is flagged (best >= kSatMx3(o)) for the int32 re-run:
+// the numbers below the threshold are exact.
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t max3b(uint32_t a, uint32_t b, uint32_t c)
+{
+    const h16x2 r = __builtin_elementwise_maximum(
+        __builtin_elementwise_maximum(__builtin_bit_cast(h16x2, a), __builtin_bit_cast(h16x2, b)),
+        __builtin_bit_cast(h16x2, c));
+    return __builtin_bit_cast(uint32_t, r);
+}
+__device__ __forceinline__ int kSatMx3(int o) { return 0x7C00 - o; }
+
+template <int R, bool MX3>
 __device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, const uint8_t* s1, int La, int i0,
                                                 int lane)
 {
@@ -671,7 +696,8 @@ __device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, co
         const int row = i0 + r;
         const int c1 = row < La ? (int)s1[row] : pad;
         const int v = S.prof[c1 * ncode + c2].x;
-        lp[c2 * (R + 8) + r] = (short)(v < kPad16 ? kPad16 : v);
+        const int lo = MX3 ? 0 : kPad16;  // the pad score (MX3: -gap_open, biased to 0)
+        lp[c2 * (R + 8) + r] = (short)(v < lo ? lo : v);
     }
 }
 
@@ -685,7 +711,7 @@ __device__ __forceinline__ void stage_profile16(int4* lprof, const SeqSet& S, co
 constexpr int kJointMaxCodes = 8;
 constexpr int kJointRow = 64;  // words per row (= kJointMaxCodes^2)
 
-template <int R>
+template <int R, bool MX3>
 __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, const uint8_t* s1, int La, int i0,
                                               int lane)
 {
@@ -696,8 +722,9 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
         const int row = i0 + r;
         const int c1 = row < La ? (int)s1[row] : pad;
         int va = S.prof[c1 * ncode + ca].x, vc = S.prof[c1 * ncode + cc].x;
-        va = va < kPad16 ? kPad16 : va;
-        vc = vc < kPad16 ? kPad16 : vc;
+        const int lo = MX3 ? 0 : kPad16;
+        va = va < lo ? lo : va;
+        vc = vc < lo ? lo : vc;
         jp[r * kJointRow + ca * kJointMaxCodes + cc] = ((uint32_t)va & 0xFFFFu) | ((uint32_t)vc << 16);
     }
 }
@@ -718,7 +745,7 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
 // per row the kernel runs score16_waves(R) waves per SIMD.
 constexpr int score16_waves(int R) { return R <= 16 ? 4 : R <= 24 ? 3 : 2; }
 
-template <int R, bool JOINT>
+template <int R, bool JOINT, bool MX3>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(score16_waves(R), score16_waves(R)))) void
 score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 {
@@ -733,7 +760,10 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
     const int ntasks = D->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
-    const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
+    const uint32_t NEG2 = splat16(kPad16), M1 = splat16(-1);
+    // t = V - o, G and H' at DP row 0 / column 0 (MX3: biased by o)
+    const uint32_t T0 = MX3 ? 0u : splat16(-o), G0 = MX3 ? 0u : NEG2, H0 = MX3 ? splat16(o) : 0u;
+    const int sat = MX3 ? kSatMx3(o) : kSat16;
 
     for (;;) {
         int v = 0;
@@ -758,20 +788,20 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
         const int lastA = (LbA + 3) >> 2, lastB = (LbB + 3) >> 2;
         const int ncols = (maxLb + 3) & ~3;
 
-        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2((int)T0, (int)NEG2);
+        for (int j = 1; j <= ncols; ++j) bnd[(int64_t)j * kWave + lane] = make_int2((int)T0, (int)G0);
 
         // the pair's first maximum: value (-1: none yet), row (1-based) and
         // first column of its block, one pair per 16-bit half
         uint32_t GV = M1, GR = 0u, GJ = 0u;
         for (int i0 = 0; i0 < La; i0 += R) {
             __syncthreads();
-            if (JOINT) stage_joint16<R>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
-            else stage_profile16<R>(lprof, S, s1, La, i0, lane);
+            if (JOINT) stage_joint16<R, MX3>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
+            else stage_profile16<R, MX3>(lprof, S, s1, La, i0, lane);
             __syncthreads();
 
             uint32_t tr[R], Hr[R], bm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; bm[r] = NEG2; }
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = H0; bm[r] = NEG2; }
             const uint32_t row1 = splat16(i0 + 1);  // row of strip row 0
             uint32_t dtb = T0;
 
@@ -858,20 +888,25 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                             // carries the 0 floor of V: an unsigned saturating
                             // subtract gives max(H' - e, 0) = max(H - e, 0), so
                             // H' = max(max(H' - e, 0), t) = max(H, 0) and
-                            // V = max(f, G, H, 0) = max(f, G, H')
-                            const s16x2 H = __builtin_elementwise_max(
-                                __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
-                                    __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))),
-                                v2(tr[r]));
+                            // V = max(f, G, H, 0) = max(f, G, H').  MX3: the
+                            // floor of H' + o is o, the third operand of a max3
+                            const uint32_t hs = u2(__builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                                __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))));
+                            const uint32_t H = MX3 ? max3b(hs, tr[r], u2(O2))
+                                                   : u2(__builtin_elementwise_max(v2(hs), v2(tr[r])));
                             const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp4[k]));
                             // G last: the row-to-row chain (t of the row above ->
                             // G -> V -> t) is three dependent operations
-                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
+                            const s16x2 V = MX3 ? v2(max3b(u2(f), H, u2(G)))
+                                                : __builtin_elementwise_max(__builtin_elementwise_max(f, v2(H)), G);
                             const s16x2 tn = V - O2;
-                            bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
+                            // the row maximum: of V, or (MX3) of t = V, two
+                            // columns at once after every second column
+                            if (!MX3) bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
+                            else if (q & 1) bm[r] = max3b(bm[r], tr[r], u2(tn));
                             dt = tr[r];
                             tr[r] = u2(tn);
-                            Hr[r] = u2(H);
+                            Hr[r] = H;
                             tu = u2(tn);
                             Gu = u2(G);
                         }
@@ -924,13 +959,13 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
         const int bestA = (short)(GV & 0xFFFFu), bestB = (short)(GV >> 16);
         const uint32_t cellA = ((GR & 0xFFFFu) << 16) | (GJ & 0xFFFFu), cellB = (GR & 0xFFFF0000u) | (GJ >> 16);
         if (bA >= 0) {
-            const uint32_t c = (La == 0 || LbA == 0 || bestA >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
+            const uint32_t c = (La == 0 || LbA == 0 || bestA >= sat) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbA) : cellA;
             const int64_t pos = (int64_t)(a - D->r0) * D->ld + (bA - D->c0);
             T.best_cell[pos] = c;
             T.best_score[pos] = (int16_t)bestA;
         }
         if (bB >= 0) {
-            const uint32_t c = (La == 0 || LbB == 0 || bestB >= kSat16) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
+            const uint32_t c = (La == 0 || LbB == 0 || bestB >= sat) ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)LbB) : cellB;
             const int64_t pos = (int64_t)(a - D->r0) * D->ld + (bB - D->c0);
             T.best_cell[pos] = c;
             T.best_score[pos] = (int16_t)bestB;
@@ -2334,25 +2369,22 @@ int lpp16_blocks_per_cu(int strip_rows, int ncode)
     return err == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int nslots,
-                          hipStream_t st)
+hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, bool mx3,
+                          int nslots, hipStream_t st)
 {
     const bool joint = S.ncode <= kJointMaxCodes;
+#define SWG_SCORE16(RR)                                                                                           \
+    if (joint && mx3) hipLaunchKernelGGL((score16_kernel<RR, true, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W); \
+    else if (joint) hipLaunchKernelGGL((score16_kernel<RR, true, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W); \
+    else if (mx3) hipLaunchKernelGGL((score16_kernel<RR, false, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);  \
+    else hipLaunchKernelGGL((score16_kernel<RR, false, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
     switch (strip_rows) {
-    case 16:
-        if (joint) hipLaunchKernelGGL((score16_kernel<16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        else hipLaunchKernelGGL((score16_kernel<16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        break;
-    case 24:
-        if (joint) hipLaunchKernelGGL((score16_kernel<24, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        else hipLaunchKernelGGL((score16_kernel<24, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        break;
-    case 32:
-        if (joint) hipLaunchKernelGGL((score16_kernel<32, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        else hipLaunchKernelGGL((score16_kernel<32, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-        break;
+    case 16: SWG_SCORE16(16) break;
+    case 24: SWG_SCORE16(24) break;
+    case 32: SWG_SCORE16(32) break;
     default: return hipErrorInvalidValue;
     }
+#undef SWG_SCORE16
     return hipGetLastError();
 }
 
@@ -2487,26 +2519,23 @@ int rerun_blocks_per_cu(int strip_rows, int ncode)
     return err == hipSuccess ? nb : 0;
 }
 
-int score16_blocks_per_cu(int strip_rows, int ncode)
+int score16_blocks_per_cu(int strip_rows, int ncode, bool mx3)
 {
     const bool joint = ncode <= kJointMaxCodes;
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
+#define SWG_SCORE16_OCC(RR)                                                                          \
+    err = joint && mx3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<RR, true, true>, kWave, 0)   \
+        : joint        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<RR, true, false>, kWave, 0)  \
+        : mx3          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<RR, false, true>, kWave, 0)  \
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<RR, false, false>, kWave, 0);
     switch (strip_rows) {
-    case 16:
-        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, true>, kWave, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<16, false>, kWave, 0);
-        break;
-    case 24:
-        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<24, true>, kWave, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<24, false>, kWave, 0);
-        break;
-    case 32:
-        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32, true>, kWave, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, score16_kernel<32, false>, kWave, 0);
-        break;
+    case 16: SWG_SCORE16_OCC(16) break;
+    case 24: SWG_SCORE16_OCC(24) break;
+    case 32: SWG_SCORE16_OCC(32) break;
     default: break;
     }
+#undef SWG_SCORE16_OCC
     return err == hipSuccess ? nb : 0;
 }
 

@@ -222,6 +222,11 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * block's 16-lane waves would not fill the GPU (a lone query), else 16 --;
  * 16 or 32 fix it. */
 #define SWG_OPT_LPS_LANES 7
+/* SWG_OPT_SCORE_MAX3 = 1 (default): the int16 score pass of the two-pass
+ * schedule runs its max3 form (values biased by gap_open, three-way maxima in
+ * one packed instruction) where score + gap_open >= 0 for every pair of the
+ * set's residue codes; 0 runs the plain form.  Results are identical. */
+#define SWG_OPT_SCORE_MAX3 8
 int  swg_set_option(swg_ctx* ctx, int32_t option, int64_t value);
 /* Timing of the last block's kernels (HIP events on the context stream
  * around its graph or launch): milliseconds, DP cells, strip height of the
