@@ -304,7 +304,8 @@ int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 
 // The configuration of variant v with its default strips, or those the
 // caller fixed (swg_set_schedule's strip_rows, SWG_OPT_SCORE_STRIP_ROWS).
 // The score pass runs its max3 form where the sequence set allows it,
-// unless SWG_OPT_SCORE_MAX3 turned it off.
+// unless SWG_OPT_SCORE_MAX3 turned it off.  (The lanes-per-pair score pass
+// always can: its byte table holds score + gap_open >= 0.)
 bool use_mx3(const swg_ctx* c) { return c->mx3_ok && c->score_max3 != 0; }
 
 VariantCfg default_cfg(const swg_ctx* c, int v)
@@ -342,7 +343,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
     int a = c->nw        ? nw_blocks_per_cu(s1, nw_region)
           : v == kVarI16 ? score16_blocks_per_cu(s1, c->ncode, use_mx3(c))
           : v == kVarLpp ? lpp16_blocks_per_cu(s1, c->ncode)
-          : v == kVarLps ? lps16_blocks_per_cu(s1, f.lanes)
+          : v == kVarLps ? lps16_blocks_per_cu(s1, f.lanes, c->score_max3 != 0)
                          : sw_blocks_per_cu(s1, false);
     int b = two_pass(v) ? rerun_blocks_per_cu(s2, c->ncode) : 0;
     int r = two_pass(v) ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
@@ -676,7 +677,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
             e = launch_fill_i32(P.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess) e = launch_fill_i32(P2.meta, 0, kMetaWords + kNumBuckets, st);
             if (e == hipSuccess)
-                e = v == kVarLps ? launch_lps16(S, ST, W, s1, f.lanes, n1, st) : launch_score16(S, ST, W, s1, use_mx3(c), n1, st);
+                e = v == kVarLps ? launch_lps16(S, ST, W, s1, f.lanes, c->score_max3 != 0, n1, st) : launch_score16(S, ST, W, s1, use_mx3(c), n1, st);
             if (e == hipSuccess) e = after(v == kVarLps ? SWG_KERNEL_LPS16 : SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
@@ -721,7 +722,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
         append_key(key, c->reverse_strip); append_key(key, nw_region); append_key(key, f.lanes);
-        append_key(key, use_mx3(c));
+        append_key(key, use_mx3(c)); append_key(key, c->score_max3);
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -902,7 +903,7 @@ VariantCfg lps_cfg(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
     const double rows = g.r1 - g.r0 + 1, cols = g.c1 - g.c0 + 1;
     const double tile = lps_tile(16);
     const double waves = g.diag_lower ? rows * std::ceil(cols / 2.0 / tile) : rows * std::ceil(cols / tile);
-    const int slots = std::max(1, lps16_blocks_per_cu(f.s1, 16) * c->num_cu / ln.share);
+    const int slots = std::max(1, lps16_blocks_per_cu(f.s1, 16, c->score_max3 != 0) * c->num_cu / ln.share);
     if (waves < slots) f.lanes = 32;
     if (c->lps_lanes) f.lanes = c->lps_lanes;
     return f;

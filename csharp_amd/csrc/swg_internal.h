@@ -201,9 +201,10 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
 // [0, 255], gap_open >= 1 and a pad code < 8 or >= 12 (the host checks this:
 // swg_ctx::lps_ok).
 constexpr int lps_tile(int lanes) { return 2 * kWave / lanes; }
+// mx3: its max3 form (always possible where the table holds the scores).
 hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int lanes,
-                        int nslots, hipStream_t stream);
-int lps16_blocks_per_cu(int strip_rows, int lanes);
+                        bool mx3, int nslots, hipStream_t stream);
+int lps16_blocks_per_cu(int strip_rows, int lanes, bool mx3);
 // Sort of the score pass's first-maximum cells (count, scan, scatter).
 hipError_t launch_sort(const SortSet& P, int grid, hipStream_t stream);
 // Re-run of the sorted pairs; strip_rows in {16, 32}; track = the flagged list.

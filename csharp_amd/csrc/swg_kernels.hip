@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void lps_fold(uint32_t (&bm)[R], uint32_t& GV, uint32
     GV = u2(__builtin_elementwise_max(v2(GV), v2(BV)));
 }
 
-template <int R, int LN>
+template <int R, int LN, bool MX3>
 __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
 {
     static_assert(LN == 16 || LN == 32, "16 or 32 lanes per pair");
@@ -1074,7 +1074,11 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
     const int ntasks = D->ntasks;
     const int o = S.gap_open, e = S.gap_ext;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
-    const uint32_t T0 = splat16(-o), NEG2 = splat16(kPad16), M1 = splat16(-1);
+    const uint32_t NEG2 = splat16(kPad16), M1 = splat16(-1);
+    // t = V - o, G and H' at DP row 0 / column 0 (MX3: biased by o, see
+    // score16_kernel; the table's bytes are score + o >= 0 and 0 for pads)
+    const uint32_t T0 = MX3 ? 0u : splat16(-o), G0 = MX3 ? 0u : NEG2, H0 = MX3 ? splat16(o) : 0u;
+    const int sat = MX3 ? kSatMx3(o) : kSat16;
     const uint32_t padsel = (uint32_t)min(pad, 12) * 0x00010001u | 0x0C000C00u;
     const u16x2 C12 = __builtin_bit_cast(u16x2, splat16(12));
 
@@ -1103,7 +1107,7 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
         // row 0 of the DP (t = -o, G = -inf), also past the last column so
         // that lane 0's look-ahead reads defined values
         for (int j = 1 + lane; j <= nsteps + 8; j += kWave)
-            for (int q = 0; q < NG; ++q) bnd[(int64_t)j * NG + q] = make_int2((int)T0, (int)NEG2);
+            for (int q = 0; q < NG; ++q) bnd[(int64_t)j * NG + q] = make_int2((int)T0, (int)G0);
         __syncthreads();  // other lanes read these columns
 
         uint32_t GV = M1, GR = 0u, GJ = 0u;
@@ -1119,9 +1123,9 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
             }
             uint32_t tr[R], Hr[R], bm[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = 0u; bm[r] = NEG2; }
+            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = H0; bm[r] = NEG2; }
             const uint32_t row1 = splat16(i0 + k * R + 1);  // (1-based) row of this lane's first row
-            uint32_t ot = T0, oG = NEG2;  // this lane's last row (t, G) of the previous step
+            uint32_t ot = T0, oG = G0;  // this lane's last row (t, G) of the previous step
             uint32_t pt = T0;             // the row above, received at the previous step (diagonal of row 0)
             uint32_t sel = padsel;        // this lane's column codes as a byte selector
 
@@ -1153,18 +1157,20 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
                     for (int r = 0; r < R; ++r) {
                         const uint32_t sp = __builtin_amdgcn_perm(thi[r], tlo[r], sel);
                         const s16x2 G = __builtin_elementwise_max(v2(Gu) - E2, v2(tu));
-                        // H' = max(H, 0), see score16_kernel
-                        const s16x2 H = __builtin_elementwise_max(
-                            __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
-                                __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))),
-                            v2(tr[r]));
+                        // H' = max(H, 0), see score16_kernel (MX3: its form)
+                        const uint32_t hs = u2(__builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                            __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))));
+                        const uint32_t H = MX3 ? max3b(hs, tr[r], u2(O2))
+                                               : u2(__builtin_elementwise_max(v2(hs), v2(tr[r])));
                         const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
-                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
+                        const s16x2 V = MX3 ? v2(max3b(u2(f), H, u2(G)))
+                                            : __builtin_elementwise_max(__builtin_elementwise_max(f, v2(H)), G);
                         const s16x2 tn = V - O2;
-                        bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
+                        if (!MX3) bm[r] = u2(__builtin_elementwise_max(v2(bm[r]), V));
+                        else if (q & 1) bm[r] = max3b(bm[r], tr[r], u2(tn));
                         dt = tr[r];
                         tr[r] = u2(tn);
-                        Hr[r] = u2(H);
+                        Hr[r] = H;
                         tu = u2(tn);
                         Gu = u2(G);
                     }
@@ -1201,7 +1207,7 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
             for (int h = 0; h < 2; ++h) {
                 const int b = h ? bB : bA, Lb = h ? LbB : LbA;
                 if (b < 0) continue;
-                const uint32_t cell = (La == 0 || Lb == 0 || best[h] >= kSat16)
+                const uint32_t cell = (La == 0 || Lb == 0 || best[h] >= sat)
                                           ? (kCellTrack | ((uint32_t)La << 16) | (uint32_t)Lb)
                                           : (((uint32_t)brow[h] << 16) | (uint32_t)bcol[h]);
                 const int64_t pos = (int64_t)(a - D->r0) * D->ld + (b - D->c0);
@@ -2389,25 +2395,26 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
 }
 
 hipError_t launch_lps16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, int lanes,
-                        int nslots, hipStream_t st)
+                        bool mx3, int nslots, hipStream_t st)
 {
-    if (strip_rows == 8 && lanes == 16)
-        hipLaunchKernelGGL((lps16_kernel<8, 16>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-    else if (strip_rows == 8 && lanes == 32)
-        hipLaunchKernelGGL((lps16_kernel<8, 32>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
-    else
-        return hipErrorInvalidValue;
+    if (strip_rows != 8 || (lanes != 16 && lanes != 32)) return hipErrorInvalidValue;
+    if (lanes == 16 && mx3) hipLaunchKernelGGL((lps16_kernel<8, 16, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else if (lanes == 16) hipLaunchKernelGGL((lps16_kernel<8, 16, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else if (mx3) hipLaunchKernelGGL((lps16_kernel<8, 32, true>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
+    else hipLaunchKernelGGL((lps16_kernel<8, 32, false>), dim3(nslots), dim3(kWave), 0, st, S, T, W);
     return hipGetLastError();
 }
 
-int lps16_blocks_per_cu(int strip_rows, int lanes)
+int lps16_blocks_per_cu(int strip_rows, int lanes, bool mx3)
 {
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
     if (strip_rows == 8 && lanes == 16)
-        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 16>, kWave, 0);
+        err = mx3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 16, true>, kWave, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 16, false>, kWave, 0);
     else if (strip_rows == 8 && lanes == 32)
-        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 32>, kWave, 0);
+        err = mx3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 32, true>, kWave, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lps16_kernel<8, 32, false>, kWave, 0);
     return err == hipSuccess ? nb : 0;
 }
 

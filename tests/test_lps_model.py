@@ -12,7 +12,9 @@ row, block of 16 columns starting at the lane's first column of the window,
 clamped to 1) and the lanes merged at the end (larger value, then earlier
 row).  It must give the oracle's score, its first-maximum row, and a block
 holding its first-maximum column -- the contract of the score pass that the
-reverse pass and the re-run rely on."""
+reverse pass and the re-run rely on.  The max3 form (values biased by
+gap_open, three-way maxima on the int16 patterns read as fp16, the row
+maximum over two steps at once) is modelled too."""
 import random
 
 import numpy as np
@@ -20,6 +22,7 @@ import pytest
 
 from oracle import oracle as O
 from tests.test_oracle import random_related
+from tests.test_score_pass_model import _add_sat, _i16, _max3_f16, _sub_sat_u16
 
 BLOCK = 16  # kFirstMaxBlock: columns per first-maximum block, steps per fold
 PAD16 = -16384
@@ -29,7 +32,7 @@ def sat16(x):
     return max(-32768, min(32767, x))
 
 
-def lps_model(s1, s2, M, o, e, R, extra=b"", LANES=16):
+def lps_model(s1, s2, M, o, e, R, extra=b"", LANES=16, mx3=False):
     """(best, row, jb) of pair (s1, s2) as lps16_kernel<R> computes it, for
     one half (the other half of a lane runs the same recurrence on its own
     sequence).  Codes as the library assigns them: the column symbols
@@ -54,20 +57,22 @@ def lps_model(s1, s2, M, o, e, R, extra=b"", LANES=16):
     q1 = [code_of[c] for c in s1]
     q2 = [code_of[c] for c in s2]
     nsteps = (Lb + (LANES - 1) + 3) & ~3
-    T0, NEG = -o, PAD16
+    NEG = PAD16
+    # t = V - o, G and H' of DP row 0 / column 0 (max3 form: biased by o)
+    T0, G0, H0 = (0, 0, o) if mx3 else (-o, NEG, 0)
     # boundary row (t, G) per column 1 .. nsteps + 8: DP row 0
     bnd_t = [T0] * (nsteps + 9)
-    bnd_G = [NEG] * (nsteps + 9)
+    bnd_G = [G0] * (nsteps + 9)
     GV = [-1] * LANES
     GR = [0] * LANES
     GJ = [0] * LANES
     for i0 in range(0, La, LANES * R):
         rowcode = [[q1[i0 + k * R + r] if i0 + k * R + r < La else pad for r in range(R)] for k in range(LANES)]
         tr = [[T0] * R for _ in range(LANES)]
-        Hr = [[0] * R for _ in range(LANES)]
+        Hr = [[H0] * R for _ in range(LANES)]
         bm = [[NEG] * R for _ in range(LANES)]
         ot = [T0] * LANES
-        oG = [NEG] * LANES
+        oG = [G0] * LANES
         pt = [T0] * LANES
         sel = [pad] * LANES
         new_bnd_t, new_bnd_G = list(bnd_t), list(bnd_G)
@@ -86,12 +91,21 @@ def lps_model(s1, s2, M, o, e, R, extra=b"", LANES=16):
                 pt[k] = tu
                 for r in range(R):
                     sp = tab(rowcode[k][r], sel[k])
-                    G = max(Gu - e, tu)
-                    H = max(max(Hr[k][r] - e, 0), tr[k][r])
-                    f = sat16(dt + sp)
-                    V = max(f, H, G)
-                    tn = V - o
-                    bm[k][r] = max(bm[k][r], V)
+                    if mx3:
+                        G = max(_i16(Gu - e), tu)
+                        H = _max3_f16(_sub_sat_u16(Hr[k][r], e), tr[k][r], o)
+                        f = _add_sat(dt, sp)
+                        V = _max3_f16(f, H, G)
+                        tn = _i16(V - o)
+                        if tt % 2 == 1:
+                            bm[k][r] = _max3_f16(bm[k][r], tr[k][r], tn)
+                    else:
+                        G = max(Gu - e, tu)
+                        H = max(max(Hr[k][r] - e, 0), tr[k][r])
+                        f = sat16(dt + sp)
+                        V = max(f, H, G)
+                        tn = V - o
+                        bm[k][r] = max(bm[k][r], V)
                     dt = tr[k][r]
                     tr[k][r] = tn
                     Hr[k][r] = H
@@ -137,11 +151,12 @@ def cases(ednafull):
     return out
 
 
+@pytest.mark.parametrize("mx3", [False, True])
 @pytest.mark.parametrize("R,extra,lanes", [(1, b"", 16), (2, b"", 16), (8, b"", 16), (2, b"BDHKMRSUVWY", 16),
                                            (1, b"", 32), (8, b"", 32)])
-def test_lps_model_matches_the_oracle(ednafull, R, extra, lanes):
+def test_lps_model_matches_the_oracle(ednafull, R, extra, lanes, mx3):
     for a, b, o, e in cases(ednafull):
-        best, bi, jb = lps_model(a, b, ednafull, o, e, R, extra, LANES=lanes)
+        best, bi, jb = lps_model(a, b, ednafull, o, e, R, extra, LANES=lanes, mx3=mx3)
         r = O.sw_literal(a, b, ednafull, o, e)
         assert best == int(r["score"]), (a, b, o, e)
         assert bi == r["best_row"], (a, b, o, e, bi, r["best_row"])
