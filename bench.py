@@ -52,7 +52,8 @@ VALU_PER_CELL = {"score16": 4.757, "reverse": 5.672, "rerun_capture": 21.797, "r
 # instructions on the 9 operations of a cell pair (its three-way maxima are
 # one instruction each), so its roofline fraction at 9 operations per cell
 # can reach 9 / 7.5 = 1.2 of the issue rate.
-VALU_PER_CELL_SCORE16 = {(True, 32): 3.981, (True, 24): 4.007, (False, 32): 4.731, (False, 24): 4.757}
+VALU_PER_CELL_SCORE16 = {(True, 40): 3.966, (True, 32): 3.981, (True, 24): 4.007, (False, 40): 4.716,
+                         (False, 32): 4.731, (False, 24): 4.757}
 
 
 def log(msg):

@@ -322,7 +322,9 @@ VariantCfg default_cfg(const swg_ctx* c, int v)
     }
     else if (v == kVarI32) f.s1 = c->strip_rows ? c->strip_rows : 16;
     else {
-        f.s1 = c->score_strip ? c->score_strip : 24;  // score pass: 24 rows at three waves per SIMD
+        // score pass: 32 rows at two waves per SIMD in the max3 form, else 24
+        // rows at three
+        f.s1 = c->score_strip ? c->score_strip : use_mx3(c) ? 32 : 24;
         f.s2 = c->strip_rows == 32 ? 32 : 16;         // re-run
     }
     return f;
@@ -801,8 +803,9 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
 //            int32; the two-pass first for alphabets of <= 8 codes, the
 //            one-pass first for larger ones, so that a run of a single block
 //            gets the variant that measured faster on such data;
-//   stage 1  the winner's first strip: score strips 24, 32, 16, int32 strips
-//            8 and 32, rows per lane 8 <-> 4, the global kernel's 8 <-> 16;
+//   stage 1  the winner's first strip: score strips 24, 32 and 40 (16, never
+//            faster on measured data, only when fixed), int32 strips 8 and 32,
+//            rows per lane 8 <-> 4, the global kernel's 8 <-> 16;
 //   stage 2  the two-pass re-run strip 16 <-> 32.
 // Strips fixed by the caller (swg_set_schedule's strip_rows,
 // SWG_OPT_SCORE_STRIP_ROWS) end the search after stage 0.
@@ -850,7 +853,7 @@ std::vector<VariantCfg> stage_candidates(const swg_ctx* c, const TuneEntry& t)
     if (t.stage == 1) {
         if (c->nw) { f.s1 = b.s1 == 16 ? 8 : 16; out.push_back(f); }
         else if (b.v == kVarI16)
-            for (int r : {24, 32, 16})
+            for (int r : {24, 32, 40})
                 if (r != b.s1) { f.s1 = r; out.push_back(f); }
         else if (b.v == kVarLpp) { f.s1 = b.s1 == 4 ? 8 : 4; out.push_back(f); }
         else if (b.v == kVarI32)
@@ -1611,8 +1614,8 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
         return SWG_OK;
     }
     if (option == SWG_OPT_SCORE_STRIP_ROWS) {
-        if (value != 0 && value != 16 && value != 24 && value != 32)
-            return fail(SWG_E_ARGUMENT, "score-pass strip rows are 0 (autotuned), 16, 24 or 32");
+        if (value != 0 && value != 16 && value != 24 && value != 32 && value != 40)
+            return fail(SWG_E_ARGUMENT, "score-pass strip rows are 0 (autotuned), 16, 24, 32 or 40");
         if (value != c->score_strip) c->tuned.clear();
         c->score_strip = (int32_t)value;
         return SWG_OK;
@@ -1625,6 +1628,7 @@ int swg_set_option(swg_ctx* c, int32_t option, int64_t value)
     }
     if (option == SWG_OPT_SCORE_MAX3) {
         if (value != 0 && value != 1) return fail(SWG_E_ARGUMENT, "the score pass's max3 form: 0 (off) or 1 (on)");
+        if (value != c->score_max3) c->tuned.clear();  // its default strip and rates differ
         c->score_max3 = (int32_t)value;
         return SWG_OK;
     }

@@ -189,7 +189,7 @@ hipError_t launch_sw(const SeqSet& S, const TaskSet& T, const Scratch& W, int st
 // PercentIdentity, or statistics requested).
 hipError_t launch_nw(const SeqSet& S, const TaskSet& T, const Scratch& W, int strip_rows, bool region,
                      bool persistent, int nslots, hipStream_t stream);
-// Score-only int16x2 pass; strip_rows in {16, 24, 32}; mx3: the biased
+// Score-only int16x2 pass; strip_rows in {16, 24, 32, 40}; mx3: the biased
 // max3 form (needs score + gap_open >= 0 for every pair of the set's codes).
 hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch& W, int strip_rows, bool mx3,
                           int nslots, hipStream_t stream);

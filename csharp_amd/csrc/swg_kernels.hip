@@ -2388,6 +2388,7 @@ hipError_t launch_score16(const SeqSet& S, const ScoreTaskSet& T, const Scratch&
     case 16: SWG_SCORE16(16) break;
     case 24: SWG_SCORE16(24) break;
     case 32: SWG_SCORE16(32) break;
+    case 40: SWG_SCORE16(40) break;
     default: return hipErrorInvalidValue;
     }
 #undef SWG_SCORE16
@@ -2540,6 +2541,7 @@ int score16_blocks_per_cu(int strip_rows, int ncode, bool mx3)
     case 16: SWG_SCORE16_OCC(16) break;
     case 24: SWG_SCORE16_OCC(24) break;
     case 32: SWG_SCORE16_OCC(32) break;
+    case 40: SWG_SCORE16_OCC(40) break;
     default: break;
     }
 #undef SWG_SCORE16_OCC

@@ -201,8 +201,8 @@ int  swg_set_schedule(swg_ctx* ctx, int32_t schedule, int32_t strip_rows);
  * after each of them, inside the block's captured graph, for
  * swg_last_kernel_times; 0 (default) leaves the stamps out. */
 #define SWG_OPT_KERNEL_TIMING 2
-/* SWG_OPT_SCORE_STRIP_ROWS = rows per strip of the int16 score pass (16, 24
- * or 32; 0 = autotuned, default 24).  Like swg_set_schedule's strip_rows, a
+/* SWG_OPT_SCORE_STRIP_ROWS = rows per strip of the int16 score pass (16, 24,
+ * 32 or 40; 0 = autotuned, first tried: 32 in the max3 form, else 24).  Like swg_set_schedule's strip_rows, a
  * fixed value limits the autotuner to the choice of variant. */
 #define SWG_OPT_SCORE_STRIP_ROWS 3
 /* SWG_OPT_TUNE_MIN_CELLS = DP cells a block needs for the autotuner to time

@@ -36,7 +36,7 @@ def wide_expected(wide_set, ednafull):
     return d.reshape(a.shape)
 
 
-@pytest.mark.parametrize("score_strip", [16, 24, 32])
+@pytest.mark.parametrize("score_strip", [16, 24, 32, 40])
 @pytest.mark.parametrize("rerun_strip", [16, 32])
 def test_forced_two_pass_tile_shapes(wide_set, wide_expected, ednafull, score_strip, rerun_strip):
     say("two-pass score strip %d re-run strip %d" % (score_strip, rerun_strip))
@@ -77,8 +77,8 @@ def test_autotuner_searches_variant_then_tile_shape(wide_set, wide_expected, edn
     trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), wide_expected)
     seen = [shape(t) for t in trials]
     # stage 0: the three variants at their default tile shapes, the two-pass
-    # first for a 4-code alphabet
-    assert seen[:3] == [(ca.SCHED_PERSISTENT_I16, 24, 16), (ca.SCHED_PERSISTENT_I32, 16, 0),
+    # first for a 4-code alphabet (its score pass in the max3 form: 32 rows)
+    assert seen[:3] == [(ca.SCHED_PERSISTENT_I16, 32, 16), (ca.SCHED_PERSISTENT_I32, 16, 0),
                         (ca.SCHED_PERSISTENT_LPP16, 4, 0)]
     # later stages vary the winner's strips only, starting with the winner
     # itself timed again; no other configuration is measured twice
@@ -101,7 +101,7 @@ def test_autotuner_with_fixed_strips_chooses_the_variant_only(wide_set, wide_exp
     al.set_schedule(ca.SCHED_PERSISTENT, 32)
     al.set_tune_min_cells(0)
     trials, final = run_until_tuned(al, (0, 2999, 3000, 3255), wide_expected)
-    assert [shape(t) for t in trials] == [(ca.SCHED_PERSISTENT_I16, 24, 32), (ca.SCHED_PERSISTENT_I32, 32, 0),
+    assert [shape(t) for t in trials] == [(ca.SCHED_PERSISTENT_I16, 32, 32), (ca.SCHED_PERSISTENT_I32, 32, 0),
                                           (ca.SCHED_PERSISTENT_LPP16, 4, 0)]
     assert shape(final) in [shape(t) for t in trials]
     al.close()

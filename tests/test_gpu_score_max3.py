@@ -35,7 +35,7 @@ def run(s, M, o, e, blk, max3, strip=0, stats=True, **kw):
     return got, gst
 
 
-@pytest.mark.parametrize("strip", [16, 24, 32])
+@pytest.mark.parametrize("strip", [16, 24, 32, 40])
 def test_max3_form_matches_the_plain_form_and_the_oracle(ednafull, strip):
     """Related and unrelated DNA of 1..1,500 residues (tiles of very unequal
     members, pad rows in every last strip)."""
