@@ -745,6 +745,11 @@ __device__ __forceinline__ void stage_joint16(uint32_t* jp, const SeqSet& S, con
 // per row the kernel runs score16_waves(R) waves per SIMD.
 constexpr int score16_waves(int R) { return R <= 16 ? 4 : R <= 24 ? 3 : 2; }
 
+template <int N>
+struct StripRows {
+    static constexpr int value = N;
+};
+
 template <int R, bool JOINT, bool MX3>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(score16_waves(R), score16_waves(R)))) void
 score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
@@ -793,15 +798,19 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
         // the pair's first maximum: value (-1: none yet), row (1-based) and
         // first column of its block, one pair per 16-bit half
         uint32_t GV = M1, GR = 0u, GJ = 0u;
-        for (int i0 = 0; i0 < La; i0 += R) {
+        // one strip of RR rows from row i0: R rows, and for the last strip
+        // of the row sequence the next multiple of 8 rows at or above what
+        // is left (fewer pad rows)
+        auto strip = [&](auto rr, const int i0) {
+            constexpr int RR = decltype(rr)::value;
             __syncthreads();
-            if (JOINT) stage_joint16<R, MX3>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
-            else stage_profile16<R, MX3>(lprof, S, s1, La, i0, lane);
+            if (JOINT) stage_joint16<RR, MX3>(reinterpret_cast<uint32_t*>(lprof), S, s1, La, i0, lane);
+            else stage_profile16<RR, MX3>(lprof, S, s1, La, i0, lane);
             __syncthreads();
 
-            uint32_t tr[R], Hr[R], bm[R];
+            uint32_t tr[RR], Hr[RR], bm[RR];
 #pragma unroll
-            for (int r = 0; r < R; ++r) { tr[r] = T0; Hr[r] = H0; bm[r] = NEG2; }
+            for (int r = 0; r < RR; ++r) { tr[r] = T0; Hr[r] = H0; bm[r] = NEG2; }
             const uint32_t row1 = splat16(i0 + 1);  // row of strip row 0
             uint32_t dtb = T0;
 
@@ -822,7 +831,7 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
             // int4 pairs (code ca, code cc) of 8 rows, shared by groups 2m and
             // 2m+1); load(g, ...) replaces the group's words, once the group
             // has been computed, with those of the next column
-            int4 cur[R / 4];
+            int4 cur[RR / 4];
             auto load = [&](int g, uint32_t ca, uint32_t cc) {
                 if (JOINT) {
                     const uint32_t* pj =
@@ -830,15 +839,15 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                     cur[g] = make_int4((int)pj[(4 * g) * kJointRow], (int)pj[(4 * g + 1) * kJointRow],
                                        (int)pj[(4 * g + 2) * kJointRow], (int)pj[(4 * g + 3) * kJointRow]);
                 } else if (g & 1) {
-                    cur[g - 1] = lprof[ca * PS + (g >> 1)];
-                    cur[g] = lprof[cc * PS + (g >> 1)];
+                    cur[g - 1] = lprof[ca * (RR / 8 + 1) + (g >> 1)];
+                    cur[g] = lprof[cc * (RR / 8 + 1) + (g >> 1)];
                 }
             };
             // JOINT: the byte offsets of the word's four columns
             auto joint_offsets = [](uint32_t wa, uint32_t wb) { return (wa << 5) | (wb << 2); };
             uint32_t nxo = JOINT ? joint_offsets(nwa, nwb) : 0u;
 #pragma unroll
-            for (int g = 0; g < R / 4; ++g) {
+            for (int g = 0; g < RR / 4; ++g) {
                 if (JOINT) load(g, nxo & 0xFFu, 0u);
                 else load(g, nwa & 0xFFu, nwb & 0xFFu);
             }
@@ -863,7 +872,7 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                     uint32_t dt = dtb;
                     dtb = tu;
 #pragma unroll
-                    for (int r4 = 0; r4 < R / 4; ++r4) {
+                    for (int r4 = 0; r4 < RR / 4; ++r4) {
                         uint32_t sp4[4];
                         if (JOINT) {
                             const int4 x = cur[r4];
@@ -932,7 +941,7 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                     uint32_t BV = bm[0], BR = 0u;
                     bm[0] = NEG2;
 #pragma unroll
-                    for (int r = 1; r < R; ++r) {
+                    for (int r = 1; r < RR; ++r) {
                         const uint32_t d = u2(v2(BV) - v2(bm[r]));
                         uint32_t mk;  // bm[r] > BV
                         asm("v_pk_ashrrev_i16 %0, %1, %2" : "=v"(mk) : "s"(0x000F000Fu), "v"(d));
@@ -955,7 +964,16 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
                     GV = u2(__builtin_elementwise_max(v2(GV), v2(BV)));
                 }
             }
-        }
+        };
+        int i0 = 0;
+        for (; La - i0 > R - 8; i0 += R) strip(StripRows<R>{}, i0);
+        const int q8 = (La - i0 + 7) >> 3;  // wave-uniform: 0 .. R / 8 - 1 blocks of 8 rows left
+        if (q8 == 1) strip(StripRows<8>{}, i0);
+        if constexpr (R >= 24) if (q8 == 2) strip(StripRows<16>{}, i0);
+        if constexpr (R >= 32) if (q8 == 3) strip(StripRows<24>{}, i0);
+        if constexpr (R >= 40) if (q8 == 4) strip(StripRows<32>{}, i0);
+        static_assert(R <= 40, "the last strip covers at most 40 rows");
+
         const int bestA = (short)(GV & 0xFFFFu), bestB = (short)(GV >> 16);
         const uint32_t cellA = ((GR & 0xFFFFu) << 16) | (GJ & 0xFFFFu), cellB = (GR & 0xFFFF0000u) | (GJ >> 16);
         if (bA >= 0) {

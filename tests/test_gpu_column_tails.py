@@ -6,8 +6,8 @@ the columns past the sequence's end see the pad code without a per-column
 length test.  These blocks put sequences of every length 0..40 (every
 residue count mod 4, the empty sequence included) on the column side and on
 the row side against longer ones, in the joint-profile path (DNA, <= 8
-codes) and the per-code path (protein), and compare distances and
-statistics bit for bit with the CPU oracle.
+codes) and the per-code path (protein), in the score pass's max3 and plain
+forms, and compare distances and statistics bit for bit with the CPU oracle.
 """
 import random
 
@@ -37,13 +37,16 @@ def tails_set(alphabet, seed):
     return ca.SequenceSet.from_strings(short + longer), len(short)
 
 
-@pytest.mark.parametrize("strip", [16, 32])
-def test_dna_joint_profile_tails(ednafull, strip):
-    say("column tails dna strip=%d" % strip)
+@pytest.mark.parametrize("max3", [True, False])
+@pytest.mark.parametrize("strip,score_strip", [(16, 40), (32, 24)])
+def test_dna_joint_profile_tails(ednafull, strip, score_strip, max3):
+    say("column tails dna strip=%d score strip=%d max3=%s" % (strip, score_strip, max3))
     seqs, ns = tails_set(b"ACGT", 31)
     n = len(seqs)
     al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
     al.set_schedule(ca.SCHED_PERSISTENT_I16, strip)
+    al.set_score_strip(score_strip)
+    al.set_score_max3(max3)
     al.set_sequences(seqs)
     al.set_kernel_timing(True)
     for r0, r1, c0, c1 in ((ns, n - 1, 0, ns - 1), (0, ns - 1, ns, n - 1)):
@@ -54,11 +57,13 @@ def test_dna_joint_profile_tails(ednafull, strip):
     al.close()
 
 
-def test_protein_per_code_profile_tails(blosum62):
-    say("column tails protein")
+@pytest.mark.parametrize("max3", [True, False])
+def test_protein_per_code_profile_tails(blosum62, max3):
+    say("column tails protein max3=%s" % max3)
     seqs, ns = tails_set(b"ACDEFGHIKLMNPQRSTVWY", 32)
     n = len(seqs)
     al = ca.Aligner(blosum62, 10, 1, alignment_type=ca.ALIGN_PROTEIN, schedule=ca.SCHED_PERSISTENT_I16)
+    al.set_score_max3(max3)
     al.set_sequences(seqs)
     al.set_kernel_timing(True)
     for r0, r1, c0, c1 in ((ns, n - 1, 0, ns - 1), (0, ns - 1, ns, n - 1)):
