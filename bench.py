@@ -346,7 +346,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16",
+        "dtype_detail": "score and reverse passes in packed int16 (saturation-checked), statistics re-runs in int32; "
+                        "exact integer arithmetic throughout",
         "data": "synthetic",
         "config": {
             "workload": "uneven-length DNA all-pairs: 50,000 sequences (60%% N(450,50), 40%% N(1450,100)), "
