@@ -510,6 +510,18 @@ def workload_run(args):
         A = np.concatenate([x[0] for x in samples]); B = np.concatenate([x[1] for x in samples])
         G = np.concatenate([x[2] for x in samples])
         d, dt, th = oracle_sample(seqs, A, B, M, o, e, nucleic, tdouble)
+        # after the timed region: one row chunk's worth of the second block
+        # (as many rows as fit a 64 MiB chunk) in kernel-timing mode, for the
+        # per-kernel split of a streamed chunk
+        c0, c1 = ranges[1]
+        chunk_rows = max(1, (64 << 20) // ((c1 - c0 + 1) * np.dtype(np.int16).itemsize))
+        al.set_kernel_timing(True)
+        _, _, cpairs = al.compute_block(r0, min(r1, r0 + chunk_rows - 1), c0, c1, False)
+        ckms, ccells, _, _ = al.last_kernel_stats()
+        out["chunk_kernels"] = {"rows": min(chunk_rows, r1 - r0 + 1), "cols": c1 - c0 + 1, "pairs": cpairs,
+                                "cells": ccells, "kernel_ms": round(ckms, 2), "gcups": round(ccells / ckms / 1e6, 1),
+                                "kernels": [{"kernel": k[0], "ms": round(k[1], 3)} for k in al.kernel_times()]}
+        al.set_kernel_timing(False)
         out.update(rank=rank, world=world, pairs=tot_pairs, cells=tot_cells, seconds=round(wall, 2),
                    gcups_end_to_end=round(tot_cells / wall / 1e9, 1), block_files_sha256=hashes,
                    parity={"checked_pairs": int(len(A)), "mismatches": int((G != d).sum())},
