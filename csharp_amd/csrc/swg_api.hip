@@ -225,6 +225,7 @@ struct swg_ctx {
     bool int16_ok = false;          // scores + penalties small enough for the int16 score pass
     bool lps_ok = false;            // and the 16-lanes-per-pair score pass (<= 8 codes, score + open in [0, 255])
     bool mx3_ok = false;            // and the score pass's max3 form (score + open >= 0 for the set's codes)
+    int32_t max_score = 0;          // the largest score of the set's codes (the reverse pass's max3 form)
     bool nw = false;                // global alignment (NeedlemanWunschTPL)
 
     // sequences
@@ -308,6 +309,13 @@ int elem_size(const swg_ctx* c) { return c->p.tdistance == SWG_TDIST_DOUBLE ? 8 
 // always can: its byte table holds score + gap_open >= 0.)
 bool use_mx3(const swg_ctx* c) { return c->mx3_ok && c->score_max3 != 0; }
 
+// The reverse pass's max3 form keeps its values biased by kRevBias: a value
+// clamped at the floor must stay dead over the band's at most row_cap rows.
+bool use_rev_mx3(const swg_ctx* c)
+{
+    return use_mx3(c) && (int64_t)c->reverse_row_cap * c->max_score + std::max(c->gap_open, c->gap_ext) < kRevBias;
+}
+
 VariantCfg default_cfg(const swg_ctx* c, int v)
 {
     VariantCfg f;
@@ -348,7 +356,7 @@ int slots_for(const swg_ctx* c, const Lane& ln, const VariantCfg& f, int* n1, in
           : v == kVarLps ? lps16_blocks_per_cu(s1, f.lanes, c->score_max3 != 0)
                          : sw_blocks_per_cu(s1, false);
     int b = two_pass(v) ? rerun_blocks_per_cu(s2, c->ncode) : 0;
-    int r = two_pass(v) ? reverse_blocks_per_cu(c->reverse_strip, c->ncode) : 0;
+    int r = two_pass(v) ? reverse_blocks_per_cu(c->reverse_strip, c->ncode, use_rev_mx3(c)) : 0;
     int q = two_pass(v) ? rect64_blocks_per_cu(c->ncode) : 0;
     if (a <= 0 || (two_pass(v) && (b <= 0 || r <= 0 || q <= 0))) return fail(SWG_E_DEVICE, "occupancy query failed");
     *n1 = std::max(1, a * c->num_cu / ln.share);
@@ -683,7 +691,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
             if (e == hipSuccess) e = after(v == kVarLps ? SWG_KERNEL_LPS16 : SWG_KERNEL_SCORE16);
             if (e == hipSuccess) e = launch_sort(P, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
-            if (e == hipSuccess) e = launch_reverse(S, RV, W, c->reverse_strip, n3, st);
+            if (e == hipSuccess) e = launch_reverse(S, RV, W, c->reverse_strip, use_rev_mx3(c), n3, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_REVERSE);
             if (e == hipSuccess) e = launch_sort(P2, sort_grid, st);
             if (e == hipSuccess) e = after(SWG_KERNEL_SORT);
@@ -724,7 +732,7 @@ int run_variant(swg_ctx* c, Lane& ln, BlockBufs& B, const BlockGeom& g, const Va
         append_key(key, S2); append_key(key, T2); append_key(key, RT2); append_key(key, RC2); append_key(key, RB2);
         append_key(key, timing); append_key(key, c->kt_stamp.p); append_key(key, pair_grid); append_key(key, tdist);
         append_key(key, c->reverse_strip); append_key(key, nw_region); append_key(key, f.lanes);
-        append_key(key, use_mx3(c)); append_key(key, c->score_max3);
+        append_key(key, use_mx3(c)); append_key(key, c->score_max3); append_key(key, use_rev_mx3(c));
         hipGraphExec_t& gexec = B.graph[v][gs];
         std::vector<unsigned char>& gkey = B.graph_key[v][gs];
         if (!gexec || key != gkey) {
@@ -1126,9 +1134,12 @@ int load_sequences(swg_ctx* c, const uint8_t* residues, const int64_t* offsets, 
     // the score pass's max3 form keeps every value biased by gap_open and
     // >= 0: score + gap_open >= 0 for every pair of codes
     c->mx3_ok = c->int16_ok && !c->nw;
-    for (int x = 0; x < pad && c->mx3_ok; ++x)
-        for (int y = 0; y < pad; ++y)
+    c->max_score = 0;
+    for (int x = 0; x < pad; ++x)
+        for (int y = 0; y < pad; ++y) {
             if (prof[(size_t)x * c->ncode + y].x < 0) c->mx3_ok = false;
+            c->max_score = std::max(c->max_score, prof[(size_t)x * c->ncode + y].x - c->gap_open);
+        }
 
     // residue codes, each sequence 4-byte aligned and followed by pad codes
     // up to the end of its next word (seq_span), then the slots, then a tail

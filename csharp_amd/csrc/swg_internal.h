@@ -214,10 +214,13 @@ int rerun_blocks_per_cu(int strip_rows, int ncode);
 // Re-run of the big rectangles, one wave per pair.
 hipError_t launch_rect64(const SeqSet& S, const RerunSet& T, const Scratch& W, int nslots, hipStream_t stream);
 int rect64_blocks_per_cu(int ncode);
-// Reverse pass; strip_rows in {8, 16}.
-hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, int nslots,
-                          hipStream_t stream);
-int reverse_blocks_per_cu(int strip_rows, int ncode);
+// Reverse pass; strip_rows in {8, 16}; mx3: its max3 form, values biased by
+// kRevBias (needs score + gap_open >= 0 for the set's codes and row_cap times
+// the largest score plus max(gap_open, gap_ext) below kRevBias).
+constexpr int kRevBias = 16384;
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, bool mx3,
+                          int nslots, hipStream_t stream);
+int reverse_blocks_per_cu(int strip_rows, int ncode, bool mx3);
 // Sixteen-lanes-per-pair kernel: tiles of kLppTile column sequences;
 // strip_rows (rows per lane) in {4, 8}.
 constexpr int kLppTile = 4;

@@ -1905,7 +1905,19 @@ size_t reverse_lds_bytes(int ncode)
 // table (158 VGPRs), else two.
 constexpr int reverse_waves(int R, bool joint) { return R == 8 && joint ? 3 : 2; }
 
-template <int R, bool JOINT>
+// The max3 form of the reverse pass (MX3): every value biased by kRevBias
+// (-inf = 0), kept in [0, 0x7BFF] with unsigned saturating subtracts, the
+// pad score -32768 (a saturating add leaves a negative pattern, below every
+// other operand of a max3), three-way maxima as v_pk_maximum3_f16 on the
+// int16 patterns (see score16_kernel), and the row and column maxima taken
+// in the t = V' - o domain, two columns (rows) per instruction.  A value
+// clamped at 0 stands for a true value below -kRevBias; along the band's at
+// most row_cap rows a path climbs by at most row_cap times the largest
+// score, which the host keeps below kRevBias - max(o, e), so nothing derived
+// from a clamped value reaches the live threshold min(0, o - e) or S: every
+// live cell is computed exactly.  A pair whose S + kRevBias passes 0x7BFF
+// takes its whole prefix as rectangle (always valid).
+template <int R, bool JOINT, bool MX3>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(reverse_waves(R, JOINT), reverse_waves(R, JOINT))))
 void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
 {
@@ -1914,7 +1926,7 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
     const int ncode = S.ncode, pad = ncode - 1, ncode2 = ncode * ncode;
     for (int k = lane; k < ncode2; k += kWave) {
         const int v = S.prof[k].x;
-        tab16[k] = (short)(v < kPad16 ? kPad16 : v);
+        tab16[k] = MX3 ? (short)(v < 0 ? -32768 : v) : (short)(v < kPad16 ? kPad16 : v);
     }
     uint32_t* jt = reinterpret_cast<uint32_t*>(tab16 + ((ncode2 + 1) & ~1));
     if (JOINT) {
@@ -1933,7 +1945,8 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
     const uint32_t ld1 = (uint32_t)D[1].ld;
     const int o = S.gap_open, e = S.gap_ext;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
-    const uint32_t NEG2 = 0x80008000u, T0 = splat16(-o);
+    // -inf and t = V' - o of V' = 0 (MX3: biased)
+    const uint32_t NEG2 = MX3 ? 0u : 0x80008000u, T0 = MX3 ? splat16(kRevBias - o) : splat16(-o);
     const int64_t slot0 = (int64_t)T.meta[kMetaFlaggedTiles] * kWave, slot1 = (int64_t)T.meta[kMetaTiles] * kWave;
     const int ntasks = (int)((slot1 - slot0 + 2 * kWave - 1) / (2 * kWave));
     uint64_t run = 0;  // cells evaluated, added to meta after the task loop
@@ -1964,6 +1977,11 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
                 wend[hh] = min(jb + kFirstMaxBlock - 1, (int)S.len[b[hh]]);
                 anc[hh] = wend[hh] - jb + 1;
                 best[hh] = T.best_score[p[hh]];
+                if (MX3 && best[hh] + kRevBias > 0x7BFF) {
+                    // outside the biased range: no reverse rows, the whole prefix
+                    bi[hh] = 0;
+                    anc[hh] = 0;
+                }
             }
             // reverse columns c'..c'+3 are s2[wend-c'-3 .. wend-c'] (0-based),
             // one misaligned word; columns past wend are pad, word indices
@@ -1977,15 +1995,17 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
         const uint8_t* s1a = S.res + (live[0] ? S.off[a[0]] : S.pad_off);
         const uint8_t* s1b = S.res + (live[1] ? S.off[a[1]] : S.pad_off);
         const uint32_t* s2w = reinterpret_cast<const uint32_t*>(S.res);
-        const uint32_t S2 = ((uint32_t)best[0] & 0xFFFFu) | ((uint32_t)best[1] << 16);
+        // the hit value: S (MX3: t = S - o, biased)
+        const int sb0 = MX3 ? best[0] + kRevBias - o : best[0], sb1 = MX3 ? best[1] + kRevBias - o : best[1];
+        const uint32_t S2 = ((uint32_t)sb0 & 0xFFFFu) | ((uint32_t)sb1 << 16);
         const uint32_t BI2 = ((uint32_t)bi[0] & 0xFFFFu) | ((uint32_t)bi[1] << 16);
         const uint32_t WE2 = ((uint32_t)wend[0] & 0xFFFFu) | ((uint32_t)wend[1] << 16);
 
         // reverse row 0: V' = 0 on each pair's block columns (t = V' - o), else
         // -inf; only columns 0 .. kFirstMaxBlock - 1 hold a 0
         for (int j = 1; j <= kFirstMaxBlock; ++j) {
-            const uint32_t tl = j < anc[0] ? (T0 & 0xFFFFu) : 0x8000u;
-            const uint32_t th = j < anc[1] ? (T0 & 0xFFFF0000u) : 0x80000000u;
+            const uint32_t tl = j < anc[0] ? (T0 & 0xFFFFu) : (NEG2 & 0xFFFFu);
+            const uint32_t th = j < anc[1] ? (T0 & 0xFFFF0000u) : (NEG2 & 0xFFFF0000u);
             bnd[(int64_t)j * kWave + lane] = make_int2((int)(tl | th), (int)NEG2);
         }
 
@@ -2006,7 +2026,8 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
         // row that lies past the last live column of the row above (later
         // columns again have no live predecessor).  bnd holds the row above for
         // columns jlo .. jtop; later columns read -inf.
-        const uint32_t LIVE2 = splat16(min(0, o - e));
+        // live: V' >= min(0, o - e); MX3: t = V' - o + kRevBias >= kRevBias - max(o, e)
+        const uint32_t LIVE2 = MX3 ? splat16(kRevBias - max(o, e)) : splat16(min(0, o - e));
         int jlo = 1, jtop = kFirstMaxBlock, live_hi = kFirstMaxBlock - 1;
         uint32_t done = 0;  // per half: 0xFFFF once a row without live cells (or past bi) was seen
         uint32_t hitr = 0, hitc = 0;
@@ -2086,15 +2107,35 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const uint32_t sp = cur[r];
-                        const s16x2 G = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Gu), E2), v2(tu));
-                        const s16x2 H = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Hr[r]), E2),
-                                                                  v2(tr[r]));
-                        const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
-                        const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
-                        rm[r] = u2(__builtin_elementwise_max(v2(rm[r]), V));
-                        cm = __builtin_elementwise_max(cm, V);
-                        if (r == R - 1) lm = u2(__builtin_elementwise_max(v2(lm), V));
-                        const uint32_t tn = u2(__builtin_elementwise_sub_sat(V, O2));
+                        uint32_t tn;
+                        s16x2 G, H;
+                        if (MX3) {
+                            // unsigned saturating subtracts: the floor 0 is -inf
+                            G = __builtin_elementwise_max(
+                                __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                                    __builtin_bit_cast(u16x2, Gu), __builtin_bit_cast(u16x2, E2))), v2(tu));
+                            H = __builtin_elementwise_max(
+                                __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(
+                                    __builtin_bit_cast(u16x2, Hr[r]), __builtin_bit_cast(u16x2, E2))), v2(tr[r]));
+                            const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                            const uint32_t V = max3b(u2(f), u2(H), u2(G));
+                            tn = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(
+                                __builtin_bit_cast(u16x2, V), __builtin_bit_cast(u16x2, O2)));
+                            // maxima of t: the row's over two columns, the
+                            // column's over two rows (tu: the row above's t)
+                            if (q & 1) rm[r] = max3b(rm[r], tr[r], tn);
+                            if (r & 1) cm = v2(max3b(u2(cm), tu, tn));
+                            if (r == R - 1) lm = u2(__builtin_elementwise_max(v2(lm), v2(tn)));
+                        } else {
+                            G = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Gu), E2), v2(tu));
+                            H = __builtin_elementwise_max(__builtin_elementwise_sub_sat(v2(Hr[r]), E2), v2(tr[r]));
+                            const s16x2 f = __builtin_elementwise_add_sat(v2(dt), v2(sp));
+                            const s16x2 V = __builtin_elementwise_max(__builtin_elementwise_max(f, H), G);
+                            rm[r] = u2(__builtin_elementwise_max(v2(rm[r]), V));
+                            cm = __builtin_elementwise_max(cm, V);
+                            if (r == R - 1) lm = u2(__builtin_elementwise_max(v2(lm), V));
+                            tn = u2(__builtin_elementwise_sub_sat(V, O2));
+                        }
                         dt = tr[r];
                         tr[r] = tn;
                         Hr[r] = u2(H);
@@ -2493,42 +2534,42 @@ int rect64_blocks_per_cu(int ncode)
     return err == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, int nslots,
-                          hipStream_t st)
+hipError_t launch_reverse(const SeqSet& S, const ReverseSet& T, const Scratch& W, int strip_rows, bool mx3,
+                          int nslots, hipStream_t st)
 {
     const size_t lds = reverse_lds_bytes(S.ncode);
     const bool joint = S.ncode <= kRevJointMaxCodes;
+#define SWG_REVERSE(RR)                                                                                            \
+    if (joint && mx3) hipLaunchKernelGGL((reverse_kernel<RR, true, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W); \
+    else if (joint) hipLaunchKernelGGL((reverse_kernel<RR, true, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W); \
+    else if (mx3) hipLaunchKernelGGL((reverse_kernel<RR, false, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);  \
+    else hipLaunchKernelGGL((reverse_kernel<RR, false, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
     switch (strip_rows) {
-    case 8:
-        if (joint) hipLaunchKernelGGL((reverse_kernel<8, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        else hipLaunchKernelGGL((reverse_kernel<8, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        break;
-    case 16:
-        if (joint) hipLaunchKernelGGL((reverse_kernel<16, true>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        else hipLaunchKernelGGL((reverse_kernel<16, false>), dim3(nslots), dim3(kWave), lds, st, S, T, W);
-        break;
+    case 8: SWG_REVERSE(8) break;
+    case 16: SWG_REVERSE(16) break;
     default: return hipErrorInvalidValue;
     }
+#undef SWG_REVERSE
     return hipGetLastError();
 }
 
-int reverse_blocks_per_cu(int strip_rows, int ncode)
+int reverse_blocks_per_cu(int strip_rows, int ncode, bool mx3)
 {
     const size_t lds = reverse_lds_bytes(ncode);
     const bool joint = ncode <= kRevJointMaxCodes;
     int nb = 0;
     hipError_t err = hipErrorInvalidValue;
+#define SWG_REVERSE_OCC(RR)                                                                                        \
+    err = joint && mx3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<RR, true, true>, kWave, lds)  \
+        : joint        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<RR, true, false>, kWave, lds) \
+        : mx3          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<RR, false, true>, kWave, lds) \
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<RR, false, false>, kWave, lds);
     switch (strip_rows) {
-    case 8:
-        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<8, true>, kWave, lds)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<8, false>, kWave, lds);
-        break;
-    case 16:
-        err = joint ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16, true>, kWave, lds)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reverse_kernel<16, false>, kWave, lds);
-        break;
+    case 8: SWG_REVERSE_OCC(8) break;
+    case 16: SWG_REVERSE_OCC(16) break;
     default: break;
     }
+#undef SWG_REVERSE_OCC
     return err == hipSuccess ? nb : 0;
 }
 

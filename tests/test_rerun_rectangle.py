@@ -264,3 +264,145 @@ def test_prefix_rectangle_is_exact(ednafull, o, e):
         assert (sub["best_row"], sub["best_col"]) == (full["best_row"], full["best_col"])
         for k in ("columns", "identity", "similarity", "gaps"):
             assert sub[k] == full[k], (k, s1, s2)
+
+
+# ---- the reverse pass's max3 form: values biased by B and kept in
+# [0, 0x7BFF] with unsigned saturating subtracts (a clamped value is dead and,
+# the band's rows being capped, cannot climb back to life), three-way maxima
+# on the int16 patterns read as fp16; the row and column maxima are taken in
+# the t = V' - o domain ----
+
+from tests.test_score_pass_model import _add_sat, _max3_f16  # noqa: E402
+
+REV_BIAS = 16384
+
+
+def _subsat_u16(a, b):
+    return max(0, a - b)
+
+
+def reverse_start_banded_max3(s1, s2, M, o, e, bi, jb, best, R, BLOCK, row_cap=512):
+    """reverse_start_banded in the kernel's max3 arithmetic; a pair whose
+    score does not fit the biased range takes its whole prefix."""
+    B = REV_BIAS
+    live_t = B - max(o, e)              # V' >= min(0, o - e)  <=>  t >= B - max(o, e)
+    hit_t = best + B - o                 # V' == S
+    wend = min(jb + BLOCK - 1, len(s2))
+    if best + B > 0x7BFF:
+        return 1, 1
+    anc = wend - jb + 1
+    r1 = s1[:bi][::-1]
+    r2 = s2[:wend][::-1]
+    ncols = (wend + 3) // 4 * 4
+    top_t = [0] * (ncols + 1)            # -inf is 0
+    top_g = [0] * (ncols + 1)
+    for c in range(anc):
+        top_t[c] = B - o                 # V' = 0
+    jlo, jtop, live_hi = 1, BLOCK, BLOCK - 1
+    hitr = hitc = 0
+    seen_dead = False
+    rows_done = 0
+    for i0 in range(0, min(bi, row_cap), R):
+        rows = range(i0 + 1, i0 + R + 1)
+        left_t = {i: 0 for i in rows}
+        left_h = {i: 0 for i in rows}
+        rmax = {i: 0 for i in rows}
+        new_t = [0] * (ncols + 1)
+        new_g = [0] * (ncols + 1)
+        diag_top = top_t[0] if (i0 == 0 and jlo == 1) else 0
+        lo_next = hi_next = 0
+        j = jlo
+        while j <= ncols:
+            group_live = last_live = False
+            for jj in range(j, j + 4):
+                up_t = top_t[jj] if jj <= jtop else 0
+                up_g = top_g[jj] if jj <= jtop else 0
+                diag = diag_top
+                diag_top = up_t
+                for i in rows:
+                    sp = (int(M[r1[i - 1]][r2[jj - 1]]) + o) if (i <= bi and jj <= wend) else -0x8000
+                    assert sp >= -0x8000
+                    g = max(_subsat_u16(up_g, e), up_t)
+                    h = max(_subsat_u16(left_h[i], e), left_t[i])
+                    f = _add_sat(diag, sp)
+                    v = _max3_f16(f, h, g)
+                    assert 0 <= v <= 0x7BFF
+                    t = _subsat_u16(v, o)
+                    if t == hit_t and i <= bi and jj <= wend:
+                        hitr, hitc = max(hitr, i), max(hitc, jj)
+                    assert t <= hit_t
+                    rmax[i] = max(rmax[i], t)
+                    group_live |= t >= live_t
+                    if i == rows[-1]:
+                        last_live |= t >= live_t
+                    diag = left_t[i]
+                    left_t[i], left_h[i] = t, h
+                    up_t, up_g = t, g
+                new_t[jj], new_g[jj] = up_t, up_g
+            jend = j + 3
+            if last_live:
+                lo_next = lo_next or j
+                hi_next = j + 3
+            if not group_live and j + 3 > live_hi:
+                break
+            j += 4
+        rows_done = i0 + R
+        for i in rows:
+            seen_dead |= rmax[i] < live_t or i > bi
+        if lo_next == 0 or seen_dead:
+            break
+        jlo, live_hi, jtop = lo_next, hi_next, jend
+        top_t, top_g = new_t, new_g
+    capped = bi > rows_done and not seen_dead
+    if hitr == 0 or capped:
+        return 1, 1
+    return bi - hitr + 1, wend - hitc + 1
+
+
+@pytest.mark.parametrize("R", [2, 16])
+def test_max3_form_reverse_pass_finds_the_same_rectangle(ednafull, blosum62, R):
+    """The biased, clamped max3 arithmetic finds the exact reverse DP's
+    rectangle start on random related and unrelated pairs, low complexity
+    and protein, at several penalties (and a row cap small enough to stop
+    some bands, which then take the whole prefix, like the exact pass)."""
+    import random
+    from tests.test_oracle import random_related
+    rng = random.Random(43)
+    cases = []
+    for _ in range(40):
+        a, b = random_related(rng, rng.randint(4, 90))
+        cases.append((a, b, ednafull, rng.choice([16, 10, 5, 4]), rng.choice([4, 1, 2, 0])))
+    aa = b"ARNDCQEGHILKMFPSTWYV"
+    for _ in range(12):
+        a = bytes(rng.choice(aa) for _ in range(rng.randint(4, 70)))
+        b = bytes(rng.choice(aa) for _ in range(rng.randint(4, 70)))
+        cases.append((a, b, blosum62, 10, 1))
+    cases += [(b"A" * 60, b"A" * 50, ednafull, 16, 4), (b"AC" * 30, b"CA" * 31, ednafull, 4, 0)]
+    # multi-residue gaps in both sequences, so that the optimal paths run
+    # through vertical and horizontal gap chains (extension, not opening)
+    for k in range(6):
+        x = bytes(rng.choice(b"ACGT") for _ in range(80))
+        g1, g2 = 2 + k % 3, 3 + k % 4
+        # unrelated prefixes, so that the alignment starts inside the matrix
+        ja = bytes(rng.choice(b"ACGT") for _ in range(7 + k))
+        jb_ = bytes(rng.choice(b"ACGT") for _ in range(12 - k))
+        a = ja + x[:25] + x[25 + g1:]            # a deletion in s1
+        b = jb_ + x[:50] + bytes(rng.choice(b"ACGT") for _ in range(g2)) + x[50:]   # an insertion in s2
+        for o, e in ((16, 4), (10, 1), (6, 2)):
+            cases += [(a, b, ednafull, o, e), (b, a, ednafull, o, e)]
+    for a, b, M, o, e in cases:
+        r = O.sw_literal(a, b, M, o, e)
+        best, bi = int(r["score"]), r["best_row"]
+        if best <= 0:
+            continue
+        jb = ((r["best_col"] - 1) // KBLOCK) * KBLOCK + 1
+        for cap in (512, 8):
+            want = reverse_start_banded(a, b, M, o, e, bi, jb, best, R, KBLOCK) if cap >= bi else None
+            got = reverse_start_banded_max3(a, b, M, o, e, bi, jb, best, R, KBLOCK, row_cap=cap)
+            if want is not None:
+                assert got == want, (a, b, o, e, got, want)
+            # whatever the cap, the rectangle holds the full DP's traceback
+            si, sj = got
+            wend = min(jb + KBLOCK - 1, len(b))
+            sub = O.sw_literal(a[si - 1:bi], b[sj - 1:wend], M, o, e)
+            assert int(sub["score"]) == best
