@@ -94,3 +94,35 @@ def test_penalties_outside_the_form_run_the_plain_form(ednafull, o, e):
     want, wst = oracle_block(s, *blk, False, ednafull, o, e)
     got, gst = run(s, ednafull, o, e, blk, True)
     assert_same(got, want, gst, wst, "o=%d e=%d" % (o, e))
+
+
+def test_open_zero_and_extension_above_opening(ednafull):
+    """A matrix with no negative score (match 3, mismatch 0) takes the max3
+    form even at gap_open 0 (pads then tie real cells, which come first in
+    row-major order), and with gap_ext above gap_open."""
+    say("max3 form: gap_open 0")
+    M = np.where(np.eye(ednafull.shape[0], dtype=bool), 3, 0).astype(ednafull.dtype)
+    s = ca.synth_dna(74, 200, 5, 0.15, 0.04, 0, [1, 400])
+    blk = (0, 69, 70, 199)
+    for o, e in ((0, 0), (0, 2), (1, 3)):
+        want, wst = oracle_block(s, *blk, False, M, o, e)
+        for max3 in (True, False):
+            got, gst = run(s, M, o, e, blk, max3)
+            assert_same(got, want, gst, wst, "o=%d e=%d max3 %s" % (o, e, max3))
+
+
+def test_reverse_row_cap_outside_its_biased_form(ednafull):
+    """A reverse row cap of 4,096 rows lets a clamped value climb 4,096 x 5
+    > 16,384: the reverse pass then runs its plain form while the score pass
+    keeps its max3 form; both caps give the oracle's distances."""
+    say("max3 form: reverse row cap 4096")
+    s = ca.synth_dna(75, 300, 3, 0.08, 0.02, 0, [600, 2500])
+    blk = (0, 49, 50, 299)
+    want, wst = oracle_block(s, *blk, False, ednafull, 16, 4)
+    for cap in (4096, 512):
+        al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
+        al.set_reverse_row_cap(cap)
+        al.set_sequences(s)
+        got, gst, _ = al.compute_block(*blk, False, stats=True)
+        al.close()
+        assert_same(got, want, gst, wst, "row cap %d" % cap)
