@@ -797,20 +797,18 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
     return B;
 }
 
-// Autotuner: per key (residue alphabet, mean-length class, block-size class)
-// the faster of the int32 one-pass and the int16 two-pass variants.  The
-// first blocks of a key run each candidate once, synchronously, on real work
-// (nothing is recomputed); the variant with the higher cells/ms wins.
 // Autotuner.  Per key (residue alphabet, mean-length class, block-size
 // class) the first blocks of the key run candidate configurations, each once
 // and synchronously, on real work (nothing is recomputed); the one with the
 // highest cells/ms wins.  The search is staged, one dimension at a time:
 //   stage 0  the variant at its default tile shape: int16 two-pass (score
-//            strip 24, re-run strip 16), int32 one-pass (strip 16), 16 lanes
-//            per pair (4 rows per lane) -- lanes per pair 1 or 16, int16 or
-//            int32; the two-pass first for alphabets of <= 8 codes, the
-//            one-pass first for larger ones, so that a run of a single block
-//            gets the variant that measured faster on such data;
+//            strip 32 in the max3 form, else 24; re-run strip 16), int32
+//            one-pass (strip 16), 16 lanes per pair (4 rows per lane) --
+//            lanes per pair 1 or 16, int16 or int32; the two-pass first for
+//            alphabets of <= 8 codes, the one-pass first for larger ones, so
+//            that a run of a single block gets the variant that measured
+//            faster on such data (the host entry point times the variants
+//            after the first on trial chunks, see trial_rows);
 //   stage 1  the winner's first strip: score strips 24, 32 and 40 (16, never
 //            faster on measured data, only when fixed), int32 strips 8 and 32,
 //            rows per lane 8 <-> 4, the global kernel's 8 <-> 16;
