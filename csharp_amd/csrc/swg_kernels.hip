@@ -1331,6 +1331,7 @@ __device__ __forceinline__ int64_t row_pairs(const BlockDesc& D1, bool lower, in
     return n < 0 ? 0 : (n < D1.ld ? n : D1.ld);
 }
 
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
 #pragma unroll
@@ -1349,6 +1350,25 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 // serialise on them.
 constexpr int kSortThreads = 512;
 
+// The sort's work items: a row's pairs in chunks of kSortChunk, so that a
+// block of a few rows (a serving query against its panel) spreads over the
+// GPU's workgroups; fn(row, column) for each pair of this workgroup's items,
+// in the same order for every call within a workgroup.
+constexpr int kSortChunk = 2048;
+template <typename F>
+__device__ __forceinline__ void for_each_sort_pair(const BlockDesc& D1, bool lower, F&& fn)
+{
+    const int64_t nchunk = (D1.ld + kSortChunk - 1) / kSortChunk;
+    const int64_t nitems = (int64_t)D1.rows * nchunk;
+    for (int64_t it = blockIdx.x; it < nitems; it += (int64_t)gridDim.x) {
+        const int row = (int)(it / nchunk);
+        const int64_t c_lo = (it - (int64_t)row * nchunk) * kSortChunk;
+        const int64_t n = row_pairs(D1, lower, row);
+        const int64_t c_hi = n < c_lo + kSortChunk ? n : c_lo + kSortChunk;
+        for (int64_t c = c_lo + threadIdx.x; c < c_hi; c += kSortThreads) fn(row, c);
+    }
+}
+
 __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
 {
     __shared__ int hist[kNumBuckets];
@@ -1357,29 +1377,38 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
     for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads) hist[k] = 0;
     __syncthreads();
     uint64_t rect = 0, flagged = 0, big = 0;
-    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
-        const int64_t n = row_pairs(D1, lower, row);
-        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
-            uint64_t cells;
-            bool fl;
-            const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
-            atomicAdd(&hist[k], 1);
-            if (fl) flagged += cells;
-            else if (k == kBigBucket) big += cells;
-            else rect += cells;
-        }
-    }
+    for_each_sort_pair(D1, lower, [&](int row, int64_t c) {
+        uint64_t cells;
+        bool fl;
+        const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
+        atomicAdd(&hist[k], 1);
+        if (fl) flagged += cells;
+        else if (k == kBigBucket) big += cells;
+        else rect += cells;
+    });
     __syncthreads();
     for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads)
         if (hist[k]) atomicAdd(&P.count[k], hist[k]);
+    // the cell totals: summed over the workgroup first, then one atomic per
+    // workgroup and total, none for a zero (every wave of every workgroup
+    // adding to the same three words serialised on them: 165 us of a
+    // 20,000-pair block's sort)
+    __shared__ uint64_t wsum[3][kSortThreads / kWave];
     rect = wave_sum_u64(rect);
     flagged = wave_sum_u64(flagged);
     big = wave_sum_u64(big);
     if ((threadIdx.x & (kWave - 1)) == 0) {
+        wsum[0][threadIdx.x / kWave] = rect;
+        wsum[1][threadIdx.x / kWave] = flagged;
+        wsum[2][threadIdx.x / kWave] = big;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+        for (int w = 0; w < kSortThreads / kWave; ++w) t += wsum[threadIdx.x][w];
         unsigned long long* m = reinterpret_cast<unsigned long long*>(P.meta);
-        atomicAdd(&m[kMetaRectCells], (unsigned long long)rect);
-        atomicAdd(&m[kMetaFlaggedCells], (unsigned long long)flagged);
-        atomicAdd(&m[kMetaBigCells], (unsigned long long)big);
+        const int slot = threadIdx.x == 0 ? kMetaRectCells : threadIdx.x == 1 ? kMetaFlaggedCells : kMetaBigCells;
+        if (t) atomicAdd(&m[slot], (unsigned long long)t);
     }
 }
 
@@ -1463,28 +1492,22 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(SortSet P)
     }
     for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads) hist[k] = 0;
     __syncthreads();
-    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
-        const int64_t n = row_pairs(D1, lower, row);
-        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
-            uint64_t cells;
-            bool fl;
-            atomicAdd(&hist[pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl)], 1);
-        }
-    }
+    for_each_sort_pair(D1, lower, [&](int row, int64_t c) {
+        uint64_t cells;
+        bool fl;
+        atomicAdd(&hist[pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl)], 1);
+    });
     __syncthreads();
     for (int k = threadIdx.x; k < kNumBuckets; k += kSortThreads)
         if (hist[k]) hist[k] = P.base[k] + atomicAdd(&P.cursor[k], hist[k]);
     __syncthreads();
-    for (int row = blockIdx.x; row < D1.rows; row += (int)gridDim.x) {
-        const int64_t n = row_pairs(D1, lower, row);
-        const int64_t first = (int64_t)row * D1.ld;
-        for (int64_t c = threadIdx.x; c < n; c += kSortThreads) {
-            uint64_t cells;
-            bool fl;
-            const int k = pair_bucket(P, D1, first + c, P.len[D1.c0 + c], &cells, &fl);
-            P.sorted[atomicAdd(&hist[k], 1)] = (uint32_t)(first + c);
-        }
-    }
+    for_each_sort_pair(D1, lower, [&](int row, int64_t c) {
+        uint64_t cells;
+        bool fl;
+        const int64_t pos = (int64_t)row * D1.ld + c;
+        const int k = pair_bucket(P, D1, pos, P.len[D1.c0 + c], &cells, &fl);
+        P.sorted[atomicAdd(&hist[k], 1)] = (uint32_t)pos;
+    });
 }
 
 // Each lane aligns its own pair: s1 differs per lane, so the substitution
@@ -1685,7 +1708,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
         }
         run += (uint64_t)Rmax * (uint64_t)ncols * kWave;
     }
-    if (lane == 0)
+    if (lane == 0 && run)
         atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + (TRACK ? kMetaFlaggedRun : kMetaCaptureRun),
                   (unsigned long long)run);
 }
@@ -1853,7 +1876,7 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
         }
         if (lane == 0) write_distance(S, D[0], a, b, best, bestT);
     }
-    if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaBigRun, (unsigned long long)run);
+    if (lane == 0 && run) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaBigRun, (unsigned long long)run);
 }
 
 // ---------------------------------------------------------------------------
@@ -2220,7 +2243,7 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
         }
     }
     // once, after the task loop (see rerun_kernel)
-    if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
+    if (lane == 0 && run) atomicAdd(reinterpret_cast<unsigned long long*>(T.meta) + kMetaCaptureRun, (unsigned long long)run);
 }
 
 // Jukes-Cantor, Kimura2 and MinMaxNormScore distances of a block from the
