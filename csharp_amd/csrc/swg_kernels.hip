@@ -119,6 +119,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SeqSet S, TaskSet T, Scratch W)
     const int lane = threadIdx.x;
     int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
     const int ntasks = T.desc->ntasks;
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     const int o = S.gap_open, e = S.gap_ext;
     const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
@@ -293,6 +294,7 @@ __global__ __launch_bounds__(64) void lpp16_kernel(SeqSet S, TaskSet T, Scratch 
     const int g = lane >> 4, k = lane & 15;
     int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
     const int ntasks = T.desc->ntasks;
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     const int o = S.gap_open, e = S.gap_ext;
     const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
@@ -454,6 +456,7 @@ __global__ __launch_bounds__(64) void nw_kernel(SeqSet S, TaskSet T, Scratch W)
     int4* __restrict__ bnd = W.bnd + (int64_t)blockIdx.x * W.bnd_stride;
     int2* __restrict__ bnd2 = reinterpret_cast<int2*>(bnd + W.bnd_stride / 2);
     const int ntasks = T.desc->ntasks;
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     const int o = S.gap_open, e = S.gap_ext;
     const int ginc = S.gap_inc;  // count-word increment per gap column
     const int pad = S.ncode - 1;
@@ -763,6 +766,7 @@ score16_kernel(SeqSet S, ScoreTaskSet T, Scratch W)
     int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
     const BlockDesc* D = T.desc;
     const int ntasks = D->ntasks;
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     const int o = S.gap_open, e = S.gap_ext;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
     const uint32_t NEG2 = splat16(kPad16), M1 = splat16(-1);
@@ -1090,6 +1094,7 @@ __global__ __launch_bounds__(64) void lps16_kernel(SeqSet S, ScoreTaskSet T, Scr
     int2* __restrict__ bnd = reinterpret_cast<int2*>(W.bnd + (int64_t)blockIdx.x * W.bnd_stride);
     const BlockDesc* D = T.desc;
     const int ntasks = D->ntasks;
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     const int o = S.gap_open, e = S.gap_ext;
     const s16x2 O2 = v2(splat16(o)), E2 = v2(splat16(e));
     const uint32_t NEG2 = splat16(kPad16), M1 = splat16(-1);
@@ -1545,6 +1550,7 @@ __global__ __launch_bounds__(64) void rerun_kernel(SeqSet S, RerunSet T, Scratch
     const int ginc = S.gap_inc;  // count-word increment per gap column
     const int t_lo = TRACK ? 0 : T.meta[kMetaBigTiles];
     const int t_hi = TRACK ? T.meta[kMetaFlaggedTiles] : T.meta[kMetaTiles];
+    if ((int)blockIdx.x >= t_hi - t_lo) return;  // more workgroups than tiles: the first ones drain the queue
     // cells evaluated, added to meta once after the task loop: a second
     // lane-0 branch inside the loop lets the compiler merge it with the
     // queue's and leave lanes 1..63 in the loop without lane 0
@@ -1754,6 +1760,7 @@ __global__ __launch_bounds__(64) void rect64_kernel(SeqSet S, RerunSet T, Scratc
     const int pad = S.ncode - 1;
     const int64_t s_lo = (int64_t)T.meta[kMetaFlaggedTiles] * kWave;
     const int64_t s_hi = (int64_t)T.meta[kMetaBigTiles] * kWave;
+    if ((int64_t)blockIdx.x >= s_hi - s_lo) return;  // more workgroups than pairs: the first ones drain the queue
     uint64_t run = 0;  // cells evaluated, added to meta once after the task loop
 
     for (;;) {
@@ -1972,6 +1979,7 @@ void reverse_kernel(SeqSet S, ReverseSet T, Scratch W)
     const uint32_t NEG2 = MX3 ? 0u : 0x80008000u, T0 = MX3 ? splat16(kRevBias - o) : splat16(-o);
     const int64_t slot0 = (int64_t)T.meta[kMetaFlaggedTiles] * kWave, slot1 = (int64_t)T.meta[kMetaTiles] * kWave;
     const int ntasks = (int)((slot1 - slot0 + 2 * kWave - 1) / (2 * kWave));
+    if ((int)blockIdx.x >= ntasks) return;  // more workgroups than tasks: the first ntasks drain the queue
     uint64_t run = 0;  // cells evaluated, added to meta after the task loop
 
     for (;;) {
