@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--strip", type=int, default=0, help="0 = autotuned strip height")
     ap.add_argument("--score-strip", type=int, default=0, help="int16 score-pass strip rows (0 = autotuned)")
     ap.add_argument("--reverse-strip", type=int, default=0, help="reverse-pass strip rows (0 = the default)")
+    ap.add_argument("--stream-chunk-mib", type=int, default=0,
+                    help="c4: device output per streamed row chunk in MiB (0 = the library's default, 64)")
     ap.add_argument("--score-max3", type=int, default=1, choices=[0, 1],
                     help="1: the score pass's max3 form where the set allows it (the default); 0: its plain form")
     ap.add_argument("--full-pairs-per-block", type=int, default=200,
@@ -433,6 +435,8 @@ def workload_run(args):
     al = ca.Aligner(M, o, e, alignment_type=cfg["alignment_type"], tdistance=cfg["tdistance"],
                     mode=ca.MODE_NW_GLOBAL if glob else ca.MODE_SW_LOCAL)
     al.set_sequences(seqs)
+    if args.stream_chunk_mib:
+        al.set_stream_chunk_bytes(args.stream_chunk_mib << 20)
     if args.schedule != "auto":  # a forced schedule, for comparison with the autotuned one
         al.set_schedule({"i32": ca.SCHED_PERSISTENT_I32, "i16": ca.SCHED_PERSISTENT_I16,
                          "lpp16": ca.SCHED_PERSISTENT_LPP16, "lps16": ca.SCHED_PERSISTENT_LPS16,
@@ -511,10 +515,10 @@ def workload_run(args):
         G = np.concatenate([x[2] for x in samples])
         d, dt, th = oracle_sample(seqs, A, B, M, o, e, nucleic, tdouble)
         # after the timed region: one row chunk's worth of the second block
-        # (as many rows as fit a 64 MiB chunk) in kernel-timing mode, for the
+        # (as many rows as fit a chunk) in kernel-timing mode, for the
         # per-kernel split of a streamed chunk
         c0, c1 = ranges[1]
-        chunk_rows = max(1, (64 << 20) // ((c1 - c0 + 1) * np.dtype(np.int16).itemsize))
+        chunk_rows = max(1, ((args.stream_chunk_mib or 64) << 20) // ((c1 - c0 + 1) * np.dtype(np.int16).itemsize))
         al.set_kernel_timing(True)
         _, _, cpairs = al.compute_block(r0, min(r1, r0 + chunk_rows - 1), c0, c1, False)
         ckms, ccells, _, _ = al.last_kernel_stats()
