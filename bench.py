@@ -63,7 +63,7 @@ def log(msg):
 # kernel variants (SWG_SCHED_* values of swg_block_config.variant)
 VNAME = {1: "plain", 2: "int32 one-pass", 3: "int16 score pass + sorted int32 re-run", 4: "16 lanes per pair",
          5: "int16 score pass with 16 lanes per pair + sorted int32 re-run"}
-TUNING = {0: "fixed", 1: "autotuner trial", 2: "autotuner choice"}
+TUNING = {0: "fixed", 1: "autotuner trial", 2: "autotuner choice", 3: "fastest so far"}
 
 
 def cfg_name(cfg):
@@ -624,7 +624,7 @@ def full_c3_run(args):
             a = rng.integers(r0, r1 + 1, size=m)
             b = rng.integers(c0, c1 + 1, size=m)
         samples.append((a, b, out[a - r0, b - c0].copy()))
-        if k % 8 == 0 or k == len(todo) - 1:
+        if k % 8 == 0 or diag or k == len(todo) - 1:
             log("c3full block %d/%d (%d, %d)%s: %.1f GCUPS, %.1f s elapsed" % (
                 k + 1, len(todo), r, j, " diagonal" if diag else "", cells / dt / 1e9, time.perf_counter() - t0))
     wall = time.perf_counter() - t0

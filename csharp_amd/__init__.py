@@ -388,7 +388,9 @@ class Aligner:
     def last_block_config(self):
         """Launch configuration of the last block: variant (SWG_SCHED_*), strip
         rows, re-run strip rows, pairs per wave, tuning state (0 fixed, 1
-        autotuner trial, 2 autotuner choice) and the key's timed trials."""
+        autotuner trial, 2 autotuner choice, 3 the fastest configuration
+        timed so far: a diagonal block while its key's search goes on) and
+        the key's timed trials."""
         cfg = swg_block_config()
         _check(lib().swg_last_block_config(self._h, ctypes.byref(cfg)))
         return cfg.as_dict()

@@ -184,7 +184,8 @@ struct VariantCfg {
 struct TuneEntry {
     // stage: the search stage the trial belonged to (0: the variants)
     struct Trial { VariantCfg cfg; double rate; int stage; };
-    int64_t key = 0;
+    int64_t key = 0;       // residue alphabet and block-size class
+    double lclass = 0.0;   // length class of the key's first block (see find_entry)
     int stage = 0;
     bool done = false;
     VariantCfg best{};
@@ -280,7 +281,7 @@ struct swg_ctx {
     int32_t last_strip = 0;
     int32_t last_variant = 0;
     VariantCfg last_cfg{};
-    int32_t last_tuning = 0;   // 0 fixed, 1 autotuner trial, 2 autotuner choice
+    int32_t last_tuning = 0;   // 0 fixed, 1 autotuner trial, 2 autotuner choice, 3 fastest so far
     int32_t last_trials = 0;   // trials measured for the last block's key
 
     // kernel-timing mode: a stamp kernel before the block's kernels and after
@@ -800,7 +801,9 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
 // Autotuner.  Per key (residue alphabet, mean-length class, block-size
 // class) the first blocks of the key run candidate configurations, each once
 // and synchronously, on real work (nothing is recomputed); the one with the
-// highest cells/ms wins.  The search is staged, one dimension at a time:
+// highest cells/ms wins.  A lower-mirror diagonal block runs a candidate only
+// as its key's first block; later ones, while the search goes on, take the
+// fastest configuration timed so far.  The search is staged, one dimension at a time:
 //   stage 0  the variant at its default tile shape: int16 two-pass (score
 //            strip 32 in the max3 form, else 24; re-run strip 16), int32
 //            one-pass (strip 16), 16 lanes per pair (4 rows per lane) --
@@ -818,18 +821,47 @@ BlockBufs& next_bufs(Lane& ln, int* rc)
 
 int64_t tune_key(const swg_ctx* c, const BlockGeom& g)
 {
-    const std::vector<int32_t>& L = c->h_len;
-    double sum = 0.0;
-    int64_t cnt = 0;
-    for (int32_t i = g.r0; i <= g.r1; ++i) { sum += L[i]; ++cnt; }
-    for (int32_t i = g.c0; i <= g.c1; ++i) { sum += L[i]; ++cnt; }
-    const int len_class = cnt ? (int)std::lround(2.0 * std::log2(1.0 + sum / (double)cnt)) : 0;
     // two size classes only: the host entry point splits a big block into
     // row chunks of different sizes, and each chunk should not re-open the
     // trials (the first chunks time one variant each, the rest use the faster)
     const double pairs = (double)(g.r1 - g.r0 + 1) * (double)(g.c1 - g.c0 + 1) * (g.diag_lower ? 0.5 : 1.0);
     const int size_class = pairs >= 4e6 ? 1 : 0;
-    return ((int64_t)c->ncode << 40) | ((int64_t)len_class << 20) | (int64_t)size_class;
+    return ((int64_t)c->ncode << 40) | (int64_t)size_class;
+}
+
+// 2 log2(1 + mean length) over the block's row and column sequences: one
+// unit is a factor of sqrt(2) in length.
+double length_class(const swg_ctx* c, const BlockGeom& g)
+{
+    const std::vector<int32_t>& L = c->h_len;
+    double sum = 0.0;
+    int64_t cnt = 0;
+    for (int32_t i = g.r0; i <= g.r1; ++i) { sum += L[i]; ++cnt; }
+    for (int32_t i = g.c0; i <= g.c1; ++i) { sum += L[i]; ++cnt; }
+    return cnt ? 2.0 * std::log2(1.0 + sum / (double)cnt) : 0.0;
+}
+
+// The autotuner record of block g: among the records of its alphabet and
+// size class, the one whose first block's length class is nearest, if less
+// than half a unit away; else a new record (create) centred on this block.
+// Classes centred on the blocks seen, not fixed bins, keep blocks of nearly
+// equal mean length together: the 16 row ranges of a 50,000-sequence set
+// have means of 833-865 residues, across the bin edge 2 log2(1 + m) = 19.5.
+TuneEntry* find_entry(swg_ctx* c, const BlockGeom& g, bool create)
+{
+    const int64_t key = tune_key(c, g);
+    const double lc = length_class(c, g);
+    TuneEntry* t = nullptr;
+    double dist = 0.5;
+    for (auto& e : c->tuned)
+        if (e.key == key && std::fabs(e.lclass - lc) < dist) { t = &e; dist = std::fabs(e.lclass - lc); }
+    if (!t && create) {
+        c->tuned.emplace_back();
+        t = &c->tuned.back();
+        t->key = key;
+        t->lclass = lc;
+    }
+    return t;
 }
 
 
@@ -937,18 +969,19 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* c
     if (c->schedule == SWG_SCHED_PERSISTENT_I32) { *cfg = default_cfg(c, kVarI32); return SWG_OK; }
     if (c->schedule == SWG_SCHED_PERSISTENT_I16 && c->int16_ok) { *cfg = default_cfg(c, kVarI16); return SWG_OK; }
     if (!c->nw && thin_block(c, ln, g)) { *cfg = c->lps_ok ? lps_cfg(c, ln, g) : default_cfg(c, kVarLpp); return SWG_OK; }
-    const int64_t key = tune_key(c, g);
-    TuneEntry* t = nullptr;
-    for (auto& e : c->tuned)
-        if (e.key == key) t = &e;
-    if (!t) {
-        c->tuned.emplace_back();
-        t = &c->tuned.back();
-        t->key = key;
-    }
+    TuneEntry* t = find_entry(c, g, true);
     tune_advance(c, *t);
     *entry = t;
     if (t->done) { *cfg = t->best; return SWG_OK; }
+    if (g.diag_lower && !t->trials.empty()) {
+        // a lower-mirror triangle cannot be split into trial chunks: rather
+        // than run a candidate that may be several times slower on all of
+        // it, it takes the fastest configuration timed so far
+        double best_rate = -1.0;
+        for (const auto& x : t->trials)
+            if (x.rate > best_rate) { best_rate = x.rate; *cfg = x.cfg; }
+        return SWG_OK;
+    }
     *cfg = t->pending.front();
     *trial = true;
     return SWG_OK;
@@ -960,10 +993,8 @@ int choose_variant(swg_ctx* c, const Lane& ln, const BlockGeom& g, VariantCfg* c
 bool tuning_pending(const swg_ctx* c, const Lane& ln, const BlockGeom& g)
 {
     if (c->schedule != SWG_SCHED_PERSISTENT || (!c->nw && thin_block(c, ln, g))) return false;
-    const int64_t key = tune_key(c, g);
-    for (const auto& e : c->tuned)
-        if (e.key == key) return !e.done && e.stage <= 1 && !e.trials.empty();
-    return false;
+    const TuneEntry* e = find_entry(const_cast<swg_ctx*>(c), g, false);
+    return e && !e->done && e->stage <= 1 && !e->trials.empty();
 }
 
 // Rows of a trial chunk: the variants after the first one (the expected
@@ -1015,7 +1046,7 @@ int enqueue_block(swg_ctx* c, Lane& ln, int32_t r0, int32_t r1, int32_t c0, int3
         entry->pending.erase(entry->pending.begin());
         tune_advance(c, *entry);
     }
-    c->last_tuning = trial ? 1 : entry ? 2 : 0;
+    c->last_tuning = trial ? 1 : !entry ? 0 : entry->done ? 2 : 3;
     c->last_trials = entry ? (int32_t)entry->trials.size() : 0;
     if (pairs_done) *pairs_done = plan.pairs;
     return SWG_OK;

@@ -243,9 +243,12 @@ int  swg_last_kernel_stats(swg_ctx* ctx, double* kernel_ms, double* cells, int32
  * key (residue alphabet, mean-length class, block-size class), on the first
  * blocks of the key (each run once, on its real work): the variant (int16
  * two-pass, int32 one-pass, 16 lanes per pair) at its default strips, then
- * the winner's first strip (score pass 24/32/16, one-pass 8/16/32, rows per lane
+ * the winner's first strip (score pass 24/32/40, one-pass 8/16/32, rows per lane
  * 4/8, global kernel 8/16), then the two-pass re-run strip (16/32); the
- * highest cells/ms wins.  Thin blocks (too few rows to fill the GPU one lane
+ * highest cells/ms wins.  A lower-mirror diagonal block, which cannot be
+ * split into trial chunks, runs a candidate only as its key's first block,
+ * later ones the fastest configuration timed so far.  Mean-length classes are
+ * a factor of sqrt(2) wide, centred on the first block of each class.  Thin blocks (too few rows to fill the GPU one lane
  * per pair) always take 16 lanes per pair: the int16 score pass so
  * (_PERSISTENT_LPS16) where its alphabet and scores allow, else the int32
  * kernel (_PERSISTENT_LPP16). */
@@ -258,7 +261,8 @@ typedef struct swg_block_config {
     int32_t pairs_per_wave;   /* 128 score pass (two pairs per lane), 64 one-pass / global, 4 sixteen
                                  lanes per pair, 8 score pass with sixteen lanes per pair */
     int32_t tuning;           /* 0 fixed by the schedule (or a thin block), 1 an autotuner trial,
-                                 2 the autotuner's choice */
+                                 2 the autotuner's choice, 3 the fastest configuration timed so
+                                 far (a diagonal block while its key's search goes on) */
     int32_t trials;           /* configurations the autotuner has timed for this block's key */
 } swg_block_config;
 int  swg_last_block_config(swg_ctx* ctx, swg_block_config* out);

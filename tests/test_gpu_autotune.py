@@ -173,3 +173,59 @@ def test_trial_chunks_of_a_large_block(ednafull):
             break
     assert al.last_block_config()["tuning"] == 2
     al.close()
+
+
+def test_length_classes_and_diagonal_blocks(ednafull):
+    """Two blocks whose mean lengths straddle the edge of a fixed length bin
+    (2 log2(1 + m) = 11.5) share one autotuner record; a lower-mirror
+    diagonal block runs a candidate only as its record's first block, later
+    ones the fastest configuration timed so far; all bit-exact on seeded
+    samples."""
+    say("length classes and diagonal blocks")
+    rng = np.random.default_rng(8)
+    lens = np.concatenate([rng.integers(40, 63, 3000), rng.integers(48, 71, 3000), rng.integers(40, 71, 256)])
+    seqs = [rng.choice(list(b"ACGT"), size=int(n)).astype(np.uint8).tobytes() for n in lens]
+    s = ca.SequenceSet.from_strings(seqs)
+    L = s.lengths.astype(np.float64)
+
+    def lclass(r0, r1, c0, c1):
+        return 2 * np.log2(1 + (L[r0:r1 + 1].sum() + L[c0:c1 + 1].sum()) / (r1 - r0 + 1 + c1 - c0 + 1))
+
+    blocks = [(0, 2999, 6000, 6255), (3000, 5999, 6000, 6255)]
+    lc = [lclass(*b) for b in blocks]
+    assert round(lc[0]) != round(lc[1]) and abs(lc[0] - lc[1]) < 0.5, lc
+
+    def check(got, r0, r1, c0, c1, diag=False):
+        a = rng.integers(r0, r1 + 1, 20000)
+        b = rng.integers(c0, c1 + 1, 20000)
+        if diag:
+            keep = a != b
+            a, b = np.maximum(a, b)[keep], np.minimum(a, b)[keep]  # the lower triangle
+        want, _ = O.sw_pairs_mt(s.residues, s.offsets, a, b, ednafull, 16, 4, threads=THREADS)
+        assert np.array_equal(got[a - r0, b - c0], want)
+        if diag:
+            assert np.array_equal(got[b - r0, a - c0], want)  # and its mirror
+
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(s)
+    al.set_tune_min_cells(0)
+    for k, blk in enumerate(blocks):
+        got, _, _ = al.compute_block(*blk, False)
+        cfg = al.last_block_config()
+        assert cfg["tuning"] == 1 and cfg["trials"] == k + 1, cfg
+        check(got, *blk)
+    got, _, _ = al.compute_block(0, 1999, 0, 1999, True)
+    cfg = al.last_block_config()
+    assert cfg["tuning"] == 3 and cfg["trials"] == 2, cfg
+    assert cfg["variant"] in (ca.SCHED_PERSISTENT_I16, ca.SCHED_PERSISTENT_I32), cfg
+    check(got, 0, 1999, 0, 1999, diag=True)
+    al.close()
+    # a diagonal block first: its record's first trial
+    al = ca.Aligner(ednafull, 16, 4)
+    al.set_sequences(s)
+    al.set_tune_min_cells(0)
+    got, _, _ = al.compute_block(0, 1999, 0, 1999, True)
+    cfg = al.last_block_config()
+    assert cfg["tuning"] == 1 and cfg["trials"] == 1, cfg
+    check(got, 0, 1999, 0, 1999, diag=True)
+    al.close()
