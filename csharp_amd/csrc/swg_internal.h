@@ -105,14 +105,18 @@ constexpr uint32_t kCellTrack = 0x80000000u;
 // Sort of the block's pairs by first-maximum cell for the re-run: buckets of
 // wr x wc cells on a kBucketDim x kBucketDim grid, plus two lists, each
 // padded to whole 64-lane tiles: flagged pairs (the last bucket) and, in the
-// sort by re-run rectangle, rectangles of more than SortSet::big_rect cells
-// (the one before it), which one wave per pair re-runs: kBigRect for
-// throughput, kBigRectThin for thin blocks, where the longest re-run of one
-// lane bounds the latency.
+// sort by re-run rectangle, rectangles of more than SortSet::big_rect cells,
+// which one wave per pair re-runs: kBigRect for throughput, kBigRectThin for
+// thin blocks, where the longest re-run of one lane bounds the latency.  The
+// big list is kBigClasses buckets (from kBigBucket) by size, class c holding
+// rectangles of (2^c, 2^(c+1)] x big_rect cells (the last class the rest),
+// placed largest class first, so that the one-wave-per-pair queue starts the
+// longest rectangles first and ends with short ones.
 constexpr int kBucketDim = 128;
-constexpr int kNumBuckets = kBucketDim * kBucketDim + 2;
+constexpr int kBigClasses = 8;
+constexpr int kBigBucket = kBucketDim * kBucketDim;
+constexpr int kNumBuckets = kBucketDim * kBucketDim + kBigClasses + 1;
 constexpr int kFlaggedBucket = kNumBuckets - 1;
-constexpr int kBigBucket = kNumBuckets - 2;
 constexpr int64_t kBigRect = 65536;
 constexpr int64_t kBigRectThin = 8192;
 

@@ -1311,17 +1311,21 @@ __device__ __forceinline__ int pair_bucket(const SortSet& P, const BlockDesc& D1
     *cells = (uint64_t)h * (uint64_t)w;
     if (!P.start_cell)
         return score_bucket((int)P.best_score[pos]) * kBucketDim + side_bucket(min(bi, wend), max(D1.wr, D1.wc), false);
-    if ((int64_t)*cells > P.big_rect) return kBigBucket;
+    if ((int64_t)*cells > P.big_rect) {
+        // size class floor(log2((cells - 1) / big_rect)): (2^c, 2^(c+1)] x big_rect cells
+        const int c = 63 - __clzll((*cells - 1) / (uint64_t)P.big_rect);
+        return kBigBucket + min(c, kBigClasses - 1);
+    }
     return side_bucket(h, D1.wr, true) * kBucketDim + side_bucket(w, D1.wc, true);
 }
 
-// Slot order: rank 0 = the flagged list, rank 1 = the big rectangles, then
-// the serpentine.
+// Slot order: rank 0 = the flagged list, ranks 1 .. kBigClasses = the big
+// rectangles, largest class first, then the serpentine.
 __device__ __forceinline__ int bucket_at_rank(int q)
 {
     if (q == 0) return kFlaggedBucket;
-    if (q == 1) return kBigBucket;
-    const int rr = q - 2, band = rr / kBucketDim, pos = rr - band * kBucketDim;
+    if (q <= kBigClasses) return kBigBucket + kBigClasses - q;
+    const int rr = q - 1 - kBigClasses, band = rr / kBucketDim, pos = rr - band * kBucketDim;
     const int qi = kBucketDim - 1 - band;
     const int qj = (band & 1) ? pos : kBucketDim - 1 - pos;
     return qi * kBucketDim + qj;
@@ -1388,7 +1392,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
         const int k = pair_bucket(P, D1, (int64_t)row * D1.ld + c, P.len[D1.c0 + c], &cells, &fl);
         atomicAdd(&hist[k], 1);
         if (fl) flagged += cells;
-        else if (k == kBigBucket) big += cells;
+        else if (k >= kBigBucket && k < kBigBucket + kBigClasses) big += cells;
         else rect += cells;
     });
     __syncthreads();
@@ -1417,12 +1421,28 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(SortSet P)
     }
 }
 
-// Slots of rank q: the flagged and big lists are padded to whole tiles, the
-// buckets are not.
+// Pairs in the big list (all its size classes).
+__device__ __forceinline__ int big_count(const SortSet& P)
+{
+    int n = 0;
+#pragma unroll
+    for (int c = 0; c < kBigClasses; ++c) n += P.count[kBigBucket + c];
+    return n;
+}
+
+__device__ __forceinline__ int whole_tiles(int n) { return (n + kWave - 1) / kWave * kWave; }
+
+// Slots of rank q: the flagged list and the big list are padded to whole
+// tiles (the big list after its last, smallest, class), the buckets are not.
 __device__ __forceinline__ int rank_slots(const SortSet& P, int q)
 {
     const int n = P.count[bucket_at_rank(q)];
-    return q <= 1 ? (n + kWave - 1) / kWave * kWave : n;
+    if (q == 0) return whole_tiles(n);
+    if (q == kBigClasses) {
+        const int nb = big_count(P);
+        return n + whole_tiles(nb) - nb;
+    }
+    return n;
 }
 
 // One workgroup of kScanThreads: exclusive scan of the slot counts in rank
@@ -1470,7 +1490,7 @@ __global__ __launch_bounds__(kScanThreads) void sort_scan_kernel(SortSet P)
     if (tid == 0) {
         const int tf = (P.count[kFlaggedBucket] + kWave - 1) / kWave;
         P.meta[kMetaFlaggedTiles] = tf;
-        P.meta[kMetaBigTiles] = tf + (P.count[kBigBucket] + kWave - 1) / kWave;
+        P.meta[kMetaBigTiles] = tf + whole_tiles(big_count(P)) / kWave;
         P.meta[kMetaTiles] = (total + kWave - 1) / kWave;
         P.meta[kMetaSlots] = total;
     }
@@ -1488,8 +1508,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(SortSet P)
         const int nf = P.count[kFlaggedBucket];
         const int endf = (nf + kWave - 1) / kWave * kWave;
         if (nf + (int)threadIdx.x < endf) P.sorted[nf + threadIdx.x] = ~0u;
-        const int nb = P.base[kBigBucket] + P.count[kBigBucket];
-        const int endb = P.base[kBigBucket] + (P.count[kBigBucket] + kWave - 1) / kWave * kWave;
+        const int nbig = big_count(P);  // the big list starts where the flagged list's tiles end
+        const int nb = endf + nbig;
+        const int endb = endf + whole_tiles(nbig);
         if (nb + (int)threadIdx.x < endb) P.sorted[nb + threadIdx.x] = ~0u;
         const int n = P.meta[kMetaSlots];
         const int end = (n + kWave - 1) / kWave * kWave;
