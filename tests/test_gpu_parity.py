@@ -519,3 +519,45 @@ def test_kernel_timing_mode(dna_set, ednafull, schedule, names):
         else:
             assert run >= c
     al.close()
+
+
+@pytest.mark.parametrize("schedule", ["two_pass", "thin"])
+def test_big_rectangles_of_every_size_class(ednafull, schedule):
+    """Related pairs of 300 to 3,200 residues: re-run rectangles from under
+    131,072 to over 8.4e6 cells, i.e. in every size class of the big list
+    (queued largest class first), 441 related pairs (not a whole number of
+    64-pair tiles) beside unrelated ones; distances and statistics match the
+    oracle, on the regular two-pass schedule (big above 65,536 cells) and on
+    the thin-block one (big above 8,192 cells)."""
+    say("big rectangles of every size class, %s" % schedule)
+    rng = np.random.default_rng(41)
+    lens = np.linspace(300, 3200, 21).astype(int)
+    anc = rng.choice(list(b"ACGT"), size=int(lens.max()))
+    fam = []
+    for L in lens:
+        s = []
+        for ch in anc[:L]:
+            r = rng.random()
+            if r < 0.015:
+                continue
+            if r < 0.03:
+                s.append(rng.choice(list(b"ACGT")))
+            s.append(rng.choice(list(b"ACGT")) if rng.random() < 0.08 else ch)
+        fam.append(bytes(s))
+    other = [rng.choice(list(b"ACGT"), size=int(L)).astype(np.uint8).tobytes() for L in lens]
+    s = ca.SequenceSet.from_strings(fam + other)
+    assert min(len(x) for x in fam) ** 2 < 2 * 65536 and min(sorted(len(x) for x in fam)[-2:]) ** 2 > 128 * 65536
+    if schedule == "two_pass":
+        al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_I16)
+    else:
+        al = ca.Aligner(ednafull, 16, 4, schedule=ca.SCHED_PERSISTENT_LPS16)
+    al.set_sequences(s)
+    n = len(s)
+    al.set_kernel_timing(True)
+    got, gst, _ = al.compute_block(0, 20, 0, n - 1, False, stats=True)
+    kt = {k[0]: k for k in al.kernel_times()}
+    al.set_kernel_timing(False)
+    want, wst = oracle_block(s, 0, 20, 0, n - 1, False, ednafull, 16, 4)
+    assert_same(got, want, gst, wst, "big rectangles of every size class")
+    assert kt["rect64"][2] > 8e6
+    al.close()
